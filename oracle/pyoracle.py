@@ -1,0 +1,293 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check of
+__graft_entry__.py and by the cpu_baseline leg of bench.py, always as the
+checker (or the timed CPU baseline), never as the product path.
+
+Arrays are numpy complex128 in the reference's Armadillo column-major layout
+(3D field: Ny rows x Nx cols, element (r, c) at c*Ny + r; numpy arrays are
+passed as shape (Nx, Ny) C-order == (Ny, Nx) Fortran-order).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SHAPE_DTYPE = np.dtype([("type", np.int32), ("op", np.int32), ("T", np.float64, 16),
+                        ("p", np.float64, 3)])
+PART_DTYPE = np.dtype([("first_shape", np.int32), ("n_shapes", np.int32), ("op", np.int32),
+                       ("_pad", np.int32), ("delta", np.float64), ("beta", np.float64)])
+MODULE_DTYPE = np.dtype([("first_part", np.int32), ("n_parts", np.int32)])
+
+SPHERE, BOX, CYLINDER = 0, 1, 2
+UNION, DIFFERENCE = 0, 1
+FIX_ADI_Y_DIAG = 1
+FIX_REFR_SAMPLES = 2
+BC_DIRICHLET, BC_TRANSPARENT = 0, 1
+
+
+class Grid(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("xmin", "xmax", "dx", "ymin", "ymax", "dy", "zmin", "zmax", "dz", "k")]
+
+
+class Material(C.Structure):
+    _fields_ = [("fn", C.c_void_p), ("ctx", C.c_void_p)]
+
+
+class CSG(C.Structure):
+    _fields_ = [("shapes", C.c_void_p), ("parts", C.c_void_p), ("modules", C.c_void_p),
+                ("n_modules", C.c_int), ("is_reference_run", C.c_int)]
+
+
+class CurvedWG(C.Structure):
+    _fields_ = [("R", C.c_double), ("width", C.c_double), ("clad_delta", C.c_double),
+                ("clad_beta", C.c_double), ("in_delta", C.c_double), ("in_beta", C.c_double),
+                ("wglength", C.c_double), ("dx", C.c_double), ("dz", C.c_double),
+                ("use_smoothed", C.c_int)]
+
+
+class Slab(C.Structure):
+    _fields_ = [("x_lo", C.c_double), ("x_hi", C.c_double), ("core_delta", C.c_double),
+                ("core_beta", C.c_double), ("clad_delta", C.c_double), ("clad_beta", C.c_double)]
+
+
+class Absorber3D(C.Structure):
+    _fields_ = [("npix_x", C.c_uint), ("npix_y", C.c_uint), ("inv_x", C.c_double),
+                ("inv_y", C.c_double)]
+
+
+class Gauss(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("k", "amplitude", "z0", "waist", "cx", "cy", "angle_x_deg", "angle_y_deg")]
+
+
+class Plane(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("k", "amplitude", "z0", "angle_rad")]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_nodes.restype = C.c_uint
+        L.orc_nodes.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_fft1d.argtypes = [P, C.c_uint, C.c_int]
+        L.orc_fft2d.argtypes = [P, C.c_uint, C.c_uint, C.c_int]
+        L.orc_thomas.argtypes = [P, P, P, C.c_uint]
+        for f in ("orc_shape_translate",):
+            getattr(L, f).argtypes = [P, C.c_double, C.c_double, C.c_double]
+        L.orc_shape_rotate.argtypes = [P, C.c_double, C.c_int]
+        L.orc_shape_scale.argtypes = [P, C.c_double, C.c_int]
+        L.orc_mat4_identity.argtypes = [P]
+        L.orc_shape_transform.argtypes = [P, P, P, P]
+        L.orc_csg_eval.argtypes = [P, C.c_double, C.c_double, C.c_double, P, P]
+        L.orc_fft3d_run.argtypes = [P, P, P, C.c_uint, C.c_uint, P, C.c_uint, C.c_uint, C.c_uint,
+                                    P, C.c_uint, C.c_uint, C.c_uint]
+        L.orc_cube_store.argtypes = [P, C.c_uint, P, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+        L.orc_fft2d_run.argtypes = [P, P, P, C.c_uint, C.c_uint, C.c_uint, P, C.c_uint]
+        L.orc_cn_run.argtypes = [P, P, C.c_int, P, C.c_uint, C.c_uint, C.c_uint, P, C.c_uint,
+                                 C.c_uint, C.c_double]
+        L.orc_cn_step_planes.argtypes = [C.c_double, C.c_double, C.c_int, P, P, C.c_uint,
+                                         P, P, P, P]
+        L.orc_adi_run.argtypes = [P, P, C.c_int, C.c_uint, P, C.c_uint, C.c_uint, C.c_uint,
+                                  P, C.c_uint, C.c_uint, C.c_uint]
+        L.orc_gauss_2d_v.argtypes = [P, C.c_double, C.c_double, P]
+        L.orc_plane_wave_v.argtypes = [P, C.c_double, C.c_double, P]
+        L.orc_gauss_fill3d.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_uint, P]
+        L.orc_num_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def nodes(vmin, vmax, step):
+    return lib().orc_nodes(vmin, vmax, step)
+
+
+def fft1d(a, sign=-1):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    lib().orc_fft1d(_p(a), a.size, sign)
+    return a
+
+
+def fft2d_colmajor(a, sign=-1):
+    """2D DFT of a Fortran-ordered (nrows, ncols) complex matrix."""
+    a = np.asfortranarray(a, dtype=np.complex128).copy(order="F")
+    lib().orc_fft2d(_p(a), a.shape[0], a.shape[1], sign)
+    return a
+
+
+def thomas(diag, sub, rhs):
+    d = np.ascontiguousarray(diag, dtype=np.complex128).copy()
+    s = np.ascontiguousarray(sub, dtype=np.complex128).copy()
+    r = np.ascontiguousarray(rhs, dtype=np.complex128).copy()
+    lib().orc_thomas(_p(d), _p(s), _p(r), d.size)
+    return d
+
+
+# --------------------------------------------------------------------------- CSG
+def mat4_identity():
+    T = np.zeros(16)
+    lib().orc_mat4_identity(_p(T))
+    return T
+
+
+def translate(T, x, y, z):
+    lib().orc_shape_translate(_p(T), x, y, z)
+
+
+def rotate(T, deg, axis):
+    lib().orc_shape_rotate(_p(T), deg, axis)
+
+
+def scale(T, f, axis):
+    lib().orc_shape_scale(_p(T), f, axis)
+
+
+def transform(T, x, y, z):
+    xs, ys, zs = (C.c_double(v) for v in (x, y, z))
+    lib().orc_shape_transform(_p(T), C.byref(xs), C.byref(ys), C.byref(zs))
+    return xs.value, ys.value, zs.value
+
+
+class CSGMaterial:
+    """Holds the flat CSG arrays alive and exposes an orc_material."""
+
+    def __init__(self, shapes, parts, modules, reference_run=False):
+        self.shapes = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)
+        self.parts = np.ascontiguousarray(parts, dtype=PART_DTYPE)
+        self.modules = np.ascontiguousarray(modules, dtype=MODULE_DTYPE)
+        self.csg = CSG(_p(self.shapes), _p(self.parts), _p(self.modules), len(self.modules),
+                       int(reference_run))
+        fn = C.cast(lib().orc_csg_matfn, C.c_void_p)
+        self.mat = Material(fn, C.cast(C.byref(self.csg), C.c_void_p))
+
+    def eval(self, x, y, z):
+        d, b = C.c_double(), C.c_double()
+        lib().orc_csg_eval(C.byref(self.csg), x, y, z, C.byref(d), C.byref(b))
+        return d.value, b.value
+
+
+class CurvedMaterial:
+    def __init__(self, **kw):
+        defaults = dict(in_delta=0.0, in_beta=0.0, wglength=1e10, use_smoothed=0)
+        defaults.update(kw)
+        self.wg = CurvedWG(**defaults)
+        self.mat = Material(C.cast(lib().orc_curved_matfn, C.c_void_p),
+                            C.cast(C.byref(self.wg), C.c_void_p))
+
+
+class SlabMaterial:
+    def __init__(self, x_lo, x_hi, core_delta, core_beta, clad_delta, clad_beta):
+        self.slab = Slab(x_lo, x_hi, core_delta, core_beta, clad_delta, clad_beta)
+        self.mat = Material(C.cast(lib().orc_slab_matfn, C.c_void_p),
+                            C.cast(C.byref(self.slab), C.c_void_p))
+
+
+class NoMaterial:
+    mat = Material(None, None)
+
+
+def grid(xmin, xmax, dx, zmin, zmax, dz, k, ymin=None, ymax=None, dy=None):
+    if ymin is None:
+        ymin, ymax, dy = xmin, xmax, dx
+    return Grid(xmin, xmax, dx, ymin, ymax, dy, zmin, zmax, dz, k)
+
+
+# ---------------------------------------------------------------------- sources
+def gauss3d_field(g, gb, n):
+    """ParaxialSimulation::setBoundaryConditions 3D: values(j,i) = src(x_i, y_j, 0)."""
+    out = np.empty((n, n), dtype=np.complex128, order="F")  # (rows=y, cols=x)
+    lib().orc_gauss_fill3d(C.byref(gb), g.xmin, g.dx, g.ymin, g.dy, n, _p(out))
+    return out
+
+
+def gauss2d_value(gb, x, z=0.0):
+    v = np.zeros(2)
+    lib().orc_gauss_2d_v(C.byref(gb), x, z, _p(v))
+    return complex(v[0], v[1])
+
+
+def plane_value(pw, x, z=0.0):
+    v = np.zeros(2)
+    lib().orc_plane_wave_v(C.byref(pw), x, z, _p(v))
+    return complex(v[0], v[1])
+
+
+# ---------------------------------------------------------------------- solvers
+def fft3d_run(g, material, field, first_step, nsteps, absorber=None, n_refr=10, quirks=0,
+              cube=None, dsz=1):
+    f = np.asfortranarray(field, dtype=np.complex128).copy(order="F")
+    n = f.shape[0]
+    ab = absorber if absorber is not None else Absorber3D(0, 0, 0.0, 0.0)
+    cube_p, cn, cs = (None, 0, 0) if cube is None else (_p(cube), cube.shape[0], cube.shape[2])
+    lib().orc_fft3d_run(C.byref(g), C.byref(material.mat), C.byref(ab), n_refr, quirks, _p(f), n,
+                        first_step, nsteps, cube_p, cn, cs, dsz)
+    return f
+
+
+def fft2d_run(g, material, field, first_step, nsteps, solution=None):
+    f = np.ascontiguousarray(field, dtype=np.complex128).copy()
+    sp, nz = (None, 0) if solution is None else (_p(solution), solution.shape[1])
+    lib().orc_fft2d_run(C.byref(g), C.byref(material.mat), _p(f), f.size, first_step, nsteps,
+                        sp, nz)
+    return f
+
+
+def cn_run(g, material, field, first_step, nsteps, bc=BC_TRANSPARENT, solution=None,
+           absorb_npix=0, absorb_inv=0.0):
+    f = np.ascontiguousarray(field, dtype=np.complex128).copy()
+    sp, nz = (None, 0) if solution is None else (_p(solution), solution.shape[1])
+    lib().orc_cn_run(C.byref(g), C.byref(material.mat), bc, _p(f), f.size, first_step, nsteps,
+                     sp, nz, absorb_npix, absorb_inv)
+    return f
+
+
+def cn_step_planes(rho, r, bc, prev, d1, b1, d0, b0):
+    prev = np.ascontiguousarray(prev, dtype=np.complex128)
+    cur = np.empty_like(prev)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (d1, b1, d0, b0)]
+    lib().orc_cn_step_planes(rho, r, bc, _p(prev), _p(cur), prev.size, *[_p(a) for a in arrs])
+    return cur
+
+
+def adi_run(g, material, field, first_step, nsteps, use_tbc=True, quirks=0, cube=None, dsz=1):
+    f = np.asfortranarray(field, dtype=np.complex128).copy(order="F")
+    n = f.shape[0]
+    cube_p, cn, cs = (None, 0, 0) if cube is None else (_p(cube), cube.shape[0], cube.shape[2])
+    lib().orc_adi_run(C.byref(g), C.byref(material.mat), int(use_tbc), quirks, _p(f), n,
+                      first_step, nsteps, cube_p, cn, cs, dsz)
+    return f
+
+
+def cube_store(field, cube, dsz, step):
+    f = np.asfortranarray(field, dtype=np.complex128)
+    lib().orc_cube_store(_p(f), f.shape[0], _p(cube), cube.shape[0], cube.shape[2], dsz, step)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def rel_l2(a, ref):
+    a = np.asarray(a).ravel(order="K")
+    ref = np.asarray(ref).ravel(order="K")
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+
+
+def rel_l2_intensity(a, ref):
+    ia = np.abs(np.asarray(a)) ** 2
+    ir = np.abs(np.asarray(ref)) ** 2
+    return float(np.linalg.norm(ia - ir) / np.linalg.norm(ir))
