@@ -1,0 +1,191 @@
+/*
+ * paxgpu.h -- C ABI of the MI355X (gfx950) propagation hot path of PaxPro.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = PXG_OK), no exceptions across the boundary, no torch types.  The
+ * reference exposes this path as a C++ virtual class API; the C++ drop-in
+ * classes in paxpro_amd/cpp (Solver, Solver2D, Solver3D, FFTSolver2D,
+ * FFTSolver3D, CrankNicholson, ADI, ThomasAlgorithm) call these entry points
+ * from their solveStep()/solve()/step() overrides.  Each entry point names the
+ * reference interface it replaces.
+ *
+ * Data layout follows Armadillo (column-major, interleaved complex<double>),
+ * so cx_vec/cx_mat memptr() maps 1:1:
+ *   2D field : Nx complex values.
+ *   3D field : Ny x Nx matrix (Ny == Nx; include/solver3D.hpp:56, src/solver3D.cpp:24),
+ *              element (row r = y, col c = x) at c*Ny + r.
+ * Host buffers are caller-owned and always complex128; device buffers are
+ * library-owned and hold complex128 (PXG_PREC_FP64) or complex64
+ * (PXG_PREC_MIXED) fields.
+ *
+ * Threading: a ctx may be shared by threads (the server does); a plan is used
+ * by one thread at a time and owns one HIP stream (default priority).
+ */
+#ifndef PAXGPU_H
+#define PAXGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  PXG_OK = 0,
+  PXG_ERR_INVALID = 1,      /* bad argument / shape mismatch (reference: std::runtime_error) */
+  PXG_ERR_UNSUPPORTED = 2,  /* size or option not implemented on the GPU path */
+  PXG_ERR_BUDGET = 3,       /* HBM budget of the context exceeded */
+  PXG_ERR_DEVICE = 4,       /* HIP runtime error (message in pxg_last_error) */
+  PXG_ERR_STATE = 5,        /* call order violated (e.g. step before set_field) */
+  PXG_ERR_CANCELLED = 6,    /* server job cancelled */
+  PXG_ERR_INJECTED = 7      /* failure forced from the host for testing */
+};
+
+typedef struct pxg_ctx_s *pxg_ctx;
+typedef struct pxg_plan_s *pxg_plan;
+
+/* ---- solver kinds and options --------------------------------------------- */
+enum { PXG_FFT2D = 0, PXG_FFT3D = 1, PXG_CN = 2, PXG_ADI = 3 };
+/* FP64: complex128 fields and FP64 arithmetic throughout (the plain path).
+ * MIXED: complex64 fields and FFT/stencil arithmetic; FP64 phase arguments
+ * (reduced mod 2*pi before rounding) and FP64 tridiagonal sweeps. */
+enum { PXG_PREC_FP64 = 0, PXG_PREC_MIXED = 1 };
+enum { PXG_BC_DIRICHLET = 0, PXG_BC_TRANSPARENT = 1 };   /* Solver2D::BC_t */
+/* Quirk switches: 0 reproduces the reference; a set bit selects the physics form. */
+enum {
+  PXG_FIX_ADI_Y_DIAG = 1u << 0,    /* src/alternatingDirectionSolver.cpp:103 parenthesis */
+  PXG_FIX_REFR_SAMPLES = 1u << 1   /* src/fftSolver3D.cpp:262 skipped trapezoid sample */
+};
+
+typedef struct {
+  int kind;              /* PXG_FFT2D / PXG_FFT3D / PXG_CN / PXG_ADI */
+  int precision;         /* PXG_PREC_* */
+  int nx;                /* transverse nodes; 3D grids are nx x nx */
+  int nz;                /* longitudinal nodes (steps run 1..nz-1) */
+  int batch;             /* independent fields sharing this geometry (>= 1) */
+  double xmin, dx;       /* transverse discretisation */
+  double ymin, dy;       /* vertical discretisation (3D) */
+  double zmin, dz;       /* longitudinal discretisation */
+  double k;              /* wavenumber (nm^-1); per-field values via pxg_set_wavenumbers */
+  int bc;                /* CN boundary condition, PXG_BC_* */
+  int use_tbc;           /* ADI::useTBC */
+  unsigned absorb_npix_x, absorb_npix_y;   /* FFTSolver3D::absorbingBC (0 = off); CN uses x */
+  double absorb_inv_x, absorb_inv_y;       /* inverse damping length in 1/pixel */
+  unsigned quirks;       /* PXG_FIX_* */
+  unsigned n_refr;       /* samples of the border-crossing integral (reference: 10) */
+  int store_dsx;         /* solution storage: transverse block size (0 = none) */
+  int store_dsz;         /* solution storage: every store_dsz-th step */
+} pxg_problem;
+
+/* ---- CSG material (layout shared with the oracle) -------------------------- */
+enum { PXG_SHAPE_SPHERE = 0, PXG_SHAPE_BOX = 1, PXG_SHAPE_CYLINDER = 2 };
+enum { PXG_OP_UNION = 0, PXG_OP_DIFFERENCE = 1 };
+typedef struct {
+  int type, op;          /* PXG_SHAPE_*, PXG_OP_* (operation inside its part) */
+  double T[16];          /* 4x4 world->shape transform, row-major */
+  double p[3];           /* sphere r | box Lx,Ly,Lz | cylinder r1,r2,height */
+} pxg_shape;
+typedef struct {
+  int first_shape, n_shapes, op, pad_;
+  double delta, beta;
+} pxg_part;
+typedef struct {
+  int first_part, n_parts;
+} pxg_module;
+
+typedef struct {
+  double R, width;
+  double clad_delta, clad_beta, in_delta, in_beta;
+  double wglength;
+  double dx, dz;
+  int use_smoothed;
+} pxg_curved_wg;
+
+/* ---- context ---------------------------------------------------------------- */
+/* Replaces nothing in the reference (it has no device): owns the device choice,
+ * the HBM budget used for admission, and the last error string. */
+int pxg_ctx_create(int device, size_t hbm_budget_bytes, pxg_ctx *out);
+int pxg_ctx_destroy(pxg_ctx ctx);
+/* Bytes currently reserved by live plans of this context. */
+size_t pxg_ctx_reserved(pxg_ctx ctx);
+/* Message of the last failing call on this thread (valid until the next call). */
+const char *pxg_last_error(void);
+/* Library build information (gfx target, kernels present). */
+const char *pxg_version(void);
+
+/* ---- plans ------------------------------------------------------------------ */
+/* Replaces Solver2D::setSimulator (src/solver2D.cpp:21-42) and
+ * Solver3D::setSimulator (src/solver3D.cpp:19-44): allocates the device fields
+ * and solution storage for the problem. */
+int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out);
+int pxg_plan_destroy(pxg_plan plan);
+/* Bytes of device memory the plan reserves (for admission by HBM budget). */
+int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes);
+/* Per-field wavenumbers of a batched plan (photon energies); n == batch. */
+int pxg_set_wavenumbers(pxg_plan plan, const double *k, int n);
+
+/* Replaces Solver2D::setInitialConditions / setLeftBC (src/solver2D.cpp:51-69) and
+ * Solver3D::setInitialConditions (src/solver3D.cpp:127-146); stores slice/column 0
+ * of the solution like copyCurrentSolution(0). */
+int pxg_set_field(pxg_plan plan, int b, const double *c128);
+/* Replaces getLastSolution / getLastSolution3D (include/solver2D.hpp:69,
+ * include/solver3D.hpp:36): copies the current field of batch entry b. */
+int pxg_get_field(pxg_plan plan, int b, double *c128);
+/* Replaces getSolution (2D: Nx x Nz matrix, include/solver2D.hpp:41) and
+ * getSolution3D (cube of (nx/dsx)^2 x (nz/dsz + 1), src/solver3D.cpp:201-204). */
+int pxg_get_solution(pxg_plan plan, int b, double *c128, size_t n_complex);
+int pxg_solution_dims(pxg_plan plan, int dims[3]);
+
+/* ---- materials (the per-cell getXrayMatProp call-back, evaluated on device) -- */
+/* Replaces CSGMaterial::getXrayMatProp (src/materialFunction.cpp:3-13,
+ * src/geometry.cpp:91-108,212-234, src/shape.cpp:126-187). */
+int pxg_set_material_csg(pxg_plan plan, const pxg_shape *shapes, int n_shapes,
+                         const pxg_part *parts, int n_parts, const pxg_module *modules,
+                         int n_modules, int is_reference_run);
+/* Replaces CurvedWaveGuideFD / WaveGuideFDSimulation::getXrayMatProp
+ * (src/waveGuideFDSimulation.cpp:219-265, src/curvedWaveGuide2D.cpp:49-52,179-205). */
+int pxg_set_material_curved_wg(pxg_plan plan, const pxg_curved_wg *wg);
+/* Host-sampled material for any MaterialFunction subclass the device cannot
+ * evaluate.  db holds n_planes planes of (delta, beta) pairs per cell, in the
+ * field's layout.  Plane p is what the solver consumes as its material input:
+ *   CN    : plane p = material at z = zmin + p*dz (step iz uses p = iz, iz-1);
+ *   FFT2D : plane p = material at z_p + dz/2 for step p;
+ *   FFT3D : plane p = effective (delta, beta) of step p after the crossing integral;
+ *   ADI   : plane 2p = x-pass of step p (z_p), plane 2p+1 = y-pass (z_p + dz/2).
+ * z_invariant != 0 means one plane serves every step. */
+int pxg_set_material_planes(pxg_plan plan, const double *db, int n_planes, int first_plane,
+                            int z_invariant);
+int pxg_set_material_none(pxg_plan plan);
+
+/* ---- stepping ----------------------------------------------------------------- */
+/* Replaces Solver2D::step/solve (src/solver2D.cpp:224-250) and Solver3D::step/solve
+ * (src/solver3D.cpp:148-199): advances every field of the plan through steps
+ * first_step .. first_step+nsteps-1, storing solution slices as configured.
+ * Enqueued on the plan's stream; returns after enqueue (use pxg_sync). */
+int pxg_step(pxg_plan plan, unsigned first_step, unsigned nsteps);
+int pxg_sync(pxg_plan plan);
+/* Same as pxg_step, bracketing every kernel launch with HIP events on the plan's
+ * stream; reports the mean duration per kernel name (ms) and the wall time of
+ * the whole enqueue-to-completion interval.  names[i] point to static strings. */
+typedef struct { const char *name; double mean_ms; int launches; double bytes_per_launch; } pxg_kernel_stat;
+int pxg_step_profiled(pxg_plan plan, unsigned first_step, unsigned nsteps, pxg_kernel_stat *stats,
+                      int max_stats, int *n_stats, double *total_ms);
+/* Device pointer of field b (advanced use: zero-copy serving, benchmarks). */
+int pxg_field_device_ptr(pxg_plan plan, int b, void **dptr);
+
+/* ---- batched tridiagonal solve ---------------------------------------------- */
+/* Replaces ThomasAlgorithm::solve<cdouble> (include/thomasAlgorithm.tpp:2-32) for
+ * many independent systems: diag/sub/rhs are host complex128 arrays of nsys
+ * systems of n rows (sub has n-1 entries per system, sup == sub); layout 0 =
+ * system-contiguous, 1 = interleaved (row i of system s at i*nsys + s).  The
+ * solution is written to diag; rhs is clobbered, as in the reference. */
+enum { PXG_LAYOUT_CONTIGUOUS = 0, PXG_LAYOUT_INTERLEAVED = 1 };
+int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
+                       double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1043 @@
+// C-ABI layer of libpaxgpu: contexts, plans, device memory, material upload and
+// the z-step drivers that enqueue the kernels of pxg_fft3d.hip, pxg_cn.hip and
+// pxg_adi.hip.  No exception crosses the boundary: every entry point catches,
+// records a thread-local message and returns a PXG_ERR_* code.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/paxgpu.h"
+#include "pxg_internal.hpp"
+#include "pxg_tridiag.hpp"
+
+using namespace pxg;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Failure {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Failure{code, buf};
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+  if (e != hipSuccess) fail(PXG_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f)
+{
+  try {
+    f();
+    g_err.clear();
+    return PXG_OK;
+  } catch (const Failure &e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return PXG_ERR_BUDGET;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return PXG_ERR_INVALID;
+  }
+}
+
+const double PI = 3.14159265358979323846264338327950288;
+
+// Device buffer owned by a plan.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t n)
+  {
+    release();
+    if (n == 0) return;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+  ~DBuf() { release(); }
+};
+
+}  // namespace
+
+struct pxg_ctx_s {
+  int device = 0;
+  size_t budget = 0;
+  std::atomic<size_t> reserved{0};
+  std::mutex mu;
+};
+
+struct pxg_plan_s {
+  pxg_ctx ctx = nullptr;
+  pxg_problem p{};
+  hipStream_t stream = nullptr;
+  bool mixed = false;
+  int esz = 16;           // bytes per complex field element
+  size_t cells = 0;       // per field
+  size_t reserved = 0;
+  int lines_x = 1, lines_y = 1;
+  DBuf field, tw, kx, ky, absx, absy, kvec;
+  std::vector<double> k_host;
+  // material
+  int mat_mode = MAT_NONE;
+  DBuf shapes, parts, boxes, part_fac, planes;
+  int n_parts = 0, is_ref = 0;
+  std::vector<DPart> parts_host;
+  int n_planes = 0, first_plane = 0, z_invariant = 0;
+  DCurved wg{};
+  // solution storage
+  DBuf sol;
+  int sol_dims[3] = {0, 0, 0};
+  size_t sol_fstride = 0;  // elements per field
+  // tridiagonal scratch (CN, ADI)
+  DBuf scratch;
+  bool field_set = false;
+  // profiling
+  std::vector<hipEvent_t> ev;
+  struct Rec {
+    const char *name;
+    int e0, e1;
+    double bytes;
+  };
+  std::vector<Rec> recs;
+  bool profiling = false;
+};
+
+namespace {
+
+// -------------------------------------------------------------- bookkeeping
+size_t table_bytes(const pxg_problem &p, int esz)
+{
+  const size_t n = p.nx;
+  size_t b = n * esz * 3 + 2 * n * 8 + p.batch * 8 + 2 * n * p.batch * esz;
+  return b + 4096;
+}
+
+size_t plan_bytes(const pxg_problem &p)
+{
+  const int esz = p.precision == PXG_PREC_MIXED ? 8 : 16;
+  const size_t n = p.nx;
+  const bool is3d = p.kind == PXG_FFT3D || p.kind == PXG_ADI;
+  const size_t cells = is3d ? n * n : n;
+  size_t b = cells * p.batch * esz;
+  b += table_bytes(p, esz);
+  if (p.store_dsz > 0) {
+    if (is3d) {
+      const size_t cn = n / std::max(1, p.store_dsx);
+      const size_t slices = p.nz / p.store_dsz + 1;
+      b += cn * cn * slices * 16 * p.batch;
+    } else {
+      b += cells * p.nz * esz * p.batch;
+    }
+  }
+  if (p.kind == PXG_ADI) b += cells * p.batch * 16 * 4;   // coefficient planes
+  if (p.kind == PXG_CN) b += cells * p.batch * 16 * 6;
+  return b;
+}
+
+void validate(const pxg_problem &p)
+{
+  if (p.kind < PXG_FFT2D || p.kind > PXG_ADI) fail(PXG_ERR_INVALID, "unknown solver kind %d", p.kind);
+  if (p.precision != PXG_PREC_FP64 && p.precision != PXG_PREC_MIXED)
+    fail(PXG_ERR_INVALID, "unknown precision %d", p.precision);
+  if (p.nx < 2) fail(PXG_ERR_INVALID, "nx must be >= 2 (got %d)", p.nx);
+  if (p.nz < 1) fail(PXG_ERR_INVALID, "nz must be >= 1");
+  if (p.batch < 1) fail(PXG_ERR_INVALID, "batch must be >= 1");
+  if (!(p.dx > 0) || !(p.dz > 0) || !(p.k > 0)) fail(PXG_ERR_INVALID, "dx, dz and k must be > 0");
+  if ((p.kind == PXG_FFT3D || p.kind == PXG_FFT2D) && !fft_size_supported(p.nx))
+    fail(PXG_ERR_UNSUPPORTED, "FFT solvers need a power-of-two nx in [16, 4096] (got %d)", p.nx);
+  if ((p.kind == PXG_FFT3D || p.kind == PXG_ADI) && !(p.dy > 0)) fail(PXG_ERR_INVALID, "dy must be > 0");
+  if (p.store_dsz < 0 || p.store_dsx < 0) fail(PXG_ERR_INVALID, "negative storage ratio");
+  if (p.store_dsz > 0 && p.store_dsx < 1) fail(PXG_ERR_INVALID, "store_dsx must be >= 1");
+  if ((p.kind == PXG_FFT2D || p.kind == PXG_CN) && p.store_dsz > 0 && p.store_dsx != 1)
+    fail(PXG_ERR_UNSUPPORTED,
+         "2D transverse downsampling needs the VISA low-pass filter, which is not implemented");
+  if ((p.kind == PXG_FFT3D || p.kind == PXG_ADI) && p.store_dsz > 0 && p.nx / p.store_dsx < 1)
+    fail(PXG_ERR_INVALID, "storage downsampling larger than the grid");
+}
+
+template <class R>
+void upload_cx(DBuf &dst, const std::vector<cx<double>> &src, hipStream_t s)
+{
+  std::vector<cx<R>> tmp(src.size());
+  for (size_t i = 0; i < src.size(); ++i) tmp[i] = cvt<R>(src[i]);
+  dst.alloc(tmp.size() * sizeof(cx<R>));
+  hip_check(hipMemcpyAsync(dst.p, tmp.data(), dst.bytes, hipMemcpyHostToDevice, s), "upload");
+  hip_check(hipStreamSynchronize(s), "upload sync");
+}
+
+template <class R>
+void upload_real(DBuf &dst, const std::vector<double> &src, hipStream_t s)
+{
+  std::vector<R> tmp(src.begin(), src.end());
+  dst.alloc(tmp.size() * sizeof(R));
+  hip_check(hipMemcpyAsync(dst.p, tmp.data(), dst.bytes, hipMemcpyHostToDevice, s), "upload");
+  hip_check(hipStreamSynchronize(s), "upload sync");
+}
+
+// Absorber::apply along one line of length n (src/absorber.cpp:4-12) applied to
+// a line of ones: the per-position factor table.
+std::vector<double> absorber_table(unsigned n, unsigned npix, double inv)
+{
+  std::vector<double> f(n, 1.0);
+  for (unsigned i = 0; i < npix && i < n; ++i) {
+    const double fac = std::exp(-static_cast<double>(npix - i - 1) * inv);
+    f[i] *= fac;
+    f[n - i - 1] *= fac;
+  }
+  return f;
+}
+
+// FFTSolver3D::spatialFreqX/Y (src/fftSolver3D.cpp:31-50).
+double spatial_freq(unsigned idx, unsigned size, double step)
+{
+  if (idx > size / 2) idx = size - idx;
+  return 2.0 * PI * idx / (step * size);
+}
+
+// exp(-i dz/(2k) q^2), the half-kernel of FFTSolver3D::kernel (:24-29).
+cx<double> kernel_half(double q, double dz, double k)
+{
+  const double ph = -(0.5 * dz / k) * (q * q);
+  return {std::cos(ph), std::sin(ph)};
+}
+
+// exp(-k (beta + i delta) dz) * inv_norm in FP64.
+cx<double> screen_factor(double k, double dz, double delta, double beta, double inv_norm)
+{
+  const double amp = std::exp((-k * beta) * dz);
+  const double ph = (-k * delta) * dz;
+  return {std::cos(ph) * amp * inv_norm, std::sin(ph) * amp * inv_norm};
+}
+
+template <class R>
+void build_tables(pxg_plan P)
+{
+  const pxg_problem &p = P->p;
+  const unsigned n = p.nx;
+  hipStream_t s = P->stream;
+  if (p.kind == PXG_FFT2D || p.kind == PXG_FFT3D) {
+    std::vector<cx<double>> tw(n);
+    for (unsigned m = 0; m < n; ++m) {
+      const double a = 2.0 * PI * (static_cast<double>(m) / n);
+      tw[m] = {std::cos(a), -std::sin(a)};
+    }
+    upload_cx<R>(P->tw, tw, s);
+    std::vector<cx<double>> kx(static_cast<size_t>(n) * p.batch), ky(kx.size());
+    for (int b = 0; b < p.batch; ++b)
+      for (unsigned i = 0; i < n; ++i) {
+        kx[b * n + i] = kernel_half(spatial_freq(i, n, p.dx), p.dz, P->k_host[b]);
+        ky[b * n + i] = kernel_half(spatial_freq(i, n, p.kind == PXG_FFT3D ? p.dy : p.dx), p.dz,
+                                    P->k_host[b]);
+      }
+    upload_cx<R>(P->kx, kx, s);
+    upload_cx<R>(P->ky, ky, s);
+    if (p.kind == PXG_FFT3D && (p.absorb_npix_x || p.absorb_npix_y)) {
+      upload_real<R>(P->absx, absorber_table(n, p.absorb_npix_x, p.absorb_inv_x), s);
+      upload_real<R>(P->absy, absorber_table(n, p.absorb_npix_y, p.absorb_inv_y), s);
+    } else {
+      P->absx.release();
+      P->absy.release();
+    }
+  }
+  P->kvec.alloc(sizeof(double) * p.batch);
+  hip_check(hipMemcpyAsync(P->kvec.p, P->k_host.data(), P->kvec.bytes, hipMemcpyHostToDevice, s),
+            "upload k");
+  hip_check(hipStreamSynchronize(s), "upload sync");
+}
+
+template <class R>
+void build_part_factors(pxg_plan P)
+{
+  const pxg_problem &p = P->p;
+  const double inv_norm = 1.0 / static_cast<double>(P->cells);
+  const int np = P->n_parts;
+  std::vector<cx<double>> pf(static_cast<size_t>(p.batch) * (np + 1));
+  for (int b = 0; b < p.batch; ++b) {
+    for (int e = 0; e < np; ++e) {
+      const DPart &q = P->parts_host[e];
+      const double d = q.op == 0 ? q.delta : 0.0, be = q.op == 0 ? q.beta : 0.0;
+      pf[b * (np + 1) + e] = screen_factor(P->k_host[b], p.dz, d, be, inv_norm);
+    }
+    pf[b * (np + 1) + np] = screen_factor(P->k_host[b], p.dz, 0.0, 0.0, inv_norm);
+  }
+  upload_cx<R>(P->part_fac, pf, P->stream);
+}
+
+void rebuild_tables(pxg_plan P)
+{
+  if (P->mixed) {
+    build_tables<float>(P);
+    if (P->mat_mode == MAT_CSG) build_part_factors<float>(P);
+  } else {
+    build_tables<double>(P);
+    if (P->mat_mode == MAT_CSG) build_part_factors<double>(P);
+  }
+}
+
+MatDev mat_dev(pxg_plan P)
+{
+  MatDev m{};
+  m.mode = P->mat_mode;
+  m.shapes = P->shapes.as<DShape>();
+  m.parts = P->parts.as<DPart>();
+  m.boxes = P->boxes.as<DBox>();
+  m.n_parts = P->n_parts;
+  m.is_ref = P->is_ref;
+  m.part_fac = P->part_fac.p;
+  m.planes = P->planes.as<double>();
+  m.n_planes = P->n_planes;
+  m.first_plane = P->first_plane;
+  m.z_invariant = P->z_invariant;
+  m.plane_stride = 2 * P->cells;
+  m.wg = P->wg;
+  return m;
+}
+
+// ------------------------------------------------------------------ launches
+void record(pxg_plan P, const char *name, double bytes, hipError_t e, int ev0)
+{
+  hip_check(e, name);
+  if (P->profiling) {
+    const int ev1 = static_cast<int>(P->recs.size()) * 2 + 1;
+    hip_check(hipEventRecord(P->ev[ev1], P->stream), "event");
+    P->recs.push_back({name, ev0, ev1, bytes});
+  }
+}
+
+int begin(pxg_plan P)
+{
+  if (!P->profiling) return -1;
+  const int ev0 = static_cast<int>(P->recs.size()) * 2;
+  if (static_cast<size_t>(ev0 + 2) > P->ev.size()) fail(PXG_ERR_STATE, "event pool exhausted");
+  hip_check(hipEventRecord(P->ev[ev0], P->stream), "event");
+  return ev0;
+}
+
+template <class R>
+FftArgs<R> fft_args(pxg_plan P)
+{
+  const pxg_problem &p = P->p;
+  FftArgs<R> a{};
+  a.field = P->field.as<cx<R>>();
+  a.fstride = P->cells;
+  a.tw = P->tw.as<cx<R>>();
+  a.kx = P->kx.as<cx<R>>();
+  a.ky = P->ky.as<cx<R>>();
+  a.absx = P->absx.as<R>();
+  a.absy = P->absy.as<R>();
+  a.kvec = P->kvec.as<double>();
+  a.mat = mat_dev(P);
+  a.xmin = p.xmin;
+  a.dx = p.dx;
+  a.ymin = p.ymin;
+  a.dy = p.dy;
+  a.zmin = p.zmin;
+  a.dz = p.dz;
+  a.inv_norm = 1.0 / static_cast<double>(P->cells);
+  a.n_refr = p.n_refr ? p.n_refr : 10;
+  a.quirks = p.quirks;
+  return a;
+}
+
+bool store_step(pxg_plan P, unsigned st)
+{
+  const pxg_problem &p = P->p;
+  return p.store_dsz > 0 && st % p.store_dsz == 0 &&
+         static_cast<int>(st / p.store_dsz) < P->sol_dims[2];
+}
+
+template <class R>
+void cube_store(pxg_plan P, unsigned st)
+{
+  const double bytes = static_cast<double>(P->cells) * P->p.batch * sizeof(cx<R>);
+  const int ev0 = begin(P);
+  record(P, "cube_store", bytes,
+         launch_cube_store<R>(P->field.as<cx<R>>(), P->cells, P->p.nx, P->sol_dims[0],
+                              P->sol.as<cx<double>>(), P->sol_fstride,
+                              static_cast<int>(st / P->p.store_dsz), P->p.batch, P->stream),
+         ev0);
+}
+
+template <class R>
+void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  const pxg_problem &p = P->p;
+  FftArgs<R> a = fft_args<R>(P);
+  const double pass_bytes = 2.0 * P->cells * p.batch * sizeof(cx<R>);
+  const int n = p.nx;
+  a.mode = Y_FWD;
+  int ev0 = begin(P);
+  record(P, "pass_y_fwd", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
+  for (unsigned st = first; st < first + nsteps; ++st) {
+    const bool last = st + 1 == first + nsteps;
+    const bool store = store_step(P, st);
+    ev0 = begin(P);
+    record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
+    a.step = st;
+    a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
+    ev0 = begin(P);
+    record(P, "pass_y", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
+    if (store) {
+      cube_store<R>(P, st);
+      if (!last) {
+        a.mode = Y_FWD;
+        ev0 = begin(P);
+        record(P, "pass_y_fwd", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream),
+               ev0);
+      }
+    }
+  }
+}
+
+template <class R>
+void run_fft2d(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  const pxg_problem &p = P->p;
+  FftArgs<R> a = fft_args<R>(P);
+  const double pass_bytes = 2.0 * P->cells * p.batch * sizeof(cx<R>);
+  for (unsigned st = first; st < first + nsteps; ++st) {
+    a.step = st;
+    a.sol = (p.store_dsz > 0 && static_cast<int>(st) < p.nz) ? P->sol.as<cx<R>>() : nullptr;
+    a.sol_fstride = P->sol_fstride;
+    const int ev0 = begin(P);
+    record(P, "fft2d_step", pass_bytes, launch_fft2d_step<R>(p.nx, a, p.batch, P->stream), ev0);
+  }
+}
+
+template <class R>
+TriArgs<R> tri_args(pxg_plan P)
+{
+  const pxg_problem &p = P->p;
+  TriArgs<R> a{};
+  a.field = P->field.as<cx<R>>();
+  a.n = p.nx;
+  a.batch = p.batch;
+  a.kvec = P->kvec.as<double>();
+  a.mat = mat_dev(P);
+  a.xmin = p.xmin;
+  a.dx = p.dx;
+  a.ymin = p.ymin;
+  a.dy = p.dy;
+  a.zmin = p.zmin;
+  a.dz = p.dz;
+  a.bc = p.bc;
+  a.use_tbc = p.use_tbc;
+  a.quirks = p.quirks;
+  a.absorb_npix = p.absorb_npix_x;
+  a.absorb_inv = p.absorb_inv_x;
+  a.scratch = P->scratch.p;
+  a.sol = p.store_dsz > 0 ? P->sol.as<cx<R>>() : nullptr;
+  a.sol_fstride = P->sol_fstride;
+  a.nz = p.nz;
+  return a;
+}
+
+template <class R>
+void run_cn(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  TriArgs<R> a = tri_args<R>(P);
+  a.first_step = first;
+  a.nsteps = nsteps;
+  const double bytes = 2.0 * P->cells * P->p.batch * sizeof(cx<R>) * nsteps;
+  const int ev0 = begin(P);
+  record(P, "cn_march", bytes, launch_cn_march<R>(a, P->stream), ev0);
+}
+
+template <class R>
+void run_adi(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  TriArgs<R> a = tri_args<R>(P);
+  const double pass_bytes = 2.0 * P->cells * P->p.batch * sizeof(cx<R>);
+  for (unsigned st = first; st < first + nsteps; ++st) {
+    a.first_step = st;
+    int ev0 = begin(P);
+    record(P, "adi_x", pass_bytes, launch_adi_half<R>(a, true, P->stream), ev0);
+    ev0 = begin(P);
+    record(P, "adi_y", pass_bytes, launch_adi_half<R>(a, false, P->stream), ev0);
+    if (store_step(P, st)) cube_store<R>(P, st);
+  }
+}
+
+void run(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  if (!P->field_set) fail(PXG_ERR_STATE, "initial conditions not set (pxg_set_field)");
+  if (first < 1) fail(PXG_ERR_INVALID, "steps start at 1 (step 0 is the initial condition)");
+  if (nsteps == 0) return;
+  const int k = P->p.kind;
+  if (P->mixed) {
+    if (k == PXG_FFT3D) run_fft3d<float>(P, first, nsteps);
+    else if (k == PXG_FFT2D) run_fft2d<float>(P, first, nsteps);
+    else if (k == PXG_CN) run_cn<float>(P, first, nsteps);
+    else run_adi<float>(P, first, nsteps);
+  } else {
+    if (k == PXG_FFT3D) run_fft3d<double>(P, first, nsteps);
+    else if (k == PXG_FFT2D) run_fft2d<double>(P, first, nsteps);
+    else if (k == PXG_CN) run_cn<double>(P, first, nsteps);
+    else run_adi<double>(P, first, nsteps);
+  }
+}
+
+void check_plan(pxg_plan P)
+{
+  if (!P) fail(PXG_ERR_INVALID, "null plan");
+}
+
+void check_b(pxg_plan P, int b)
+{
+  check_plan(P);
+  if (b < 0 || b >= P->p.batch) fail(PXG_ERR_INVALID, "batch index %d out of range [0,%d)", b, P->p.batch);
+}
+
+// Shape AABB in world coordinates from p = A^-1 (q - t) over the shape-space box.
+DBox shape_box(const pxg_shape &s)
+{
+  const double *T = s.T;
+  double A[3][3] = {{T[0], T[1], T[2]}, {T[4], T[5], T[6]}, {T[8], T[9], T[10]}};
+  double t[3] = {T[3], T[7], T[11]};
+  double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+               A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+               A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  DBox bx;
+  if (!(std::fabs(det) > 1e-300)) {
+    for (int i = 0; i < 3; ++i) {
+      bx.lo[i] = -INFINITY;
+      bx.hi[i] = INFINITY;
+    }
+    return bx;
+  }
+  double inv[3][3];
+  inv[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+  inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+  inv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+  inv[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+  inv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+  inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+  inv[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+  inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+  inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+  double e[3];
+  if (s.type == PXG_SHAPE_SPHERE) {
+    e[0] = e[1] = e[2] = std::fabs(s.p[0]);
+  } else if (s.type == PXG_SHAPE_BOX) {
+    for (int i = 0; i < 3; ++i) e[i] = std::fabs(s.p[i]) / 2.0;
+  } else {
+    const double r = std::max(std::fabs(s.p[0]), std::fabs(s.p[1]));
+    e[0] = e[1] = r;
+    e[2] = std::fabs(s.p[2]) / 2.0;
+  }
+  for (int i = 0; i < 3; ++i) {
+    double c = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]);
+    double h = std::fabs(inv[i][0]) * e[0] + std::fabs(inv[i][1]) * e[1] + std::fabs(inv[i][2]) * e[2];
+    const double pad = 1e-9 * (std::fabs(c) + h) + 1e-9;
+    bx.lo[i] = c - h - pad;
+    bx.hi[i] = c + h + pad;
+  }
+  return bx;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char *pxg_last_error(void) { return g_err.c_str(); }
+
+const char *pxg_version(void)
+{
+  return "paxgpu 0.1 gfx950: stockham-fft(16..4096) fft3d-2pass fft2d cn-partitioned adi";
+}
+
+int pxg_ctx_create(int device, size_t hbm_budget_bytes, pxg_ctx *out)
+{
+  return guarded([&] {
+    if (!out) fail(PXG_ERR_INVALID, "null output pointer");
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count) fail(PXG_ERR_INVALID, "device %d not present (%d devices)", device, count);
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    auto *c = new pxg_ctx_s;
+    c->device = device;
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<size_t>(free_b * 0.9);
+    *out = c;
+  });
+}
+
+int pxg_ctx_destroy(pxg_ctx ctx)
+{
+  return guarded([&] {
+    if (!ctx) fail(PXG_ERR_INVALID, "null context");
+    delete ctx;
+  });
+}
+
+size_t pxg_ctx_reserved(pxg_ctx ctx) { return ctx ? ctx->reserved.load() : 0; }
+
+int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes)
+{
+  return guarded([&] {
+    if (!prob || !bytes) fail(PXG_ERR_INVALID, "null argument");
+    validate(*prob);
+    *bytes = plan_bytes(*prob);
+  });
+}
+
+int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
+{
+  pxg_plan P = nullptr;
+  int rc = guarded([&] {
+    if (!ctx || !prob || !out) fail(PXG_ERR_INVALID, "null argument");
+    validate(*prob);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t need = plan_bytes(*prob);
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->reserved.load() + need > ctx->budget)
+        fail(PXG_ERR_BUDGET, "plan needs %zu bytes, budget %zu has %zu reserved", need, ctx->budget,
+             ctx->reserved.load());
+      ctx->reserved += need;
+    }
+    P = new pxg_plan_s;
+    P->ctx = ctx;
+    P->reserved = need;
+    P->p = *prob;
+    if (P->p.n_refr == 0) P->p.n_refr = 10;
+    P->mixed = prob->precision == PXG_PREC_MIXED;
+    P->esz = P->mixed ? 8 : 16;
+    const bool is3d = prob->kind == PXG_FFT3D || prob->kind == PXG_ADI;
+    P->cells = is3d ? static_cast<size_t>(prob->nx) * prob->nx : prob->nx;
+    hip_check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
+    P->field.alloc(P->cells * prob->batch * P->esz);
+    hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
+    P->k_host.assign(prob->batch, prob->k);
+    if (prob->kind == PXG_FFT3D) {
+      const int ml = max_lines(prob->nx, P->esz);
+      P->lines_y = ml;
+      P->lines_x = ml;
+      // keep enough work-groups in flight for small batches
+      while (P->lines_y > 1 && (prob->nx / P->lines_y) * prob->batch < 512) P->lines_y /= 2;
+      while (P->lines_x > 4 && (prob->nx / P->lines_x) * prob->batch < 512) P->lines_x /= 2;
+    }
+    if (prob->store_dsz > 0) {
+      if (is3d) {
+        const int cn = prob->nx / prob->store_dsx;
+        P->sol_dims[0] = P->sol_dims[1] = cn;
+        P->sol_dims[2] = prob->nz / prob->store_dsz + 1;
+        P->sol_fstride = static_cast<size_t>(cn) * cn * P->sol_dims[2];
+        P->sol.alloc(P->sol_fstride * prob->batch * sizeof(cx<double>));
+      } else {
+        P->sol_dims[0] = prob->nx;
+        P->sol_dims[1] = prob->nz;
+        P->sol_dims[2] = 1;
+        P->sol_fstride = static_cast<size_t>(prob->nx) * prob->nz;
+        P->sol.alloc(P->sol_fstride * prob->batch * P->esz);
+      }
+      hip_check(hipMemsetAsync(P->sol.p, 0, P->sol.bytes, P->stream), "memset");
+    }
+    if (prob->kind == PXG_CN || prob->kind == PXG_ADI) {
+      const size_t sb = tridiag_scratch_bytes(*prob, P->mixed);
+      if (sb) P->scratch.alloc(sb);
+    }
+    rebuild_tables(P);
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    *out = P;
+  });
+  if (rc != PXG_OK && P) {
+    std::string keep = g_err;
+    pxg_plan_destroy(P);
+    g_err = keep;
+  } else if (rc != PXG_OK && ctx && prob && rc != PXG_ERR_BUDGET && rc != PXG_ERR_INVALID &&
+             rc != PXG_ERR_UNSUPPORTED) {
+    // reservation already released with the plan
+  }
+  return rc;
+}
+
+int pxg_plan_destroy(pxg_plan P)
+{
+  return guarded([&] {
+    if (!P) fail(PXG_ERR_INVALID, "null plan");
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (auto e : P->ev) (void)hipEventDestroy(e);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    if (P->ctx) P->ctx->reserved -= P->reserved;
+    delete P;
+  });
+}
+
+int pxg_set_wavenumbers(pxg_plan P, const double *k, int n)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!k || n != P->p.batch) fail(PXG_ERR_INVALID, "need %d wavenumbers", P->p.batch);
+    for (int i = 0; i < n; ++i)
+      if (!(k[i] > 0)) fail(PXG_ERR_INVALID, "wavenumber %d must be > 0", i);
+    P->k_host.assign(k, k + n);
+    rebuild_tables(P);
+  });
+}
+
+int pxg_set_field(pxg_plan P, int b, const double *c128)
+{
+  return guarded([&] {
+    check_b(P, b);
+    if (!c128) fail(PXG_ERR_INVALID, "null field");
+    const size_t n = P->cells;
+    char *dst = P->field.as<char>() + static_cast<size_t>(b) * n * P->esz;
+    if (P->mixed) {
+      std::vector<cx<float>> tmp(n);
+      for (size_t i = 0; i < n; ++i) tmp[i] = {static_cast<float>(c128[2 * i]), static_cast<float>(c128[2 * i + 1])};
+      hip_check(hipMemcpyAsync(dst, tmp.data(), n * 8, hipMemcpyHostToDevice, P->stream), "set_field");
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+    } else {
+      hip_check(hipMemcpyAsync(dst, c128, n * 16, hipMemcpyHostToDevice, P->stream), "set_field");
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+    }
+    P->field_set = true;
+    // copyCurrentSolution(0): slice / column 0 of the stored solution
+    const pxg_problem &p = P->p;
+    if (p.store_dsz > 0) {
+      const bool is3d = p.kind == PXG_FFT3D || p.kind == PXG_ADI;
+      if (is3d) {
+        // store slice 0 for this field only: temporarily view the batch entry
+        const size_t cn = P->sol_dims[0];
+        std::vector<double> host(2 * n);
+        std::memcpy(host.data(), c128, sizeof(double) * 2 * n);
+        if (P->mixed)
+          for (size_t i = 0; i < 2 * n; ++i) host[i] = static_cast<float>(host[i]);
+        const double d = static_cast<double>(p.nx) / cn;
+        std::vector<cx<double>> slice(cn * cn);
+        for (size_t i = 0; i < cn * cn; ++i) {
+          const unsigned row = i % cn, col = i / cn;
+          const unsigned r0 = row * d, r1 = row * d + d - 1, c0 = col * d, c1 = col * d + d - 1;
+          double tx = 0, ty = 0;
+          for (unsigned cc = c0; cc <= c1; ++cc) {
+            double sx = 0, sy = 0;
+            for (unsigned rr = r0; rr <= r1; ++rr) {
+              sx += host[2 * (static_cast<size_t>(cc) * p.nx + rr)];
+              sy += host[2 * (static_cast<size_t>(cc) * p.nx + rr) + 1];
+            }
+            tx += sx;
+            ty += sy;
+          }
+          slice[i] = {tx / (d * d), ty / (d * d)};
+        }
+        char *sd = P->sol.as<char>() + static_cast<size_t>(b) * P->sol_fstride * sizeof(cx<double>);
+        hip_check(hipMemcpyAsync(sd, slice.data(), slice.size() * sizeof(cx<double>),
+                                 hipMemcpyHostToDevice, P->stream), "store slice 0");
+      } else {
+        char *sd = P->sol.as<char>() + static_cast<size_t>(b) * P->sol_fstride * P->esz;
+        hip_check(hipMemcpyAsync(sd, dst, n * P->esz, hipMemcpyDeviceToDevice, P->stream),
+                  "store column 0");
+      }
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+    }
+  });
+}
+
+int pxg_get_field(pxg_plan P, int b, double *c128)
+{
+  return guarded([&] {
+    check_b(P, b);
+    if (!c128) fail(PXG_ERR_INVALID, "null output");
+    const size_t n = P->cells;
+    const char *src = P->field.as<char>() + static_cast<size_t>(b) * n * P->esz;
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    if (P->mixed) {
+      std::vector<cx<float>> tmp(n);
+      hip_check(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost), "get_field");
+      for (size_t i = 0; i < n; ++i) {
+        c128[2 * i] = tmp[i].x;
+        c128[2 * i + 1] = tmp[i].y;
+      }
+    } else {
+      hip_check(hipMemcpy(c128, src, n * 16, hipMemcpyDeviceToHost), "get_field");
+    }
+  });
+}
+
+int pxg_solution_dims(pxg_plan P, int dims[3])
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!dims) fail(PXG_ERR_INVALID, "null output");
+    for (int i = 0; i < 3; ++i) dims[i] = P->sol_dims[i];
+  });
+}
+
+int pxg_get_solution(pxg_plan P, int b, double *c128, size_t n_complex)
+{
+  return guarded([&] {
+    check_b(P, b);
+    if (P->p.store_dsz <= 0) fail(PXG_ERR_STATE, "plan stores no solution (store_dsz = 0)");
+    if (!c128 || n_complex < P->sol_fstride)
+      fail(PXG_ERR_INVALID, "output needs %zu complex values", P->sol_fstride);
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    const bool is3d = P->p.kind == PXG_FFT3D || P->p.kind == PXG_ADI;
+    const size_t n = P->sol_fstride;
+    if (is3d || !P->mixed) {
+      const char *src = P->sol.as<char>() + static_cast<size_t>(b) * n * 16;
+      hip_check(hipMemcpy(c128, src, n * 16, hipMemcpyDeviceToHost), "get_solution");
+    } else {
+      std::vector<cx<float>> tmp(n);
+      const char *src = P->sol.as<char>() + static_cast<size_t>(b) * n * 8;
+      hip_check(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost), "get_solution");
+      for (size_t i = 0; i < n; ++i) {
+        c128[2 * i] = tmp[i].x;
+        c128[2 * i + 1] = tmp[i].y;
+      }
+    }
+  });
+}
+
+int pxg_set_material_none(pxg_plan P)
+{
+  return guarded([&] {
+    check_plan(P);
+    P->mat_mode = MAT_NONE;
+  });
+}
+
+int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, const pxg_part *parts,
+                         int n_parts, const pxg_module *modules, int n_modules, int is_reference_run)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (n_shapes < 0 || n_parts < 0 || n_modules < 0) fail(PXG_ERR_INVALID, "negative count");
+    if ((n_shapes && !shapes) || (n_parts && !parts) || (n_modules && !modules))
+      fail(PXG_ERR_INVALID, "null CSG array");
+    std::vector<DShape> ds(n_shapes);
+    for (int i = 0; i < n_shapes; ++i) {
+      if (shapes[i].type < 0 || shapes[i].type > 2) fail(PXG_ERR_INVALID, "shape %d: bad type", i);
+      ds[i].type = shapes[i].type;
+      ds[i].op = shapes[i].op;
+      std::memcpy(ds[i].T, shapes[i].T, sizeof(double) * 12);
+      std::memcpy(ds[i].p, shapes[i].p, sizeof(double) * 3);
+    }
+    std::vector<DPart> order;
+    std::vector<DBox> boxes;
+    for (int m = 0; m < n_modules; ++m) {
+      const pxg_module &mod = modules[m];
+      if (mod.first_part < 0 || mod.n_parts < 0 || mod.first_part + mod.n_parts > n_parts)
+        fail(PXG_ERR_INVALID, "module %d: part range out of bounds", m);
+      for (int i = mod.n_parts - 1; i >= 0; --i) {
+        const pxg_part &q = parts[mod.first_part + i];
+        if (q.first_shape < 0 || q.n_shapes < 0 || q.first_shape + q.n_shapes > n_shapes)
+          fail(PXG_ERR_INVALID, "part %d: shape range out of bounds", mod.first_part + i);
+        order.push_back(DPart{q.first_shape, q.n_shapes, q.op, 0, q.delta, q.beta});
+        DBox bx;
+        for (int a = 0; a < 3; ++a) {
+          bx.lo[a] = INFINITY;
+          bx.hi[a] = -INFINITY;
+        }
+        for (int s = 0; s < q.n_shapes; ++s) {
+          const pxg_shape &sh = shapes[q.first_shape + s];
+          if (sh.op != PXG_OP_UNION) continue;
+          DBox sb = shape_box(sh);
+          for (int a = 0; a < 3; ++a) {
+            bx.lo[a] = std::min(bx.lo[a], sb.lo[a]);
+            bx.hi[a] = std::max(bx.hi[a], sb.hi[a]);
+          }
+        }
+        boxes.push_back(bx);
+      }
+    }
+    if (order.size() > 1024) fail(PXG_ERR_UNSUPPORTED, "at most 1024 CSG parts (got %zu)", order.size());
+    P->shapes.alloc(std::max<size_t>(1, ds.size()) * sizeof(DShape));
+    P->parts.alloc(std::max<size_t>(1, order.size()) * sizeof(DPart));
+    P->boxes.alloc(std::max<size_t>(1, boxes.size()) * sizeof(DBox));
+    if (!ds.empty())
+      hip_check(hipMemcpy(P->shapes.p, ds.data(), ds.size() * sizeof(DShape), hipMemcpyHostToDevice), "csg");
+    if (!order.empty()) {
+      hip_check(hipMemcpy(P->parts.p, order.data(), order.size() * sizeof(DPart), hipMemcpyHostToDevice), "csg");
+      hip_check(hipMemcpy(P->boxes.p, boxes.data(), boxes.size() * sizeof(DBox), hipMemcpyHostToDevice), "csg");
+    }
+    P->parts_host = order;
+    P->n_parts = static_cast<int>(order.size());
+    P->is_ref = is_reference_run;
+    P->mat_mode = MAT_CSG;
+    if (P->mixed) build_part_factors<float>(P);
+    else build_part_factors<double>(P);
+  });
+}
+
+int pxg_set_material_curved_wg(pxg_plan P, const pxg_curved_wg *wg)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!wg) fail(PXG_ERR_INVALID, "null waveguide");
+    if (P->p.kind == PXG_FFT3D || P->p.kind == PXG_ADI)
+      fail(PXG_ERR_INVALID, "the curved waveguide is a 2D material");
+    P->wg = DCurved{wg->R, wg->width, wg->clad_delta, wg->clad_beta, wg->in_delta, wg->in_beta,
+                    wg->wglength, wg->dx, wg->dz, wg->use_smoothed};
+    P->mat_mode = MAT_CURVED;
+  });
+}
+
+int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int first_plane, int z_invariant)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!db || n_planes < 1) fail(PXG_ERR_INVALID, "need at least one plane");
+    const size_t per = 2 * P->cells;
+    P->planes.alloc(per * n_planes * sizeof(double));
+    hip_check(hipMemcpy(P->planes.p, db, P->planes.bytes, hipMemcpyHostToDevice), "planes");
+    P->n_planes = n_planes;
+    P->first_plane = first_plane;
+    P->z_invariant = z_invariant;
+    P->mat_mode = MAT_PLANES;
+  });
+}
+
+int pxg_step(pxg_plan P, unsigned first_step, unsigned nsteps)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (P->mat_mode == MAT_PLANES && !P->z_invariant) {
+      const pxg_problem &p = P->p;
+      const int per_step = p.kind == PXG_ADI ? 2 : 1;
+      const long lo = (p.kind == PXG_CN ? static_cast<long>(first_step) - 1 : static_cast<long>(first_step)) * per_step;
+      const long hi = static_cast<long>(first_step + nsteps) * per_step - 1;
+      if (lo < P->first_plane || hi >= static_cast<long>(P->first_plane) + P->n_planes)
+        fail(PXG_ERR_INVALID, "material planes [%d, %d) do not cover steps %u..%u", P->first_plane,
+             P->first_plane + P->n_planes, first_step, first_step + nsteps - 1);
+    }
+    P->profiling = false;
+    run(P, first_step, nsteps);
+  });
+}
+
+int pxg_sync(pxg_plan P)
+{
+  return guarded([&] {
+    check_plan(P);
+    hip_check(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+  });
+}
+
+int pxg_step_profiled(pxg_plan P, unsigned first_step, unsigned nsteps, pxg_kernel_stat *stats,
+                      int max_stats, int *n_stats, double *total_ms)
+{
+  return guarded([&] {
+    check_plan(P);
+    const size_t need = 2 * (static_cast<size_t>(nsteps) * 4 + 8);
+    while (P->ev.size() < need) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      P->ev.push_back(e);
+    }
+    P->recs.clear();
+    P->profiling = true;
+    hipEvent_t t0, t1;
+    hip_check(hipEventCreate(&t0), "event");
+    hip_check(hipEventCreate(&t1), "event");
+    hip_check(hipEventRecord(t0, P->stream), "event");
+    try {
+      run(P, first_step, nsteps);
+    } catch (...) {
+      P->profiling = false;
+      (void)hipEventDestroy(t0);
+      (void)hipEventDestroy(t1);
+      throw;
+    }
+    P->profiling = false;
+    hip_check(hipEventRecord(t1, P->stream), "event");
+    hip_check(hipEventSynchronize(t1), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+    if (total_ms) *total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    struct Acc {
+      const char *name;
+      double sum;
+      int n;
+      double bytes;
+    };
+    std::vector<Acc> acc;
+    for (const auto &r : P->recs) {
+      float k = 0;
+      hip_check(hipEventElapsedTime(&k, P->ev[r.e0], P->ev[r.e1]), "elapsed");
+      auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc &a) { return std::strcmp(a.name, r.name) == 0; });
+      if (it == acc.end()) acc.push_back({r.name, k, 1, r.bytes});
+      else {
+        it->sum += k;
+        it->n++;
+      }
+    }
+    const int n = std::min<int>(max_stats, static_cast<int>(acc.size()));
+    for (int i = 0; i < n && stats; ++i) stats[i] = {acc[i].name, acc[i].sum / acc[i].n, acc[i].n, acc[i].bytes};
+    if (n_stats) *n_stats = n;
+  });
+}
+
+int pxg_field_device_ptr(pxg_plan P, int b, void **dptr)
+{
+  return guarded([&] {
+    check_b(P, b);
+    if (!dptr) fail(PXG_ERR_INVALID, "null output");
+    *dptr = P->field.as<char>() + static_cast<size_t>(b) * P->cells * P->esz;
+  });
+}
+
+int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
+                       double *rhs)
+{
+  return guarded([&] {
+    if (!ctx) fail(PXG_ERR_INVALID, "null context");
+    if (nsys < 1 || n < 1) fail(PXG_ERR_INVALID, "need nsys >= 1 and n >= 1");
+    if (layout != PXG_LAYOUT_CONTIGUOUS && layout != PXG_LAYOUT_INTERLEAVED)
+      fail(PXG_ERR_INVALID, "unknown layout %d", layout);
+    if (!diag || !rhs || (n > 1 && !sub)) fail(PXG_ERR_INVALID, "null array");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t N = static_cast<size_t>(nsys) * n;
+    const size_t need = N * 16 * 3 + N * 16 * 2;
+    if (ctx->reserved.load() + need > ctx->budget) fail(PXG_ERR_BUDGET, "thomas needs %zu bytes", need);
+    DBuf d, s, r, w;
+    d.alloc(N * 16);
+    s.alloc(std::max<size_t>(1, static_cast<size_t>(nsys) * (n - 1)) * 16);
+    r.alloc(N * 16);
+    w.alloc(N * 16 * 2);
+    hip_check(hipMemcpy(d.p, diag, N * 16, hipMemcpyHostToDevice), "thomas upload");
+    if (n > 1) hip_check(hipMemcpy(s.p, sub, static_cast<size_t>(nsys) * (n - 1) * 16, hipMemcpyHostToDevice), "thomas upload");
+    hip_check(hipMemcpy(r.p, rhs, N * 16, hipMemcpyHostToDevice), "thomas upload");
+    hip_check(launch_thomas_batched(layout, nsys, n, d.as<cx<double>>(), s.as<cx<double>>(),
+                                    r.as<cx<double>>(), w.as<cx<double>>(), nullptr),
+              "thomas kernel");
+    hip_check(hipDeviceSynchronize(), "thomas sync");
+    hip_check(hipMemcpy(diag, d.p, N * 16, hipMemcpyDeviceToHost), "thomas download");
+    hip_check(hipMemcpy(rhs, r.p, N * 16, hipMemcpyDeviceToHost), "thomas download");
+  });
+}
+
+}  // extern "C"

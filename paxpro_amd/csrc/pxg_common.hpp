@@ -1,0 +1,77 @@
+// Shared device/host helpers of libpaxgpu: complex types, arithmetic, and the
+// status/error plumbing behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace pxg {
+
+// Interleaved complex of real type R; 8-byte (float) or 16-byte (double) aligned
+// so a lane moves one complex per load.
+template <class R>
+struct alignas(2 * sizeof(R)) cx {
+  R x, y;
+};
+using cf = cx<float>;
+using cd = cx<double>;
+
+template <class R>
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b)
+{
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class R>
+__host__ __device__ __forceinline__ cx<R> scale(cx<R> a, R s) { return {a.x * s, a.y * s}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { return {a.x, -a.y}; }
+// multiply by S*i, S = +1 or -1
+template <int S, class R>
+__host__ __device__ __forceinline__ cx<R> mul_si(cx<R> a) { return S > 0 ? cx<R>{-a.y, a.x} : cx<R>{a.y, -a.x}; }
+// complex division (a / b), scaled to avoid overflow (Smith's algorithm)
+template <class R>
+__host__ __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b)
+{
+  R ax = b.x < 0 ? -b.x : b.x, ay = b.y < 0 ? -b.y : b.y;
+  if (ax >= ay) {
+    R r = b.y / b.x, den = b.x + b.y * r;
+    return {(a.x + a.y * r) / den, (a.y - a.x * r) / den};
+  }
+  R r = b.x / b.y, den = b.x * r + b.y;
+  return {(a.x * r + a.y) / den, (a.y * r - a.x) / den};
+}
+template <class R>
+__host__ __device__ __forceinline__ R norm2(cx<R> a) { return a.x * a.x + a.y * a.y; }
+
+template <class D, class S>
+__host__ __device__ __forceinline__ cx<D> cvt(cx<S> a) { return {static_cast<D>(a.x), static_cast<D>(a.y)}; }
+
+// exp(i*phi) for a phase argument held in FP64: reduced to [-pi, pi] in FP64
+// before the sine and cosine are taken, then rounded to R.
+template <class R>
+__device__ __forceinline__ cx<R> expi_fp64(double phi)
+{
+  const double two_pi = 6.283185307179586476925286766559;
+  double r = phi - two_pi * rint(phi / two_pi);
+  double s, c;
+  sincos(r, &s, &c);
+  return {static_cast<R>(c), static_cast<R>(s)};
+}
+
+// Grid coordinate min + i*step, evaluated without FMA contraction like the
+// reference's getX/getY/getZ (src/paraxialSimulation.cpp:250-263).
+__host__ __device__ __forceinline__ double coord(double min, unsigned i, double step)
+{
+#pragma clang fp contract(off)
+  return min + static_cast<double>(i) * step;
+}
+
+}  // namespace pxg

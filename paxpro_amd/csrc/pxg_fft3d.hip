@@ -1,0 +1,352 @@
+// Split-step Fourier kernels: FFTSolver3D (two fused passes per z-step) and
+// FFTSolver2D (one fused pass per z-step), plus the Solver3D cube storage.
+//
+// The 3D step of the reference (src/fftSolver3D.cpp:52-156,327-374) is
+//   u <- A . P_z . IFFT2[ K . FFT2[u] ] / N^2
+// with K(kx, ky) = exp(-i dz/(2k) (kx^2 + ky^2)) = Kx(kx) Ky(ky) separable, P_z
+// the per-cell phase screen exp(-k (beta + i delta) dz) and A the separable edge
+// absorber.  The field is stored with y contiguous (Armadillo column-major), so
+// FFT2 = FFT_y (contiguous lines) then FFT_x (strided lines).  Because Ky
+// commutes with the x transforms, one z-step becomes two passes over HBM:
+//   pass X (strided x-lines):     FFT_x -> x Kx Ky -> IFFT_x
+//   pass Y (contiguous y-lines):  IFFT_y -> x P_z/N^2 -> x A -> FFT_y (next step)
+// A run of n steps is one forward FFT_y pass, then n (X, Y) pairs, the last Y
+// pass leaving the field in real space.  Each pass reads and writes the field
+// once: 4 x sizeof(complex) bytes per cell and step.
+#include "pxg_fft.hpp"
+#include "pxg_internal.hpp"
+
+namespace pxg {
+
+constexpr int MAX_MASK_WORDS = 16;  // CSG parts per material: up to 1024
+
+// Phase-screen factor exp(-k (beta + i delta) dz) / N^2 from FP64 material
+// values; the phase argument stays in FP64 and is reduced mod 2 pi before the
+// sine and cosine (src/fftSolver3D.cpp:154).
+template <class R>
+__device__ __forceinline__ cx<R> screen_from_values(double k, double dz, double delta, double beta,
+                                                    double inv_norm)
+{
+  double amp = exp((-k * beta) * dz) * inv_norm;
+  cx<R> e = expi_fp64<R>((-k * delta) * dz);
+  return {static_cast<R>(e.x * amp), static_cast<R>(e.y * amp)};
+}
+
+template <>
+__device__ __forceinline__ cx<double> screen_from_values<double>(double k, double dz, double delta,
+                                                                 double beta, double inv_norm)
+{
+  double amp = exp((-k * beta) * dz);
+  double s, c;
+  sincos((-k * delta) * dz, &s, &c);
+  return {c * amp * inv_norm, s * amp * inv_norm};
+}
+
+// Border-crossing integral of FFTSolver3D::refractionIntegral
+// (src/fftSolver3D.cpp:253-273): f(z0) + f(z1) + 2 sum_{i=1}^{n-2} f(z0 + i dz/n),
+// divided by 2n; the reference skips sample n-1 (PXG_FIX_REFR_SAMPLES restores it).
+__device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_t *mask, int nw,
+                                                  unsigned n_refr, unsigned quirks, double x,
+                                                  double y, double za, double zb, double &delta,
+                                                  double &beta)
+{
+  double dt, bt;
+  part_values(m, csg_eval(m, mask, nw, x, y, za), delta, beta);
+  part_values(m, csg_eval(m, mask, nw, x, y, zb), dt, bt);
+  delta += dt;
+  beta += bt;
+  const double h = (zb - za) / n_refr;
+  const unsigned last = (quirks & 2u) ? n_refr : n_refr - 1;
+  for (unsigned i = 1; i < last; ++i) {
+    const double z = za + i * h;
+    part_values(m, csg_eval(m, mask, nw, x, y, z), dt, bt);
+    delta += 2.0 * dt;
+    beta += 2.0 * bt;
+  }
+  delta /= (2.0 * n_refr);
+  beta /= (2.0 * n_refr);
+}
+
+// ---------------------------------------------------------------------------
+// Pass Y: contiguous y-lines (matrix columns); lane t of line `line` holds rows
+// t + m*N/8.  L lines (columns) per work-group, blockIdx.y = field.
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
+{
+  constexpr int T = N / 8;
+  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  const int tid = threadIdx.x, line = tid / T, t = tid % T;
+  const int c = blockIdx.x * L + line;
+  const int b = blockIdx.y;
+  cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N;
+  cx<R> *ll = lds + line * LdsLine<N>::STRIDE;
+  cx<R> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = f[t + m * T];
+
+  if (a.mode & Y_INV) {
+    fft_line<N, +1>(v, ll, t, a.tw);
+    const double k = a.kvec[b];
+    const double x = coord(a.xmin, c, a.dx);
+    if (a.mat.mode == MAT_NONE) {
+      const R s = static_cast<R>(a.inv_norm);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = scale(v[m], s);
+    } else if (a.mat.mode == MAT_CSG) {
+      const double z1 = coord(a.zmin, a.step, a.dz);
+      const double z0 = coord(a.zmin, a.step - 1, a.dz);
+      const int nw = (a.mat.n_parts + 63) >> 6;
+      const int c0 = blockIdx.x * L;
+      csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx),
+               a.ymin, coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
+      const cx<R> *pf = static_cast<const cx<R> *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
+#pragma unroll 1
+      for (int m = 0; m < 8; ++m) {
+        const int r = t + m * T;
+        const double y = coord(a.ymin, r, a.dy);
+        const int e1 = csg_eval(a.mat, mask, nw, x, y, z1);
+        const int e0 = csg_eval(a.mat, mask, nw, x, y, z0);
+        cx<R> fac;
+        if (e1 == e0) {
+          fac = pf[e1];
+        } else {
+          double d1, b1, d0, b0;
+          part_values(a.mat, e1, d1, b1);
+          part_values(a.mat, e0, d0, b0);
+          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
+            crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, y, z0, z1, d1, b1);
+            fac = screen_from_values<R>(k, a.dz, d1, b1, a.inv_norm);
+          } else {
+            fac = pf[e1];
+          }
+        }
+        v[m] = v[m] * fac;
+      }
+    } else {  // MAT_PLANES: effective (delta, beta) of this step per cell
+      const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
+      const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride +
+                         2 * static_cast<size_t>(c) * N;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int r = t + m * T;
+        v[m] = v[m] * screen_from_values<R>(k, a.dz, pl[2 * r], pl[2 * r + 1], a.inv_norm);
+      }
+    }
+    if (a.absy) {
+      const R ax = a.absx[c];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = scale(scale(v[m], a.absy[t + m * T]), ax);
+    }
+  }
+  if (a.mode & Y_FWD) fft_line<N, -1>(v, ll, t, a.tw);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) f[t + m * T] = v[m];
+}
+
+// ---------------------------------------------------------------------------
+// Pass X: strided x-lines (matrix rows).  Lanes run line-fastest so that each
+// load/store instruction touches L consecutive rows of a column (L*8 or L*16
+// contiguous bytes).  FFT_x, multiply by Kx(c) Ky(r), IFFT_x.
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
+{
+  constexpr int T = N / 8;
+  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  const int tid = threadIdx.x, l = tid % L, t = tid / L;
+  const int r = blockIdx.x * L + l;
+  const int b = blockIdx.y;
+  cx<R> *f = a.field + b * a.fstride + r;
+  cx<R> *ll = lds + l * LdsLine<N>::STRIDE;
+  cx<R> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = f[static_cast<size_t>(t + m * T) * N];
+  fft_line<N, -1>(v, ll, t, a.tw);
+  const cx<R> kyr = a.ky[b * N + r];
+  const cx<R> *kx = a.kx + b * N;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[t + m * T] * kyr);
+  fft_line<N, +1>(v, ll, t, a.tw);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) f[static_cast<size_t>(t + m * T) * N] = v[m];
+}
+
+// ---------------------------------------------------------------------------
+// FFTSolver2D step (src/fftSolver2D.cpp:15-74): one field per line,
+// FFT -> x exp(-i dz/(2k) kx^2) -> IFFT -> x exp(-k(beta + i delta) dz)/N with
+// the material at z_step + dz/2; the result is also stored as solution column.
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_fft2d_step(FftArgs<R> a)
+{
+  constexpr int T = N / 8;
+  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  const int tid = threadIdx.x, line = tid / T, t = tid % T;
+  const int b = blockIdx.x * L + line;
+  cx<R> *f = a.field + b * a.fstride;
+  cx<R> *ll = lds + line * LdsLine<N>::STRIDE;
+  cx<R> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = f[t + m * T];
+  fft_line<N, -1>(v, ll, t, a.tw);
+  const cx<R> *kx = a.kx + b * N;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = v[m] * kx[t + m * T];
+  fft_line<N, +1>(v, ll, t, a.tw);
+  const double k = a.kvec[b];
+  const double z = coord(a.zmin, a.step, a.dz) + 0.5 * a.dz;
+  if (a.mat.mode == MAT_CSG) {
+    const int nw = (a.mat.n_parts + 63) >> 6;
+    csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), 0.0, 0.0, z, z);
+  }
+#pragma unroll 1
+  for (int m = 0; m < 8; ++m) {
+    const int i = t + m * T;
+    const double x = coord(a.xmin, i, a.dx);
+    double d = 0.0, be = 0.0;
+    if (a.mat.mode == MAT_CSG) {
+      part_values(a.mat, csg_eval(a.mat, mask, (a.mat.n_parts + 63) >> 6, x, 0.0, z), d, be);
+    } else if (a.mat.mode == MAT_CURVED) {
+      curved_material(a.mat.wg, x, z, d, be);
+    } else if (a.mat.mode == MAT_PLANES) {
+      const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
+      const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride;
+      d = pl[2 * i];
+      be = pl[2 * i + 1];
+    }
+    v[m] = v[m] * screen_from_values<R>(k, a.dz, d, be, a.inv_norm);
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) f[t + m * T] = v[m];
+  if (a.sol) {
+    cx<R> *sc = a.sol + b * a.sol_fstride + static_cast<size_t>(a.step) * N;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) sc[t + m * T] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solver3D::copyCurrentSolution block average (src/solver3D.cpp:78-125): each
+// output cell sums its block column by column, then divides by the block area.
+template <class R>
+__global__ void k_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
+                             cx<double> *cube, size_t cube_fstride, int slice)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= cube_n * cube_n) return;
+  const int row = i % cube_n, col = i / cube_n;
+  const double d = static_cast<double>(n) / cube_n;
+  const unsigned r0 = static_cast<unsigned>(row * d), r1 = static_cast<unsigned>(row * d + d - 1);
+  const unsigned c0 = static_cast<unsigned>(col * d), c1 = static_cast<unsigned>(col * d + d - 1);
+  const cx<R> *f = field + b * fstride;
+  double tx = 0.0, ty = 0.0;
+  for (unsigned cc = c0; cc <= c1; ++cc) {
+    double sx = 0.0, sy = 0.0;
+    for (unsigned rr = r0; rr <= r1; ++rr) {
+      const cx<R> u = f[static_cast<size_t>(cc) * n + rr];
+      sx += static_cast<double>(u.x);
+      sy += static_cast<double>(u.y);
+    }
+    tx += sx;
+    ty += sy;
+  }
+  cube[b * cube_fstride + static_cast<size_t>(slice) * cube_n * cube_n + i] = {tx / (d * d),
+                                                                             ty / (d * d)};
+}
+
+// ---------------------------------------------------------------------------
+bool fft_size_supported(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
+
+// Lines per work-group: at most 1024 lanes and 80 KiB of line storage, so two
+// work-groups fit in the 160 KiB LDS of a gfx950 CU.
+constexpr int LDS_CAP = 81920;
+int max_lines(int n, int elem_bytes)
+{
+  int l = 1024 / (n / 8);
+  while (l > 1 && l * (n + n / 8 + 1) * elem_bytes > LDS_CAP) l /= 2;
+  return l > 16 ? 16 : l;
+}
+
+#define PXG_LINES_CASE(KER, NN, LL)                                                      \
+  case LL:                                                                               \
+    if constexpr (LL * NN / 8 <= 1024 &&                                                 \
+                  LL * (NN + NN / 8 + 1) * sizeof(cx<R>) <= 81920) {                     \
+      hipLaunchKernelGGL((KER<NN, R, LL>), grid_of(NN, LL), dim3(LL * NN / 8), 0, s, a); \
+      return hipGetLastError();                                                          \
+    }                                                                                    \
+    return hipErrorInvalidValue;
+
+#define PXG_SIZE_CASE(KER, NN)            \
+  case NN:                                \
+    switch (lines) {                      \
+      PXG_LINES_CASE(KER, NN, 1)          \
+      PXG_LINES_CASE(KER, NN, 2)          \
+      PXG_LINES_CASE(KER, NN, 4)          \
+      PXG_LINES_CASE(KER, NN, 8)          \
+      PXG_LINES_CASE(KER, NN, 16)         \
+    default:                              \
+      return hipErrorInvalidValue;        \
+    }
+
+#define PXG_SIZE_SWITCH(KER)       \
+  switch (n) {                     \
+    PXG_SIZE_CASE(KER, 16)         \
+    PXG_SIZE_CASE(KER, 32)         \
+    PXG_SIZE_CASE(KER, 64)         \
+    PXG_SIZE_CASE(KER, 128)        \
+    PXG_SIZE_CASE(KER, 256)        \
+    PXG_SIZE_CASE(KER, 512)        \
+    PXG_SIZE_CASE(KER, 1024)       \
+    PXG_SIZE_CASE(KER, 2048)       \
+    PXG_SIZE_CASE(KER, 4096)       \
+  default:                         \
+    return hipErrorInvalidValue;   \
+  }
+
+template <class R>
+hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s)
+{
+  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, batch); };
+  PXG_SIZE_SWITCH(k_pass_y)
+}
+
+template <class R>
+hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s)
+{
+  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, batch); };
+  PXG_SIZE_SWITCH(k_pass_x)
+}
+
+template <class R>
+hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
+{
+  // one field per line; 4 lines per work-group when the batch divides evenly
+  const int lines = (batch % 4 == 0 && max_lines(n, sizeof(cx<R>)) >= 4) ? 4 : 1;
+  auto grid_of = [&](int, int ll) { return dim3(batch / ll, 1); };
+  PXG_SIZE_SWITCH(k_fft2d_step)
+}
+
+template <class R>
+hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
+                             cx<double> *cube, size_t cube_fstride, int slice, int batch,
+                             hipStream_t s)
+{
+  const int cells = cube_n * cube_n;
+  hipLaunchKernelGGL(k_cube_store<R>, dim3((cells + 255) / 256, batch), dim3(256), 0, s, field,
+                     fstride, n, cube_n, cube, cube_fstride, slice);
+  return hipGetLastError();
+}
+
+template hipError_t launch_pass_y<float>(int, int, const FftArgs<float> &, int, hipStream_t);
+template hipError_t launch_pass_y<double>(int, int, const FftArgs<double> &, int, hipStream_t);
+template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, hipStream_t);
+template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
+template hipError_t launch_fft2d_step<float>(int, const FftArgs<float> &, int, hipStream_t);
+template hipError_t launch_fft2d_step<double>(int, const FftArgs<double> &, int, hipStream_t);
+template hipError_t launch_cube_store<float>(const cx<float> *, size_t, int, int, cx<double> *,
+                                             size_t, int, int, hipStream_t);
+template hipError_t launch_cube_store<double>(const cx<double> *, size_t, int, int, cx<double> *,
+                                              size_t, int, int, hipStream_t);
+
+}  // namespace pxg

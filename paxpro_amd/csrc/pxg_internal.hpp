@@ -1,0 +1,48 @@
+// Internal structures shared by the C-ABI layer (pxg_api.hip) and the kernel
+// translation units.  Not part of the public interface.
+#pragma once
+
+#include "pxg_common.hpp"
+#include "pxg_material.hpp"
+
+namespace pxg {
+
+// Arguments of the split-step passes (FFT3D and FFT2D).
+template <class R>
+struct FftArgs {
+  cx<R> *field;            // batch of fields
+  size_t fstride;          // elements between consecutive fields
+  const cx<R> *tw;         // tw[m] = exp(-2 pi i m / N), m < N
+  const cx<R> *kx;         // [batch][N] exp(-i dz/(2k) kx^2), column (x) index
+  const cx<R> *ky;         // [batch][N] exp(-i dz/(2k) ky^2), row (y) index
+  const R *absx;           // [N] absorber factor along x, or null
+  const R *absy;           // [N] absorber factor along y, or null
+  const double *kvec;      // [batch] wavenumbers
+  MatDev mat;
+  double xmin, dx, ymin, dy, zmin, dz;
+  double inv_norm;         // 1/(number of cells)
+  unsigned step;           // z-step index being applied by the screen
+  unsigned n_refr;         // crossing-integral samples
+  unsigned quirks;
+  int mode;                // pass-Y mode bits (Y_INV | Y_FWD)
+  cx<R> *sol;              // FFT2D: solution columns (Nx x Nz per field) or null
+  size_t sol_fstride;
+};
+
+enum { Y_INV = 1, Y_FWD = 2 };
+
+// Launchers (pxg_fft3d.hip).  `lines` = lines per work-group.
+template <class R>
+hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
+                             cx<double> *cube, size_t cube_fstride, int slice, int batch,
+                             hipStream_t s);
+int max_lines(int n, int elem_bytes);
+bool fft_size_supported(int n);
+
+}  // namespace pxg

@@ -1,0 +1,177 @@
+// Device-side material evaluation: the per-cell getXrayMatProp call-back of the
+// reference, restated for the GPU.  Materials are evaluated in FP64 so that
+// membership tests decide exactly as the reference's double-precision code.
+#pragma once
+
+#include "pxg_common.hpp"
+
+namespace pxg {
+
+enum MatMode { MAT_NONE = 0, MAT_CSG = 1, MAT_PLANES = 2, MAT_CURVED = 3 };
+
+// One shape: rows 0..2 of its 4x4 world->shape transform and its parameters.
+struct DShape {
+  int type, op;
+  double T[12];
+  double p[3];
+};
+
+// One part in *evaluation order*: modules in order, each module's parts last
+// to first (src/materialFunction.cpp:3-13, src/geometry.cpp:212-234); the first
+// entry whose shapes contain the point decides.
+struct DPart {
+  int first_shape, n_shapes, op, pad;
+  double delta, beta;
+};
+
+// World-space bounding box of a part's union shapes (culling only).
+struct DBox {
+  double lo[3], hi[3];
+};
+
+struct DCurved {
+  double R, width, clad_delta, clad_beta, in_delta, in_beta, wglength, dx, dz;
+  int use_smoothed;
+};
+
+struct MatDev {
+  int mode;
+  // CSG
+  const DShape *shapes;
+  const DPart *parts;
+  const DBox *boxes;
+  int n_parts;
+  int is_ref;
+  const void *part_fac;   // cx<R>[batch][n_parts + 1]; entry n_parts = vacuum
+  // host-sampled planes: [plane][cell][2] doubles
+  const double *planes;
+  int n_planes, first_plane, z_invariant;
+  size_t plane_stride;    // doubles per plane
+  // curved waveguide
+  DCurved wg;
+};
+
+// Shape membership (src/shape.cpp:126-187), strict inequalities, FP64.
+__device__ __forceinline__ bool shape_inside(const DShape &s, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+  double px = s.T[0] * x + s.T[1] * y + s.T[2] * z + s.T[3];
+  double py = s.T[4] * x + s.T[5] * y + s.T[6] * z + s.T[7];
+  double pz = s.T[8] * x + s.T[9] * y + s.T[10] * z + s.T[11];
+  if (s.type == 0) return sqrt(px * px + py * py + pz * pz) < s.p[0];
+  if (s.type == 1)
+    return (px > -s.p[0] / 2.0) && (px < s.p[0] / 2.0) && (py > -s.p[1] / 2.0) &&
+           (py < s.p[1] / 2.0) && (pz > -s.p[2] / 2.0) && (pz < s.p[2] / 2.0);
+  double h = s.p[2];
+  if (!((pz < h / 2.0) && (pz > -h / 2.0))) return false;
+  double slope = (s.p[1] - s.p[0]) / h;
+  double radius = slope * (pz + h / 2.0) + s.p[0];
+  return sqrt(px * px + py * py) < radius;
+}
+
+// Part membership: shapes last to first, first containing shape decides
+// (src/geometry.cpp:91-108).
+__device__ __forceinline__ bool part_inside(const MatDev &m, const DPart &p, double x, double y,
+                                            double z)
+{
+  for (int i = p.n_shapes - 1; i >= 0; --i) {
+    const DShape &s = m.shapes[p.first_shape + i];
+    if (shape_inside(s, x, y, z)) return s.op == 0;
+  }
+  return false;
+}
+
+// Evaluation-order index of the deciding part, or n_parts when no part claims
+// the point (vacuum).  `mask` selects the parts that survived culling
+// (bit e of word e/64), nw words.
+__device__ __forceinline__ int csg_eval(const MatDev &m, const uint64_t *mask, int nw, double x,
+                                        double y, double z)
+{
+  if (m.is_ref) return m.n_parts;
+  for (int w = 0; w < nw; ++w) {
+    uint64_t bits = mask[w];
+    while (bits) {
+      int e = w * 64 + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      if (part_inside(m, m.parts[e], x, y, z)) return e;
+    }
+  }
+  return m.n_parts;
+}
+
+__device__ __forceinline__ void part_values(const MatDev &m, int e, double &d, double &b)
+{
+  if (e >= m.n_parts || m.parts[e].op != 0) {
+    d = 0.0;
+    b = 0.0;
+  } else {
+    d = m.parts[e].delta;
+    b = m.parts[e].beta;
+  }
+}
+
+__device__ __forceinline__ bool box_hits(const DBox &bx, double xlo, double xhi, double ylo,
+                                         double yhi, double zlo, double zhi)
+{
+  return bx.lo[0] <= xhi && bx.hi[0] >= xlo && bx.lo[1] <= yhi && bx.hi[1] >= ylo &&
+         bx.lo[2] <= zhi && bx.hi[2] >= zlo;
+}
+
+// Work-group culling: parts whose boxes meet the region are marked in the LDS
+// mask (nw words, cleared here).  Must be called by every lane.
+__device__ __forceinline__ void csg_cull(const MatDev &m, uint64_t *mask, int nw, double xlo,
+                                         double xhi, double ylo, double yhi, double zlo,
+                                         double zhi)
+{
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int w = tid; w < nw; w += nt) mask[w] = 0;
+  __syncthreads();
+  for (int e = tid; e < m.n_parts; e += nt)
+    if (box_hits(m.boxes[e], xlo, xhi, ylo, yhi, zlo, zhi))
+      atomicOr(reinterpret_cast<unsigned long long *>(&mask[e >> 6]), 1ull << (e & 63));
+  __syncthreads();
+}
+
+// Curved waveguide membership and material (src/curvedWaveGuide2D.cpp:49-52,
+// 179-205; src/waveGuideFDSimulation.cpp:219-265).
+__device__ __forceinline__ bool curved_inside(const DCurved &w, double x, double z)
+{
+#pragma clang fp contract(off)
+  return (2.0 * x * w.R + z * z > 0.0) && (2.0 * x * w.R + z * z < 2.0 * w.width * w.R);
+}
+
+__device__ __forceinline__ void curved_material(const DCurved &w, double x, double z, double &d,
+                                                double &b)
+{
+#pragma clang fp contract(off)
+  if (w.use_smoothed) {
+    double ww = 0.1 * w.width;
+    double dd = 2.0 * w.R * x + z * z;
+    double f1 = 1.0 / (1.0 + exp(-dd / ww));
+    double f2 = 1.0 / (1.0 + exp((2 * w.R * w.width - dd) / ww));
+    double f = 1.0 - (f1 - f2);
+    d = w.clad_delta * f;
+    b = w.clad_beta * f;
+    return;
+  }
+  if (z > w.wglength) {
+    d = 0.0;
+    b = 0.0;
+    return;
+  }
+  bool inside = curved_inside(w, x, z);
+  bool nb = curved_inside(w, x + w.dx, z) && curved_inside(w, x - w.dx, z) &&
+            curved_inside(w, x, z + w.dz) && curved_inside(w, x, z - w.dz);
+  if (inside && nb) {
+    d = w.in_delta;
+    b = w.in_beta;
+  } else if (inside) {
+    d = 0.5 * (w.clad_delta + w.in_delta);
+    b = 0.5 * (w.clad_beta + w.in_beta);
+  } else {
+    d = w.clad_delta;
+    b = w.clad_beta;
+  }
+}
+
+}  // namespace pxg

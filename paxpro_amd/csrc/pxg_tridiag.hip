@@ -1,0 +1,509 @@
+// Tridiagonal kernels: the Crank-Nicolson march (CrankNicholson::solveStep,
+// src/crancNicholson.cpp:17-148), the ADI half-steps (src/alternatingDirectionSolver.cpp)
+// and the batched form of ThomasAlgorithm::solve (include/thomasAlgorithm.tpp:2-32).
+//
+// CN: one work-group owns one field for the whole run (a persistent march over
+// all z-steps, no launch per step).  Each step builds the complex tridiagonal
+// system row by row in FP64 and solves it with a partitioned method: P lanes
+// each eliminate a contiguous chunk of m rows so that every interior unknown is
+// expressed through the chunk's first and last unknowns; those 2P unknowns form
+// a tridiagonal system solved by parallel cyclic reduction in LDS; the lanes
+// then recover their interiors.  The arithmetic is Gaussian elimination without
+// pivoting, as in the reference's Thomas sweep, reordered for parallelism.
+//
+// ADI: one lane per tridiagonal line.  The x-implicit pass walks x-lines whose
+// consecutive lines are consecutive in memory (one lane per system, interleaved
+// layout, coalesced); the y-implicit pass walks contiguous y-lines.  The global
+// Nx*Ny system of the reference has zero couplings between lines
+// (src/alternatingDirectionSolver.cpp:49-52,106-109), so per-line sweeps are the
+// same elimination.
+#include "pxg_tridiag.hpp"
+
+namespace pxg {
+
+constexpr int CN_MAX_P = 512;
+constexpr int TRI_MASK_WORDS = 16;
+
+CnPartition cn_partition(int n)
+{
+  int P = 1;
+  while (P * 2 <= CN_MAX_P && P * 2 <= n / 4) P *= 2;
+  int m = (n + P - 1) / P;
+  while (P > 1 && (P - 1) * m > n - 2) {
+    P /= 2;
+    m = (n + P - 1) / P;
+  }
+  return {P, m};
+}
+
+__host__ __device__ static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+static size_t cn_field_scratch(int n, int esz, const CnPartition &cp)
+{
+  return align256(static_cast<size_t>(n) * esz) + 3 * align256(static_cast<size_t>(cp.m) * cp.P * 16);
+}
+
+size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed)
+{
+  const int esz = mixed ? 8 : 16;
+  if (p.kind == PXG_CN) return cn_field_scratch(p.nx, esz, cn_partition(p.nx)) * p.batch;
+  if (p.kind == PXG_ADI) {
+    const size_t cells = static_cast<size_t>(p.nx) * p.nx;
+    return (align256(cells * esz) + 2 * align256(cells * 16)) * p.batch;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) { return a * b; }
+
+// Transparent-boundary factor exp(i kd) with kd = -i log(u_out/u_in) and
+// Re(kd) clamped at 0 (src/crancNicholson.cpp:131-141,
+// src/alternatingDirectionSolver.cpp:157-234).  Returns false when
+// |u_in| <= thresh, in which case the reference leaves the row unchanged.
+__device__ __forceinline__ bool tbc_factor(cx<double> u_out, cx<double> u_in, double thresh,
+                                           cx<double> &e)
+{
+  if (!(sqrt(norm2(u_in)) > thresh)) return false;
+  const cx<double> ratio = cdiv(u_out, u_in);
+  const double lr = log(hypot(ratio.x, ratio.y));
+  double kr = atan2(ratio.y, ratio.x);  // Re(kd) = arg(ratio)
+  const double ki = -lr;                 // Im(kd) = -log|ratio|
+  if (kr < 0.0) kr = 0.0;
+  const double amp = exp(-ki);
+  double s, c;
+  sincos(kr, &s, &c);
+  e = {amp * c, amp * s};
+  return true;
+}
+
+__device__ __forceinline__ void material_2d(const MatDev &m, const uint64_t *mask, int nw,
+                                            unsigned plane, int i, double x, double z, double &d,
+                                            double &b)
+{
+  if (m.mode == MAT_CSG) {
+    part_values(m, csg_eval(m, mask, nw, x, 0.0, z), d, b);
+  } else if (m.mode == MAT_CURVED) {
+    curved_material(m.wg, x, z, d, b);
+  } else if (m.mode == MAT_PLANES) {
+    const int p = m.z_invariant ? 0 : static_cast<int>(plane) - m.first_plane;
+    const double *pl = m.planes + static_cast<size_t>(p) * m.plane_stride;
+    d = pl[2 * i];
+    b = pl[2 * i + 1];
+  } else {
+    d = 0.0;
+    b = 0.0;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ cx<double> ld(const cx<R> *f, int i)
+{
+  return cvt<double>(f[i]);
+}
+
+// Row i of the CN system with the default ParaxialEquation (F=G=H=1, J=0):
+// diag = 1 + i rho/2 + (r/2)(beta + i delta), off-diagonals -i rho/4,
+// rhs = i rho/4 (u_{i-1} + u_{i+1}) - i rho/2 u_i + (1 - (r/2)(beta0 + i delta0)) u_i,
+// transparent boundary rows at i = 0 and N-1 (src/crancNicholson.cpp:30-89,107-148).
+template <class R>
+__device__ __forceinline__ void cn_row(const TriArgs<R> &a, const cx<R> *prev, int i, unsigned st,
+                                       double z, double rho, double r, const uint64_t *mask, int nw,
+                                       cx<double> &ai, cx<double> &bi, cx<double> &ci,
+                                       cx<double> &di)
+{
+  const int N = a.n;
+  const double x = coord(a.xmin, i, a.dx);
+  double d1, b1, d0, b0;
+  material_2d(a.mat, mask, nw, st, i, x, z, d1, b1);
+  material_2d(a.mat, mask, nw, st - 1, i, x, z - a.dz, d0, b0);
+  const cx<double> off = {0.0, -0.25 * rho};
+  ai = i > 0 ? off : cx<double>{0.0, 0.0};
+  ci = i < N - 1 ? off : cx<double>{0.0, 0.0};
+  bi = {1.0 + 0.5 * (b1 * r), 0.5 * rho + 0.5 * (d1 * r)};
+  const cx<double> u = ld(prev, i);
+  const cx<double> ul = i > 0 ? ld(prev, i - 1) : cx<double>{0.0, 0.0};
+  const cx<double> ur = i < N - 1 ? ld(prev, i + 1) : cx<double>{0.0, 0.0};
+  cx<double> v = (ul + ur) * cx<double>{0.0, 0.25 * rho};
+  v = v - cx<double>{-0.5 * rho * u.y, 0.5 * rho * u.x};
+  v = v + cx<double>{1.0 - 0.5 * (b0 * r), -0.5 * (d0 * r)} * u;
+  di = v;
+  if (a.bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
+    const int inner = i == 0 ? 1 : N - 2;
+    cx<double> e;
+    if (tbc_factor(u, ld(prev, inner), 1e-16, e)) {
+      const cx<double> q = cx<double>{0.0, 0.25 * rho} * e;
+      bi = bi - q;
+      di = di + q * u;
+    }
+  }
+}
+
+__device__ __forceinline__ double absorb_factor(int i, int N, unsigned npix, double inv)
+{
+  double f = 1.0;
+  if (i < static_cast<int>(npix)) f *= exp(-static_cast<double>(npix - i - 1) * inv);
+  const int j = N - 1 - i;
+  if (j < static_cast<int>(npix)) f *= exp(-static_cast<double>(npix - j - 1) * inv);
+  return f;
+}
+
+// ---------------------------------------------------------------- CN march
+template <class R>
+__global__ __launch_bounds__(CN_MAX_P) void k_cn_march(TriArgs<R> a, int P, int m, size_t per_field)
+{
+  __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  __shared__ uint64_t mask[TRI_MASK_WORDS];
+  const int p = threadIdx.x, b = blockIdx.x, N = a.n;
+  char *base = static_cast<char *>(a.scratch) + per_field * b;
+  cx<R> *bufA = a.field + static_cast<size_t>(b) * N;
+  cx<R> *bufB = reinterpret_cast<cx<R> *>(base);
+  const size_t cst = align256(static_cast<size_t>(m) * P * 16);
+  cx<double> *ca = reinterpret_cast<cx<double> *>(base + align256(static_cast<size_t>(N) * sizeof(cx<R>)));
+  cx<double> *cc = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(ca) + cst);
+  cx<double> *cd = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(cc) + cst);
+  const double k = a.kvec[b];
+  const double rho = a.dz / (k * a.dx * a.dx);
+  const double r = k * a.dz;
+  const int s = p * m;
+  const int e = min(N, s + m) - 1;
+  const int M = 2 * P;
+  const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
+
+  for (unsigned it = 0; it < a.nsteps; ++it) {
+    const unsigned st = a.first_step + it;
+    const cx<R> *prev = (it & 1) ? bufB : bufA;
+    cx<R> *next = (it & 1) ? bufA : bufB;
+    const double z = coord(a.zmin, st, a.dz);
+    if (a.mat.mode == MAT_CSG)
+      csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), 0.0, 0.0, z - a.dz, z);
+
+    // chunk elimination: rows s+1..e expressed through x_s (and x_{e+1})
+    cx<double> as, bs, cs, ds;
+    cn_row(a, prev, s, st, z, rho, r, mask, nw, as, bs, cs, ds);
+    cx<double> ah{}, ch{}, dh{};
+    for (int i = s + 1; i <= e; ++i) {
+      cx<double> ai, bi, ci, di;
+      cn_row(a, prev, i, st, z, rho, r, mask, nw, ai, bi, ci, di);
+      if (i == s + 1) {
+        ah = cdiv(ai, bi);
+        ch = cdiv(ci, bi);
+        dh = cdiv(di, bi);
+      } else {
+        const cx<double> den = bi - ai * ch;
+        const cx<double> ahn = cdiv(cx<double>{0.0, 0.0} - ai * ah, den);
+        ch = cdiv(ci, den);
+        dh = cdiv(di - ai * dh, den);
+        ah = ahn;
+      }
+      const int j = i - s;
+      ca[j * P + p] = ah;
+      cc[j * P + p] = ch;
+      cd[j * P + p] = dh;
+    }
+    // last row of the chunk is a reduced equation: ah x_s + x_e + ch x_{e+1} = dh
+    ra[2 * p + 1] = ah;
+    rb[2 * p + 1] = {1.0, 0.0};
+    rc[2 * p + 1] = ch;
+    rd[2 * p + 1] = dh;
+    // backward: x_i = dt_i - at_i x_s - ct_i x_e, base at i = e: (0, 0, -1)
+    cx<double> at{0.0, 0.0}, ct{-1.0, 0.0}, dt{0.0, 0.0};
+    for (int i = e - 1; i >= s + 1; --i) {
+      const int j = i - s;
+      const cx<double> a0 = ca[j * P + p], c0 = cc[j * P + p], d0 = cd[j * P + p];
+      dt = d0 - c0 * dt;
+      at = a0 - c0 * at;
+      ct = cx<double>{0.0, 0.0} - c0 * ct;
+      ca[j * P + p] = at;
+      cc[j * P + p] = ct;
+      cd[j * P + p] = dt;
+    }
+    // first row of the chunk: a_s x_{s-1} + (b_s - c_s at) x_s - c_s ct x_e = d_s - c_s dt
+    ra[2 * p] = as;
+    rb[2 * p] = bs - cs * at;
+    rc[2 * p] = cx<double>{0.0, 0.0} - cs * ct;
+    rd[2 * p] = ds - cs * dt;
+    __syncthreads();
+
+    // parallel cyclic reduction of the 2P-row reduced system
+    for (int h = 1; h < M; h *= 2) {
+      cx<double> na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = 2 * p + q;
+        cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
+        cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
+        cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
+        cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
+        if (row - h >= 0) {
+          k1 = cdiv(a_, rb[row - h]);
+          am = ra[row - h];
+          cm = rc[row - h];
+          dm = rd[row - h];
+        }
+        if (row + h < M) {
+          k2 = cdiv(c_, rb[row + h]);
+          ap = ra[row + h];
+          cp = rc[row + h];
+          dp = rd[row + h];
+        }
+        na[q] = cx<double>{0.0, 0.0} - am * k1;
+        nc[q] = cx<double>{0.0, 0.0} - cp * k2;
+        nb[q] = b_ - cm * k1 - ap * k2;
+        nd[q] = d_ - dm * k1 - dp * k2;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = 2 * p + q;
+        ra[row] = na[q];
+        rb[row] = nb[q];
+        rc[row] = nc[q];
+        rd[row] = nd[q];
+      }
+      __syncthreads();
+    }
+    const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
+    const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
+
+    cx<R> *sc = (a.sol && static_cast<int>(st) < a.nz)
+                    ? a.sol + b * a.sol_fstride + static_cast<size_t>(st) * N
+                    : nullptr;
+    for (int i = s; i <= e; ++i) {
+      cx<double> x;
+      if (i == s) x = xs;
+      else if (i == e) x = xe;
+      else {
+        const int j = i - s;
+        x = cd[j * P + p] - ca[j * P + p] * xs - cc[j * P + p] * xe;
+      }
+      if (a.absorb_npix) x = scale(x, absorb_factor(i, N, a.absorb_npix, a.absorb_inv));
+      const cx<R> xr = cvt<R>(x);
+      next[i] = xr;
+      if (sc) sc[i] = xr;
+    }
+    __syncthreads();
+  }
+  if (a.nsteps & 1)
+    for (int i = s; i <= e; ++i) bufA[i] = bufB[i];
+}
+
+template <class R>
+hipError_t launch_cn_march(const TriArgs<R> &a, hipStream_t s)
+{
+  const CnPartition cp = cn_partition(a.n);
+  const size_t per = cn_field_scratch(a.n, sizeof(cx<R>), cp);
+  hipLaunchKernelGGL(k_cn_march<R>, dim3(a.batch), dim3(cp.P), 0, s, a, cp.P, cp.m, per);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- ADI
+// One lane per line.  IMPX: line = row i (y), walking x (j); cell (i, j) at
+// j*N + i; material at z_step.  Otherwise line = column i (x), walking y (j);
+// cell (j, i) at i*N + j; material at z_step + dz/2.
+template <class R, bool IMPX>
+__global__ __launch_bounds__(256) void k_adi_line(TriArgs<R> a, const cx<R> *__restrict__ prev_all,
+                                                  cx<R> *__restrict__ next_all,
+                                                  cx<double> *__restrict__ work_all, size_t per_field)
+{
+  __shared__ uint64_t mask[TRI_MASK_WORDS];
+  const int N = a.n;
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  const size_t cells = static_cast<size_t>(N) * N;
+  const cx<R> *prev = prev_all + b * cells;
+  cx<R> *next = next_all + b * cells;
+  cx<double> *wsup = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(work_all) + per_field * b);
+  cx<double> *wrhs = wsup + cells;
+  const double k = a.kvec[b];
+  const double h = a.dz / 2.0;
+  const unsigned st = a.first_step;
+  const double z = IMPX ? coord(a.zmin, st, a.dz) : coord(a.zmin, st, a.dz) + h;
+  const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
+  if (a.mat.mode == MAT_CSG) {
+    const int l0 = blockIdx.x * blockDim.x, l1 = min(N - 1, l0 + static_cast<int>(blockDim.x) - 1);
+    if (IMPX)
+      csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), coord(a.ymin, l0, a.dy),
+               coord(a.ymin, l1, a.dy), z, z);
+    else
+      csg_cull(a.mat, mask, nw, coord(a.xmin, l0, a.dx), coord(a.xmin, l1, a.dx), a.ymin,
+               coord(a.ymin, N - 1, a.dy), z, z);
+  }
+  if (line >= N) return;
+  const double ddx = k * a.dx * a.dx, ddy = k * a.dy * a.dy;
+  // implicit-direction spacing term and explicit-direction spacing term
+  const double dimp = IMPX ? ddx : ddy;
+  const double dexp = IMPX ? ddy : ddx;
+  const cx<double> off = {0.0, -0.5 / dimp};
+  // cell access: (pos along line j, neighbour line offset o) -> element index
+  auto at = [&](int j, int lo) -> size_t {
+    const int l = line + lo;
+    return IMPX ? static_cast<size_t>(j) * N + l : static_cast<size_t>(l) * N + j;
+  };
+  const double *planes = nullptr;
+  if (a.mat.mode == MAT_PLANES) {
+    const int pi = a.mat.z_invariant ? 0 : static_cast<int>(2 * st + (IMPX ? 0 : 1)) - a.mat.first_plane;
+    planes = a.mat.planes + static_cast<size_t>(pi) * a.mat.plane_stride;
+  }
+  // explicit-direction boundary: this line is the first/last line
+  const bool edge_lo = line == 0, edge_hi = line == N - 1;
+
+  cx<double> supp{0.0, 0.0}, rhsp{0.0, 0.0};
+  for (int j = 0; j < N; ++j) {
+    const size_t idx = at(j, 0);
+    double d = 0.0, be = 0.0;
+    if (a.mat.mode == MAT_CSG) {
+      const double x = IMPX ? coord(a.xmin, j, a.dx) : coord(a.xmin, line, a.dx);
+      const double y = IMPX ? coord(a.ymin, line, a.dy) : coord(a.ymin, j, a.dy);
+      part_values(a.mat, csg_eval(a.mat, mask, nw, x, y, z), d, be);
+    } else if (planes) {
+      d = planes[2 * idx];
+      be = planes[2 * idx + 1];
+    }
+    cx<double> diag;
+    if (IMPX || (a.quirks & PXG_FIX_ADI_Y_DIAG)) {
+      diag = {1.0 / h + be * k, 1.0 / dimp + d * k};
+    } else {
+      diag = cx<double>{1.0 / h, 0.0} + cdiv(cx<double>{0.0, 1.0}, cx<double>{ddy + be * k, d * k});
+    }
+    const cx<double> u = ld(prev, static_cast<int>(idx));
+    cx<double> rhs = cx<double>{1.0 / h, -1.0 / dexp} * u;
+    if (line > 0) {
+      const cx<double> un = ld(prev, static_cast<int>(at(j, -1)));
+      rhs = rhs + cx<double>{-0.5 * un.y / dexp, 0.5 * un.x / dexp};
+    }
+    if (line < N - 1) {
+      const cx<double> un = ld(prev, static_cast<int>(at(j, +1)));
+      rhs = rhs + cx<double>{-0.5 * un.y / dexp, 0.5 * un.x / dexp};
+    }
+    if (a.use_tbc) {
+      cx<double> e;
+      // implicit direction: diagonal of the line ends
+      if (j == N - 1 && tbc_factor(u, ld(prev, static_cast<int>(at(N - 2, 0))), 1e-6, e))
+        diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
+      if (j == 0 && tbc_factor(u, ld(prev, static_cast<int>(at(1, 0))), 1e-6, e))
+        diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
+      // explicit direction: right-hand side of the first/last line
+      if (edge_hi && tbc_factor(u, ld(prev, static_cast<int>(at(j, -1))), 1e-6, e)) {
+        const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
+        rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
+      }
+      if (edge_lo && tbc_factor(u, ld(prev, static_cast<int>(at(j, +1))), 1e-6, e)) {
+        const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
+        rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
+      }
+    }
+    // Thomas forward sweep (sup = sub = off inside the line, 0 at its ends)
+    const cx<double> sup = j < N - 1 ? off : cx<double>{0.0, 0.0};
+    if (j == 0) {
+      supp = cdiv(sup, diag);
+      rhsp = cdiv(rhs, diag);
+    } else {
+      const cx<double> den = diag - off * supp;
+      rhsp = cdiv(rhs - off * rhsp, den);
+      supp = j < N - 1 ? cdiv(sup, den) : cx<double>{0.0, 0.0};
+    }
+    wsup[static_cast<size_t>(j) * N + line] = supp;
+    wrhs[static_cast<size_t>(j) * N + line] = rhsp;
+  }
+  cx<double> x = rhsp;
+  next[at(N - 1, 0)] = cvt<R>(x);
+  for (int j = N - 2; j >= 0; --j) {
+    x = wrhs[static_cast<size_t>(j) * N + line] - wsup[static_cast<size_t>(j) * N + line] * x;
+    next[at(j, 0)] = cvt<R>(x);
+  }
+}
+
+template <class R>
+hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s)
+{
+  const size_t cells = static_cast<size_t>(a.n) * a.n;
+  const size_t fbytes = align256(cells * sizeof(cx<R>));
+  const size_t per = fbytes + 2 * align256(cells * 16);
+  // scratch per field: [field copy][sup'][rhs']; work pointer of field 0 and stride per field
+  char *s0 = static_cast<char *>(a.scratch);
+  cx<R> *buf = reinterpret_cast<cx<R> *>(s0);  // field copies laid out with stride `per`
+  cx<double> *work = reinterpret_cast<cx<double> *>(s0 + fbytes);
+  const dim3 grid((a.n + 255) / 256, a.batch);
+  // field copies of different batch entries are `per` bytes apart; pass them
+  // through one contiguous-by-entry view when batch == 1, else per entry.
+  if (a.batch == 1) {
+    if (implicit_x)
+      hipLaunchKernelGGL((k_adi_line<R, true>), grid, dim3(256), 0, s, a, a.field, buf, work, per);
+    else
+      hipLaunchKernelGGL((k_adi_line<R, false>), grid, dim3(256), 0, s, a, buf, a.field, work, per);
+    return hipGetLastError();
+  }
+  for (int b = 0; b < a.batch; ++b) {
+    TriArgs<R> ab = a;
+    ab.batch = 1;
+    ab.field = a.field + b * cells;
+    ab.kvec = a.kvec + b;
+    cx<R> *bb = reinterpret_cast<cx<R> *>(s0 + per * b);
+    cx<double> *wb = reinterpret_cast<cx<double> *>(s0 + per * b + fbytes);
+    const dim3 g1((a.n + 255) / 256, 1);
+    if (implicit_x)
+      hipLaunchKernelGGL((k_adi_line<R, true>), g1, dim3(256), 0, s, ab, ab.field, bb, wb, per);
+    else
+      hipLaunchKernelGGL((k_adi_line<R, false>), g1, dim3(256), 0, s, ab, bb, ab.field, wb, per);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------- Thomas
+// One lane per system, the reference's sweep: sup = sub, sup[0] /= diag[0],
+// rhs[0] /= diag[0], then sup[i] /= (diag[i] - sub[i-1] sup[i-1]),
+// rhs[i] = (rhs[i] - sub[i-1] rhs[i-1]) / (same), back substitution into diag.
+__global__ void k_thomas(int layout, int nsys, int n, cx<double> *diag, const cx<double> *sub,
+                         cx<double> *rhs, cx<double> *work)
+{
+  const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sy >= nsys) return;
+  auto ix = [&](int i) -> size_t {
+    return layout == 0 ? static_cast<size_t>(sy) * n + i : static_cast<size_t>(i) * nsys + sy;
+  };
+  auto sx = [&](int i) -> size_t {
+    return layout == 0 ? static_cast<size_t>(sy) * (n - 1) + i : static_cast<size_t>(i) * nsys + sy;
+  };
+  if (n == 1) {
+    rhs[ix(0)] = cdiv(rhs[ix(0)], diag[ix(0)]);
+    diag[ix(0)] = rhs[ix(0)];
+    return;
+  }
+  cx<double> supp = cdiv(sub[sx(0)], diag[ix(0)]);
+  cx<double> rp = cdiv(rhs[ix(0)], diag[ix(0)]);
+  work[ix(0)] = supp;
+  rhs[ix(0)] = rp;
+  for (int i = 1; i < n; ++i) {
+    const cx<double> s1 = sub[sx(i - 1)];
+    const cx<double> den = diag[ix(i)] - s1 * supp;
+    if (i < n - 1) supp = cdiv(sub[sx(i)], den);
+    rp = cdiv(rhs[ix(i)] - s1 * rp, den);
+    work[ix(i)] = supp;
+    rhs[ix(i)] = rp;
+  }
+  cx<double> x = rp;
+  diag[ix(n - 1)] = x;
+  for (int i = n - 2; i >= 0; --i) {
+    x = rhs[ix(i)] - work[ix(i)] * x;
+    diag[ix(i)] = x;
+  }
+}
+
+hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
+                                 const cx<double> *sub, cx<double> *rhs, cx<double> *work,
+                                 hipStream_t s)
+{
+  hipLaunchKernelGGL(k_thomas, dim3((nsys + 127) / 128), dim3(128), 0, s, layout, nsys, n, diag,
+                     sub, rhs, work);
+  return hipGetLastError();
+}
+
+template hipError_t launch_cn_march<float>(const TriArgs<float> &, hipStream_t);
+template hipError_t launch_cn_march<double>(const TriArgs<double> &, hipStream_t);
+template hipError_t launch_adi_half<float>(const TriArgs<float> &, bool, hipStream_t);
+template hipError_t launch_adi_half<double>(const TriArgs<double> &, bool, hipStream_t);
+
+}  // namespace pxg

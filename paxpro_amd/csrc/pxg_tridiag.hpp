@@ -1,0 +1,48 @@
+// Tridiagonal family: Crank-Nicolson march, ADI half-steps and the batched
+// Thomas entry point.  Internal to libpaxgpu.
+#pragma once
+
+#include "../../include/paxgpu.h"
+#include "pxg_internal.hpp"
+
+namespace pxg {
+
+template <class R>
+struct TriArgs {
+  cx<R> *field;          // batch of fields (CN: nx each; ADI: nx*nx each)
+  int n;                 // nx
+  int batch;
+  const double *kvec;    // [batch]
+  MatDev mat;
+  double xmin, dx, ymin, dy, zmin, dz;
+  int bc;                // CN boundary condition
+  int use_tbc;           // ADI transparent boundary
+  unsigned quirks;
+  unsigned absorb_npix;  // CN edge absorber extension (0 = off)
+  double absorb_inv;
+  void *scratch;         // library-owned work space (tridiag_scratch_bytes)
+  cx<R> *sol;            // CN: solution columns (nx x nz per field) or null
+  size_t sol_fstride;
+  int nz;
+  unsigned first_step, nsteps;
+};
+
+size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
+
+template <class R>
+hipError_t launch_cn_march(const TriArgs<R> &a, hipStream_t s);
+template <class R>
+hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
+hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
+                                 const cx<double> *sub, cx<double> *rhs, cx<double> *work,
+                                 hipStream_t s);
+
+// CN partition: P chunks of m rows (the last chunk may be shorter but keeps
+// at least two rows); P is a power of two so the reduced system of 2P rows
+// suits parallel cyclic reduction.
+struct CnPartition {
+  int P, m;
+};
+CnPartition cn_partition(int n);
+
+}  // namespace pxg

@@ -1,0 +1,266 @@
+"""ctypes binding of libpaxgpu.so (include/paxgpu.h).
+
+Fields cross the boundary as numpy complex128 arrays in the reference's
+Armadillo column-major layout: a 3D field is an (Ny, Nx) Fortran-ordered
+matrix (rows = y), a 2D field is a length-Nx vector.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpaxgpu.so")
+_LIB = None
+
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_BUDGET, ERR_DEVICE, ERR_STATE, ERR_CANCELLED, ERR_INJECTED = range(8)
+FFT2D, FFT3D, CN, ADI = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+BC_DIRICHLET, BC_TRANSPARENT = 0, 1
+FIX_ADI_Y_DIAG, FIX_REFR_SAMPLES = 1, 2
+LAYOUT_CONTIGUOUS, LAYOUT_INTERLEAVED = 0, 1
+
+SHAPE_DTYPE = np.dtype([("type", np.int32), ("op", np.int32), ("T", np.float64, 16),
+                        ("p", np.float64, 3)])
+PART_DTYPE = np.dtype([("first_shape", np.int32), ("n_shapes", np.int32), ("op", np.int32),
+                       ("_pad", np.int32), ("delta", np.float64), ("beta", np.float64)])
+MODULE_DTYPE = np.dtype([("first_part", np.int32), ("n_parts", np.int32)])
+
+EXPORTS = [
+    "pxg_ctx_create", "pxg_ctx_destroy", "pxg_ctx_reserved", "pxg_last_error", "pxg_version",
+    "pxg_plan_create", "pxg_plan_destroy", "pxg_plan_bytes", "pxg_set_wavenumbers",
+    "pxg_set_field", "pxg_get_field", "pxg_get_solution", "pxg_solution_dims",
+    "pxg_set_material_csg", "pxg_set_material_curved_wg", "pxg_set_material_planes",
+    "pxg_set_material_none", "pxg_step", "pxg_sync", "pxg_step_profiled",
+    "pxg_field_device_ptr", "pxg_thomas_batched",
+]
+
+
+class PxgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"paxgpu error {code}: {msg}")
+        self.code = code
+
+
+class Problem(C.Structure):
+    _fields_ = [("kind", C.c_int), ("precision", C.c_int), ("nx", C.c_int), ("nz", C.c_int),
+                ("batch", C.c_int), ("xmin", C.c_double), ("dx", C.c_double),
+                ("ymin", C.c_double), ("dy", C.c_double), ("zmin", C.c_double),
+                ("dz", C.c_double), ("k", C.c_double), ("bc", C.c_int), ("use_tbc", C.c_int),
+                ("absorb_npix_x", C.c_uint), ("absorb_npix_y", C.c_uint),
+                ("absorb_inv_x", C.c_double), ("absorb_inv_y", C.c_double),
+                ("quirks", C.c_uint), ("n_refr", C.c_uint), ("store_dsx", C.c_int),
+                ("store_dsz", C.c_int)]
+
+
+class CurvedWG(C.Structure):
+    _fields_ = [("R", C.c_double), ("width", C.c_double), ("clad_delta", C.c_double),
+                ("clad_beta", C.c_double), ("in_delta", C.c_double), ("in_beta", C.c_double),
+                ("wglength", C.c_double), ("dx", C.c_double), ("dz", C.c_double),
+                ("use_smoothed", C.c_int)]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("mean_ms", C.c_double), ("launches", C.c_int),
+                ("bytes_per_launch", C.c_double)]
+
+
+def lib():
+    """Load libpaxgpu.so; raises if it has not been built (no CPU fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C paxpro_amd`")
+        L = C.CDLL(LIB_PATH)
+        P, I, U, D, S = C.c_void_p, C.c_int, C.c_uint, C.c_double, C.c_size_t
+        L.pxg_last_error.restype = C.c_char_p
+        L.pxg_version.restype = C.c_char_p
+        L.pxg_ctx_create.argtypes = [I, S, P]
+        L.pxg_ctx_destroy.argtypes = [P]
+        L.pxg_ctx_reserved.restype = S
+        L.pxg_ctx_reserved.argtypes = [P]
+        L.pxg_plan_create.argtypes = [P, P, P]
+        L.pxg_plan_destroy.argtypes = [P]
+        L.pxg_plan_bytes.argtypes = [P, P]
+        L.pxg_set_wavenumbers.argtypes = [P, P, I]
+        L.pxg_set_field.argtypes = [P, I, P]
+        L.pxg_get_field.argtypes = [P, I, P]
+        L.pxg_get_solution.argtypes = [P, I, P, S]
+        L.pxg_solution_dims.argtypes = [P, P]
+        L.pxg_set_material_csg.argtypes = [P, P, I, P, I, P, I, I]
+        L.pxg_set_material_curved_wg.argtypes = [P, P]
+        L.pxg_set_material_planes.argtypes = [P, P, I, I, I]
+        L.pxg_set_material_none.argtypes = [P]
+        L.pxg_step.argtypes = [P, U, U]
+        L.pxg_sync.argtypes = [P]
+        L.pxg_step_profiled.argtypes = [P, U, U, P, I, P, P]
+        L.pxg_field_device_ptr.argtypes = [P, I, P]
+        L.pxg_thomas_batched.argtypes = [P, I, I, I, P, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != OK:
+        raise PxgError(rc, lib().pxg_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def version():
+    return lib().pxg_version().decode()
+
+
+def problem(kind, nx, nz, dx, dz, k, xmin=0.0, ymin=None, dy=None, zmin=0.0, precision=PREC_FP64,
+            batch=1, bc=BC_TRANSPARENT, use_tbc=1, absorb=(0, 0, 0.0, 0.0), quirks=0, n_refr=10,
+            store_dsx=0, store_dsz=0):
+    if ymin is None:
+        ymin = xmin
+    if dy is None:
+        dy = dx
+    return Problem(kind, precision, nx, nz, batch, xmin, dx, ymin, dy, zmin, dz, k, bc, use_tbc,
+                   absorb[0], absorb[1], absorb[2], absorb[3], quirks, n_refr, store_dsx,
+                   store_dsz)
+
+
+class Ctx:
+    def __init__(self, device=0, budget=0):
+        self.h = C.c_void_p()
+        _check(lib().pxg_ctx_create(device, budget, C.byref(self.h)))
+
+    def reserved(self):
+        return lib().pxg_ctx_reserved(self.h)
+
+    def thomas_batched(self, diag, sub, rhs, layout=LAYOUT_CONTIGUOUS):
+        """ThomasAlgorithm::solve for nsys systems: arrays (nsys, n) / (nsys, n-1)
+        complex128 for the contiguous layout, (n, nsys) / (n-1, nsys) interleaved.
+        Returns (solution, clobbered rhs)."""
+        d = np.ascontiguousarray(diag, dtype=np.complex128).copy()
+        s = np.ascontiguousarray(sub, dtype=np.complex128)
+        r = np.ascontiguousarray(rhs, dtype=np.complex128).copy()
+        if layout == LAYOUT_CONTIGUOUS:
+            nsys, n = d.shape
+        else:
+            n, nsys = d.shape
+        _check(lib().pxg_thomas_batched(self.h, layout, nsys, n, _p(d), _p(s), _p(r)))
+        return d, r
+
+    def close(self):
+        if self.h:
+            _check(lib().pxg_ctx_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_bytes(prob):
+    n = C.c_size_t()
+    _check(lib().pxg_plan_bytes(C.byref(prob), C.byref(n)))
+    return n.value
+
+
+class Plan:
+    def __init__(self, ctx, prob):
+        self.ctx = ctx
+        self.prob = prob
+        self.h = C.c_void_p()
+        _check(lib().pxg_plan_create(ctx.h, C.byref(prob), C.byref(self.h)))
+        self.is3d = prob.kind in (FFT3D, ADI)
+        self.nx = prob.nx
+
+    def shape(self):
+        return (self.nx, self.nx) if self.is3d else (self.nx,)
+
+    def set_field(self, field, b=0):
+        order = "F" if self.is3d else "C"
+        f = np.asarray(field, dtype=np.complex128)
+        if f.shape != self.shape():
+            raise ValueError(f"field shape {f.shape} != {self.shape()}")
+        f = np.require(f, requirements=[order[0] if order == "F" else "C"]).copy(order=order)
+        _check(lib().pxg_set_field(self.h, b, _p(f)))
+
+    def get_field(self, b=0):
+        out = np.empty(self.shape(), dtype=np.complex128, order="F" if self.is3d else "C")
+        _check(lib().pxg_get_field(self.h, b, _p(out)))
+        return out
+
+    def set_wavenumbers(self, ks):
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        _check(lib().pxg_set_wavenumbers(self.h, _p(ks), ks.size))
+
+    def material_none(self):
+        _check(lib().pxg_set_material_none(self.h))
+
+    def material_csg(self, shapes, parts, modules, reference_run=False):
+        self._csg = [np.ascontiguousarray(shapes, SHAPE_DTYPE), np.ascontiguousarray(parts, PART_DTYPE),
+                     np.ascontiguousarray(modules, MODULE_DTYPE)]
+        s, p, m = self._csg
+        _check(lib().pxg_set_material_csg(self.h, _p(s), len(s), _p(p), len(p), _p(m), len(m),
+                                          int(reference_run)))
+
+    def material_curved(self, **kw):
+        d = dict(in_delta=0.0, in_beta=0.0, wglength=1e10, use_smoothed=0)
+        d.update(kw)
+        wg = CurvedWG(**d)
+        _check(lib().pxg_set_material_curved_wg(self.h, C.byref(wg)))
+
+    def material_planes(self, db, first_plane=0, z_invariant=False):
+        """db: (n_planes, cells, 2) float64, cells in the field's memory order."""
+        db = np.ascontiguousarray(db, dtype=np.float64)
+        _check(lib().pxg_set_material_planes(self.h, _p(db), db.shape[0], first_plane,
+                                             int(z_invariant)))
+
+    def step(self, first, nsteps):
+        _check(lib().pxg_step(self.h, first, nsteps))
+
+    def sync(self):
+        _check(lib().pxg_sync(self.h))
+
+    def step_profiled(self, first, nsteps, max_stats=16):
+        stats = (KernelStat * max_stats)()
+        n = C.c_int()
+        total = C.c_double()
+        _check(lib().pxg_step_profiled(self.h, first, nsteps, stats, max_stats, C.byref(n),
+                                       C.byref(total)))
+        out = {}
+        for i in range(n.value):
+            s = stats[i]
+            out[s.name.decode()] = dict(mean_ms=s.mean_ms, launches=s.launches,
+                                        bytes_per_launch=s.bytes_per_launch)
+        return total.value, out
+
+    def solution_dims(self):
+        d = (C.c_int * 3)()
+        _check(lib().pxg_solution_dims(self.h, d))
+        return tuple(d)
+
+    def get_solution(self, b=0):
+        d = self.solution_dims()
+        if self.is3d:
+            out = np.empty(d, dtype=np.complex128, order="F")
+        else:
+            out = np.empty((d[0], d[1]), dtype=np.complex128, order="F")
+        _check(lib().pxg_get_solution(self.h, b, _p(out), out.size))
+        return out
+
+    def device_ptr(self, b=0):
+        p = C.c_void_p()
+        _check(lib().pxg_field_device_ptr(self.h, b, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            _check(lib().pxg_plan_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

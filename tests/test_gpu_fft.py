@@ -1,0 +1,176 @@
+"""GPU parity of the split-step Fourier path (FFTSolver3D / FFTSolver2D) against
+the CPU FP64 oracle, through the C ABI.
+
+Tolerances: the FP64 path must agree with the oracle to 1e-11 relative L2 on
+the exit field (both are FP64 transforms with different but equally accurate
+rounding); the mixed path (complex64 fields, FP64 phase arguments) to 1e-5 on
+the field and 2e-5 on the intensity, the bound stated for this path.
+"""
+import numpy as np
+import pytest
+
+import paxpro_amd as P
+import pyoracle as O
+from gpu_common import announce, fft3d_problem, rel_l2, rel_l2_int
+from scenes import K_01, fft3d_case, gaussian_field, random_spheres, spheres_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = P.Ctx(0)
+    yield c
+    c.close()
+
+
+def _run_gpu(ctx, prob, field, nsteps, csg=None, first=1, b_fields=None):
+    plan = P.Plan(ctx, prob)
+    if csg is not None:
+        plan.material_csg(*csg)
+    fields = b_fields if b_fields is not None else [field]
+    for b, f in enumerate(fields):
+        plan.set_field(f, b)
+    plan.step(first, nsteps)
+    plan.sync()
+    out = [plan.get_field(b) for b in range(len(fields))]
+    return plan, out
+
+
+@pytest.mark.parametrize("n,nsteps", [(64, 20), (128, 12), (256, 6)])
+def test_fft3d_fp64_sphere_vs_oracle(ctx, n, nsteps):
+    announce("fft3d fp64 sphere", n, nsteps)
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)])
+    ref = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps)
+    _, (out,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1), u0, nsteps, csg)
+    err = rel_l2(out, ref)
+    announce("rel_l2", err)
+    assert err < 1e-11
+
+
+def test_fft3d_fp64_many_spheres_crossing_and_absorber(ctx):
+    n, nsteps = 128, 10
+    announce("fft3d fp64 random spheres + absorber", n, nsteps)
+    g = fft3d_case(n, dz=200.0, nsteps=nsteps)
+    sph = random_spheres(0x5EED0002, 24, (-8000, 8000), (-1500, 1500), (500, 3000), (1e-6, 1e-5))
+    csg = spheres_scene(sph)
+    ab = O.Absorber3D(6, 4, 0.3, 0.2)
+    u0 = gaussian_field(g, n, 5000.0)
+    ref = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, absorber=ab)
+    _, (out,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1, absorber=ab), u0, nsteps, csg)
+    err = rel_l2(out, ref)
+    announce("rel_l2", err)
+    assert err < 1e-11
+    # the quirk switch changes the crossing cells
+    ref_fix = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, absorber=ab, quirks=O.FIX_REFR_SAMPLES)
+    _, (out_fix,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1, absorber=ab, quirks=P.FIX_REFR_SAMPLES),
+                             u0, nsteps, csg)
+    assert rel_l2(out_fix, ref_fix) < 1e-11
+    assert rel_l2(ref_fix, ref) > 1e-9
+
+
+def test_fft3d_mixed_vs_fp64(ctx):
+    n, nsteps = 256, 200
+    announce("fft3d mixed vs fp64", n, nsteps)
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)])
+    _, (ref,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1), u0, nsteps, csg)
+    _, (mix,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1, precision=P.PREC_MIXED), u0, nsteps, csg)
+    ef, ei = rel_l2(mix, ref), rel_l2_int(mix, ref)
+    announce("mixed rel_l2 field", ef, "intensity", ei)
+    assert ef < 1e-5 and ei < 2e-5
+
+
+def test_fft3d_batched_equals_single(ctx):
+    """Three fields with different wavenumbers in one plan equal three single runs."""
+    n, nsteps = 128, 8
+    announce("fft3d batch of 3", n, nsteps)
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    csg = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)])
+    ks = [K_01, 2 * np.pi / 0.08, 2 * np.pi / 0.15]
+    fields = [gaussian_field(g, n, w) for w in (3000.0, 2500.0, 4000.0)]
+    for prec, tol in ((P.PREC_FP64, 1e-13), (P.PREC_MIXED, 1e-6)):
+        plan = P.Plan(ctx, fft3d_problem(g, n, nsteps + 1, precision=prec, batch=3))
+        plan.set_wavenumbers(ks)
+        plan.material_csg(*csg)
+        for b, f in enumerate(fields):
+            plan.set_field(f, b)
+        plan.step(1, nsteps)
+        outs = [plan.get_field(b) for b in range(3)]
+        for b in range(3):
+            gb = fft3d_case(n, dz=10.0, nsteps=nsteps, k=ks[b])
+            _, (single,) = _run_gpu(ctx, fft3d_problem(gb, n, nsteps + 1, precision=prec), fields[b],
+                                    nsteps, csg)
+            assert rel_l2(outs[b], single) < tol
+            if prec == P.PREC_FP64:
+                ref = O.fft3d_run(gb, O.CSGMaterial(*csg), fields[b], 1, nsteps)
+                assert rel_l2(outs[b], ref) < 1e-11
+
+
+def test_fft3d_split_calls_and_cube_storage(ctx):
+    """Two pxg_step calls equal one; the block-averaged cube matches the oracle's."""
+    n, nsteps, ds, dsz = 64, 12, 8, 4
+    announce("fft3d cube storage", n, nsteps)
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)])
+    slices = (nsteps + 1) // dsz + 1
+    cube = np.zeros((n // ds, n // ds, slices), complex, order="F")
+    O.cube_store(u0, cube, dsz, 0)
+    ref = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, cube=cube, dsz=dsz)
+    plan = P.Plan(ctx, fft3d_problem(g, n, nsteps + 1, store=(ds, dsz)))
+    plan.material_csg(*csg)
+    plan.set_field(u0)
+    plan.step(1, 5)
+    plan.step(6, nsteps - 5)
+    assert rel_l2(plan.get_field(), ref) < 1e-11
+    sol = plan.get_solution()
+    assert sol.shape == cube.shape
+    assert rel_l2(sol, cube) < 1e-11
+
+
+def test_fft3d_vacuum_and_unsupported(ctx):
+    n = 32
+    g = fft3d_case(n, dz=10.0, nsteps=4)
+    u0 = gaussian_field(g, n, 3000.0)
+    ref = O.fft3d_run(g, O.NoMaterial, u0, 1, 4)
+    _, (out,) = _run_gpu(ctx, fft3d_problem(g, n, 5), u0, 4)
+    assert rel_l2(out, ref) < 1e-12
+    with pytest.raises(P.PxgError) as e:
+        P.Plan(ctx, P.problem(P.FFT3D, 48, 5, 1.0, 1.0, 1.0))
+    assert e.value.code == P.ERR_UNSUPPORTED
+    plan = P.Plan(ctx, fft3d_problem(g, n, 5))
+    with pytest.raises(P.PxgError) as e:
+        plan.step(1, 1)
+    assert e.value.code == P.ERR_STATE
+
+
+@pytest.mark.parametrize("material", ["curved", "csg"])
+def test_fft2d_vs_oracle(ctx, material):
+    nx, nsteps = 1024, 50
+    announce("fft2d", material, nx, nsteps)
+    k = K_01
+    xmin, dx, dz = -600.0, 800.0 / (nx - 1), 100.0
+    g = O.grid(xmin, xmin + (nx - 1) * dx, dx, 0.0, nsteps * dz, dz, k)
+    x = xmin + dx * np.arange(nx)
+    u0 = np.exp(-((x - 50) / 80.0) ** 2).astype(complex)
+    prob = P.problem(P.FFT2D, nx, nsteps + 1, dx, dz, k, xmin=xmin, store_dsx=1, store_dsz=1)
+    plan = P.Plan(ctx, prob)
+    if material == "curved":
+        kw = dict(R=2e6, width=100.0, clad_delta=4.5e-5, clad_beta=3.5e-6, dx=dx, dz=dz)
+        m = O.CurvedMaterial(**kw)
+        plan.material_curved(**kw)
+    else:
+        csg = spheres_scene([(0.0, 0.0, 2000.0, 150.0, 4e-6, 1e-7), (-200.0, 0.0, 3000.0, 100.0, 8e-6, 3e-7)])
+        m = O.CSGMaterial(*csg)
+        plan.material_csg(*csg)
+    sol_ref = np.zeros((nx, nsteps + 1), complex, order="F")
+    sol_ref[:, 0] = u0
+    ref = O.fft2d_run(g, m, u0, 1, nsteps, solution=sol_ref)
+    plan.set_field(u0)
+    plan.step(1, nsteps)
+    assert rel_l2(plan.get_field(), ref) < 1e-11
+    assert rel_l2(plan.get_solution(), sol_ref) < 1e-11

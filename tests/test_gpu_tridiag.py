@@ -1,0 +1,160 @@
+"""GPU parity of the tridiagonal family (CrankNicholson, ADI, batched Thomas)
+against the CPU FP64 oracle through the C ABI.
+
+Tolerances: FP64 paths 1e-10 relative L2 (partitioned elimination and PCR
+reorder the reference's serial Thomas sweep; both are FP64); the mixed CN path
+(complex64 storage, FP64 sweeps) 1e-5 on field and intensity.
+"""
+import numpy as np
+import pytest
+
+import paxpro_amd as P
+import pyoracle as O
+from gpu_common import announce, rel_l2, rel_l2_int
+from scenes import K_01, random_spheres, spheres_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = P.Ctx(0)
+    yield c
+    c.close()
+
+
+def slab_planes(x, lo, hi, core=(0.0, 0.0), clad=(4.5e-5, 3.5e-6)):
+    inside = (x < hi) & (x > lo)
+    db = np.empty((1, x.size, 2))
+    db[0, :, 0] = np.where(inside, core[0], clad[0])
+    db[0, :, 1] = np.where(inside, core[1], clad[1])
+    return db
+
+
+@pytest.mark.parametrize("nx,nsteps,absorb", [(1024, 200, 0), (300, 100, 50), (5, 20, 0), (2, 10, 0)])
+def test_cn_slab_vs_oracle(ctx, nx, nsteps, absorb):
+    """Config-0 style slab waveguide (host-sampled material), plane wave, TBC."""
+    announce("cn slab", nx, nsteps, absorb)
+    k, dz = K_01, 500.0
+    xmin, dx = -500.0, 1000.0 / (nx - 1)
+    g = O.grid(xmin, xmin + (nx - 1) * dx, dx, 0.0, nsteps * dz, dz, k)
+    x = xmin + dx * np.arange(nx)
+    mat = O.SlabMaterial(-50.0, 50.0, 0.0, 0.0, 4.5e-5, 3.5e-6)
+    u0 = np.ones(nx, complex)
+    sol_ref = np.zeros((nx, nsteps + 1), complex, order="F")
+    sol_ref[:, 0] = u0
+    ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol_ref,
+                   absorb_npix=absorb, absorb_inv=0.1)
+    prob = P.problem(P.CN, nx, nsteps + 1, dx, dz, k, xmin=xmin, bc=P.BC_TRANSPARENT,
+                     absorb=(absorb, 0, 0.1, 0.0), store_dsx=1, store_dsz=1)
+    plan = P.Plan(ctx, prob)
+    plan.material_planes(slab_planes(x, -50.0, 50.0), z_invariant=True)
+    plan.set_field(u0)
+    plan.step(1, nsteps)
+    out = plan.get_field()
+    err = rel_l2(out, ref)
+    announce("rel_l2", err)
+    assert err < 1e-10
+    assert rel_l2(plan.get_solution(), sol_ref) < 1e-10
+
+
+def test_cn_curved_waveguide_and_dirichlet(ctx):
+    nx, nsteps, dz = 777, 150, 100.0
+    announce("cn curved", nx, nsteps)
+    k = 2 * np.pi / 0.13
+    R, w = 5e6, 120.0
+    L = nsteps * dz
+    lo = -L * L / (2 * R) - 2 * w - 200
+    hi = w + 200
+    dx = (hi - lo) / (nx - 1)
+    g = O.grid(lo, hi, dx, 0.0, L, dz, k)
+    kw = dict(R=R, width=w, clad_delta=4.5e-5, clad_beta=3.5e-6, dx=dx, dz=dz)
+    u0 = np.ones(nx, complex)
+    for bc in (O.BC_TRANSPARENT, O.BC_DIRICHLET):
+        ref = O.cn_run(g, O.CurvedMaterial(**kw), u0, 1, nsteps, bc=bc)
+        plan = P.Plan(ctx, P.problem(P.CN, nx, nsteps + 1, dx, dz, k, xmin=lo, bc=bc))
+        plan.material_curved(**kw)
+        plan.set_field(u0)
+        plan.step(1, 60)
+        plan.step(61, nsteps - 60)
+        assert rel_l2(plan.get_field(), ref) < 1e-10
+
+
+def test_cn_mixed_precision(ctx):
+    nx, nsteps = 1024, 400
+    announce("cn mixed", nx, nsteps)
+    k, dz = K_01, 500.0
+    xmin, dx = -500.0, 1000.0 / (nx - 1)
+    x = xmin + dx * np.arange(nx)
+    out = []
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        plan = P.Plan(ctx, P.problem(P.CN, nx, nsteps + 1, dx, dz, k, xmin=xmin, precision=prec))
+        plan.material_planes(slab_planes(x, -50.0, 50.0), z_invariant=True)
+        plan.set_field(np.ones(nx, complex))
+        plan.step(1, nsteps)
+        out.append(plan.get_field())
+    ef, ei = rel_l2(out[1], out[0]), rel_l2_int(out[1], out[0])
+    announce("cn mixed rel_l2", ef, ei)
+    assert ef < 1e-5 and ei < 1e-5
+
+
+def test_cn_batched_uneven_k(ctx):
+    """A batch of CN fields with different wavenumbers equals single runs."""
+    nx, nsteps = 512, 50
+    dz, xmin, dx = 500.0, -500.0, 1000.0 / 511
+    x = xmin + dx * np.arange(nx)
+    ks = np.array([K_01, 2 * np.pi / 0.05, 2 * np.pi / 0.2, 2 * np.pi / 0.12])
+    plan = P.Plan(ctx, P.problem(P.CN, nx, nsteps + 1, dx, dz, K_01, xmin=xmin, batch=4))
+    plan.set_wavenumbers(ks)
+    plan.material_planes(slab_planes(x, -50.0, 50.0), z_invariant=True)
+    for b in range(4):
+        plan.set_field(np.ones(nx, complex), b)
+    plan.step(1, nsteps)
+    for b in range(4):
+        g = O.grid(xmin, xmin + (nx - 1) * dx, dx, 0.0, nsteps * dz, dz, ks[b])
+        ref = O.cn_run(g, O.SlabMaterial(-50.0, 50.0, 0.0, 0.0, 4.5e-5, 3.5e-6), np.ones(nx, complex), 1, nsteps)
+        assert rel_l2(plan.get_field(b), ref) < 1e-10
+
+
+@pytest.mark.parametrize("quirks", [0, P.FIX_ADI_Y_DIAG])
+def test_adi_vs_oracle(ctx, quirks):
+    n, nsteps = 64, 4
+    announce("adi", n, nsteps, quirks)
+    k, dz = K_01, 20.0
+    dx = 4000.0 / n
+    xmin = -2000.0
+    g = O.grid(xmin, xmin + (n - 1) * dx, dx, -40.0, 40.0, dz, k)
+    sph = random_spheres(0x5EED0002, 6, (-1500, 1500), (-40, 40), (300, 900), (1e-6, 1e-5))
+    csg = spheres_scene(sph)
+    xs = xmin + dx * np.arange(n)
+    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + xs[None, :] ** 2) / 800.0 ** 2).astype(complex))
+    for use_tbc in (True, False):
+        ref = O.adi_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, use_tbc=use_tbc, quirks=quirks)
+        plan = P.Plan(ctx, P.problem(P.ADI, n, nsteps + 1, dx, dz, k, xmin=xmin, zmin=-40.0,
+                                     use_tbc=int(use_tbc), quirks=quirks))
+        plan.material_csg(*csg)
+        plan.set_field(u0)
+        plan.step(1, nsteps)
+        err = rel_l2(plan.get_field(), ref)
+        announce("adi rel_l2", use_tbc, err)
+        assert err < 1e-10
+
+
+def test_thomas_batched_matches_reference_fixtures(ctx):
+    g = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                           "thomas_golden.npz"))
+    for key in g["cases"]:
+        d, s, r = g[f"diag_{key}"], g[f"sub_{key}"], g[f"rhs_{key}"]
+        x, _ = ctx.thomas_batched(d[None, :], s[None, :], r[None, :])
+        assert rel_l2(x[0], g[f"x_{key}"]) < 1e-12
+    rng = np.random.default_rng(3)
+    nsys, n = 300, 257
+    D = 4 + rng.standard_normal((nsys, n)) + 1j * rng.standard_normal((nsys, n))
+    S = rng.standard_normal((nsys, n - 1)) + 1j * rng.standard_normal((nsys, n - 1))
+    B = rng.standard_normal((nsys, n)) + 1j * rng.standard_normal((nsys, n))
+    xc, _ = ctx.thomas_batched(D, S, B)
+    xi, _ = ctx.thomas_batched(D.T.copy(), S.T.copy(), B.T.copy(), layout=P.LAYOUT_INTERLEAVED)
+    for i in (0, 17, 299):
+        ref = O.thomas(D[i], S[i], B[i])
+        assert rel_l2(xc[i], ref) < 1e-12
+        assert rel_l2(xi[:, i], ref) < 1e-12
