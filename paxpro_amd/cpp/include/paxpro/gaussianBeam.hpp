@@ -1,0 +1,3 @@
+// gaussianBeam (include/gaussianBeam.hpp of the reference) lives in paraxialSource.hpp.
+#pragma once
+#include "paraxialSource.hpp"

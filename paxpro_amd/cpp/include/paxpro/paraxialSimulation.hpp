@@ -1,0 +1,70 @@
+// ParaxialSimulation (include/paraxialSimulation.hpp:20-236 of the reference):
+// grid, wavenumber, material dispatch and the initial field; HDF5 output and
+// post-processing modules are not part of this build.
+#pragma once
+
+#include <complex>
+#include <string>
+
+#include "arma_lite.hpp"
+#include "materialFunction.hpp"
+
+class Solver;
+class ParaxialSource;
+
+struct Disctretization {
+  double min{0.0};
+  double max{0.0};
+  double step{1.0};
+  unsigned int downsamplingRatio{1};
+};
+typedef std::complex<double> cdouble;
+
+class ParaxialSimulation {
+ public:
+  explicit ParaxialSimulation(const char *name);
+  ParaxialSimulation(const ParaxialSimulation &) = delete;
+  ParaxialSimulation &operator=(const ParaxialSimulation) = delete;
+  virtual ~ParaxialSimulation();
+
+  void setTransverseDiscretization(double xmin, double xmax, double step);
+  void setTransverseDiscretization(double xmin, double xmax, double step, unsigned int ratio);
+  void setLongitudinalDiscretization(double zmin, double zmax, double step);
+  void setLongitudinalDiscretization(double zmin, double zmax, double step, unsigned int ratio);
+  void setVerticalDiscretization(double ymin, double ymax, double step);
+  unsigned int nodeNumberTransverse() const;
+  unsigned int nodeNumberLongitudinal() const;
+  unsigned int nodeNumberVertical() const;
+  const Disctretization &transverseDiscretization() const { return xDisc; }
+  const Disctretization &longitudinalDiscretization() const { return zDisc; }
+  const Disctretization &verticalDiscretization() const { return yDisc; }
+  double getWavenumber() const { return wavenumber; }
+  double getWavelength() const;
+  double getEnergy() const;
+  void setWavenumber(double k) { wavenumber = k; }
+  void setWaveLength(double lambda);
+  void setSolver(Solver &solv);
+  std::string getName() const { return name; }
+  void step();
+  double getZ(unsigned int iz) const;
+  double getX(int ix) const;
+  double getY(int iy) const;
+  const Solver &getSolver() const { return *solver; }
+  void reset();
+  virtual void solve();
+  virtual void setBoundaryConditions(const ParaxialSource &src);
+  virtual void getXrayMatProp(double x, double z, double &delta, double &beta) const;
+  virtual void getXrayMatProp(double x, double y, double z, double &delta, double &beta) const;
+  /** HDF5 output is outside the GPU hot path of this build: throws. */
+  virtual void save(const char *fname);
+
+  std::string description{""};
+  const MaterialFunction *material{nullptr};
+
+ protected:
+  Solver *solver{nullptr};
+  Disctretization xDisc, zDisc, yDisc;
+  double wavenumber{1.0};
+  std::string name;
+  const ParaxialSource *src{nullptr};
+};

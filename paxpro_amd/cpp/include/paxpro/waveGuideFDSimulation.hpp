@@ -1,0 +1,49 @@
+// 2D waveguide simulations (include/waveGuideFDSimulation.hpp,
+// include/curvedWaveGuide2D.hpp of the reference).  The curved guide's material
+// is evaluated on the device; border tracking, eigenmode projection and HDF5
+// output are not part of this build.
+#pragma once
+
+#include "cladding.hpp"
+#include "paraxialSimulation.hpp"
+
+class WaveGuideFDSimulation : public ParaxialSimulation {
+ public:
+  WaveGuideFDSimulation() : ParaxialSimulation("noname") {}
+  WaveGuideFDSimulation(const char *name) : ParaxialSimulation(name) {}
+  void setWaveguideLength(double wgl) { wglength = wgl; }
+  void setCladding(const Cladding &clad) { cladding = &clad; }
+  void setInsideMaterial(const Cladding &mat) { insideMaterial = &mat; }
+  const Cladding &getCladding() const { return *cladding; }
+  virtual bool isInsideGuide(double x, double z) const { return true; }
+  virtual void getXrayMatProp(double x, double z, double &delta, double &beta) const override;
+  double getWaveguideLength() const { return wglength; }
+  const Cladding *getInsideMaterial() const { return insideMaterial; }
+
+ protected:
+  double wglength{1E10};
+  const Cladding *cladding{nullptr};
+  const Cladding *insideMaterial{nullptr};
+  virtual bool waveguideEnded(double x, double z) const { return z > wglength; }
+};
+
+class CurvedWaveGuideFD : public WaveGuideFDSimulation {
+ public:
+  CurvedWaveGuideFD() : WaveGuideFDSimulation("CurvedWaveGuide2D") {}
+  void setRadiusOfCurvature(double newR) { R = newR; }
+  void setWidth(double newWidth) { width = newWidth; }
+  double getWidth() const { return width; }
+  double getRadiusOfCurvature() const { return R; }
+  double smoothedWG(double x, double z) const;
+  void useSmoothedWG() { useSmoothed = true; }
+  bool usesSmoothedWG() const { return useSmoothed; }
+  virtual bool isInsideGuide(double x, double z) const override;
+  virtual void getXrayMatProp(double x, double z, double &delta, double &beta) const override;
+  virtual double waveGuideStartX(double z) const { return -0.5 * z * z / R; }
+  virtual double waveGuideEndX(double z) const { return waveGuideStartX(z) + width; }
+
+ protected:
+  double R{1.0};
+  double width{1.0};
+  bool useSmoothed{false};
+};

@@ -1,0 +1,193 @@
+// Test driver of the C++ drop-in classes: builds simulations the way the
+// reference's examples do (Examples/sphere.cpp, Examples/waveguide2D.cpp,
+// Examples/circularWG.py) and writes the exit fields as raw complex128 so the
+// Python tests can compare them with the CPU oracle.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "paxpro/paxpro.hpp"
+
+static void write(const char *path, const cdouble *p, size_t n)
+{
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(p), n * sizeof(cdouble));
+}
+
+static int kats()
+{
+  struct K { int op; double a, b, c; double p[3]; double e[3]; } ks[] = {
+      {0, 90.0, 2, 0, {1, 0, 0}, {0, -1, 0}},
+      {0, 90.0, 1, 0, {1, 0, 0}, {0, 0, 1}},
+      {0, 90.0, 0, 0, {0, 1, 0}, {0, 0, -1}},
+      {1, 10.0, -10.0, -3.0, {1, -1, 10}, {-9, 9, 13}}};
+  for (const K &k : ks) {
+    geom::Sphere s(10.0);
+    if (k.op == 0) s.rotate(k.a, static_cast<geom::Axis_t>(static_cast<int>(k.b)));
+    else s.translate(k.a, k.b, k.c);
+    double x = k.p[0], y = k.p[1], z = k.p[2];
+    s.transform(x, y, z);
+    if (std::fabs(x - k.e[0]) > 1e-6 || std::fabs(y - k.e[1]) > 1e-6 || std::fabs(z - k.e[2]) > 1e-6) {
+      std::printf("KAT FAIL %f %f %f\n", x, y, z);
+      return 1;
+    }
+  }
+  // CSG membership through the Part/Module/CSGMaterial chain
+  geom::Sphere sp(4.0);
+  sp.translate(1.0, -2.0, 0.5);
+  geom::Part part("p");
+  part.add(sp);
+  part.delta = 5e-6;
+  part.beta = 1e-7;
+  geom::Module mod("m");
+  mod.add(part);
+  CSGMaterial csg;
+  csg.addModule(mod);
+  double d, b;
+  csg.getXrayMatProp(1.0, -2.0, 0.5, d, b);
+  if (d != 5e-6 || b != 1e-7) return 1;
+  csg.getXrayMatProp(6.0, -2.0, 0.5, d, b);
+  if (d != 0.0 || b != 0.0) return 1;
+  std::printf("KAT OK\n");
+  return 0;
+}
+
+// A sphere as a user MaterialFunction (Examples/sphere.cpp:6-24): host-sampled.
+class UserSphere : public MaterialFunction {
+ public:
+  explicit UserSphere(double r) : r(r) {}
+  void getXrayMatProp(double x, double y, double z, double &delta, double &beta) const override
+  {
+    if (x * x + y * y + z * z < r * r) {
+      delta = 5e-6;
+      beta = 1e-7;
+    } else {
+      delta = beta = 0.0;
+    }
+  }
+  double r;
+};
+
+// Slab of Examples/waveguide2D.cpp:9-27 centred on 0.
+class Slab : public MaterialFunction {
+ public:
+  void getXrayMatProp(double x, double y, double z, double &delta, double &beta) const override
+  {
+    if (x < 50.0 && x > -50.0) {
+      delta = 0.0;
+      beta = 0.0;
+    } else {
+      delta = 4.5e-5;
+      beta = 3.5e-6;
+    }
+  }
+};
+
+static int run3d(const std::string &mode, int n, int nsteps, const char *out, int prec)
+{
+  const double extent = 20000.0, dx = extent / n, xmin = -extent / 2, dz = 10.0;
+  const double zmin = -nsteps * dz / 2;
+  ParaxialSimulation sim("sphere");
+  sim.setTransverseDiscretization(xmin, xmin + (n - 1) * dx, dx, 8);
+  sim.setVerticalDiscretization(xmin, xmin + (n - 1) * dx, dx);
+  sim.setLongitudinalDiscretization(zmin, zmin + nsteps * dz, dz, nsteps / 2);
+  GaussianBeam gb;
+  gb.setWaist(3000.0);
+  gb.setWavelength(0.1);
+  gb.setDim(ParaxialSource::Dim_t::THREE_D);
+  geom::Sphere sphere(4000.0);
+  geom::Part part("sphere");
+  part.add(sphere);
+  part.delta = 5e-6;
+  part.beta = 1e-7;
+  geom::Module mod("phantom");
+  mod.add(part);
+  CSGMaterial csg;
+  csg.addModule(mod);
+  UserSphere user(4000.0);
+  FFTSolver3D fft;
+  ADI adi;
+  Solver3D *solver = mode.rfind("adi", 0) == 0 ? static_cast<Solver3D *>(&adi) : &fft;
+  solver->setPrecision(prec);
+  sim.setSolver(*solver);
+  sim.material = (mode == "fft3d_user") ? static_cast<const MaterialFunction *>(&user) : &csg;
+  sim.setBoundaryConditions(gb);
+  if (mode == "fft3d_abs") fft.absorbingBC(600.0, 300.0);
+  if (mode == "fft3d_step") {
+    for (int i = 1; i < n * 0 + nsteps + 1; ++i) sim.step();
+  } else {
+    sim.solve();
+  }
+  const arma::cx_mat &last = solver->getLastSolution3D();
+  write(out, last.memptr(), last.n_elem);
+  const arma::cx_cube &cube = solver->getSolution3D();
+  write((std::string(out) + ".cube").c_str(), cube.memptr(), cube.n_elem);
+  std::printf("%s n=%d steps=%d cube=%zux%zux%zu\n", mode.c_str(), n, nsteps, (size_t)cube.n_rows,
+              (size_t)cube.n_cols, (size_t)cube.n_slices);
+  return 0;
+}
+
+static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
+{
+  const double dz = 500.0;
+  CurvedWaveGuideFD wg;
+  ParaxialSimulation plain("wg2D");
+  ParaxialSimulation &sim = mode == "cn_curved" ? static_cast<ParaxialSimulation &>(wg) : plain;
+  double lo = -500.0, hi = 500.0;
+  Cladding clad;
+  clad.setRefractiveIndex(4.5e-5, 3.5e-6);
+  if (mode == "cn_curved") {
+    wg.setRadiusOfCurvature(5e6);
+    wg.setWidth(120.0);
+    wg.setCladding(clad);
+    const double L = nsteps * dz;
+    lo = -L * L / (2 * 5e6) - 2 * 120.0 - 200;
+    hi = 120.0 + 200;
+  }
+  const double dx = (hi - lo) / (nx - 1);
+  sim.setTransverseDiscretization(lo, hi, dx);
+  sim.setLongitudinalDiscretization(0.0, nsteps * dz, dz);
+  Slab slab;
+  sim.material = mode == "cn_curved" ? nullptr : &slab;
+  PlaneWave pw;
+  pw.setWavelength(0.1);
+  CrankNicholson cn;
+  FFTSolver2D f2;
+  Solver2D *solver = mode == "fft2d" ? static_cast<Solver2D *>(&f2) : &cn;
+  if (mode != "fft2d") cn.setBoundaryCondition(Solver2D::BC_t::TRANSPARENT);
+  sim.setSolver(*solver);
+  sim.setBoundaryConditions(pw);
+  if (mode == "cn_step") {
+    for (int i = 1; i <= nsteps; ++i) sim.step();
+  } else {
+    sim.solve();
+  }
+  const arma::cx_vec &last = solver->getLastSolution();
+  write(out, last.memptr(), last.n_elem);
+  const arma::cx_mat &sol = solver->getSolution();
+  write((std::string(out) + ".sol").c_str(), sol.memptr(), sol.n_elem);
+  std::printf("%s nx=%d steps=%d sol=%zux%zu\n", mode.c_str(), nx, nsteps, (size_t)sol.n_rows, (size_t)sol.n_cols);
+  return 0;
+}
+
+int main(int argc, char **argv)
+{
+  try {
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    if (mode == "kat") return kats();
+    if (argc < 5) return 2;
+    const int n = std::atoi(argv[2]), nsteps = std::atoi(argv[3]);
+    const int prec = argc > 5 ? std::atoi(argv[5]) : PXG_PREC_FP64;
+    if (mode.rfind("fft3d", 0) == 0 || mode.rfind("adi", 0) == 0) return run3d(mode, n, nsteps, argv[4], prec);
+    return run2d(mode, n, nsteps, argv[4]);
+  } catch (const std::exception &e) {
+    std::printf("ERROR %s\n", e.what());
+    return 1;
+  }
+}

@@ -1,0 +1,102 @@
+"""The C++ drop-in classes (paxpro_amd/cpp) driven like the reference's
+examples, compared with the CPU oracle on the same inputs.  The driver binary
+is built by __graft_entry__.build() (tests/dropin/Makefile)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_common import announce, rel_l2
+from scenes import K_01, gaussian_field, spheres_scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "dropin", "dropin_driver")
+
+
+def run(tmp_path, *args):
+    out = str(tmp_path / "out.bin")
+    cmd = [DRIVER, args[0], str(args[1]), str(args[2]), out] + [str(a) for a in args[3:]]
+    announce("dropin", " ".join(cmd[1:4]))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return out
+
+
+def load(path, shape, order="F"):
+    return np.fromfile(path, dtype=np.complex128).reshape(shape, order=order)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="drop-in driver not built")
+def test_dropin_transform_kats_cpu():
+    r = subprocess.run([DRIVER, "kat"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "KAT OK" in r.stdout
+
+
+def _grid3d(n, nsteps):
+    extent, dz = 20000.0, 10.0
+    dx = extent / n
+    xmin = -extent / 2
+    zmin = -nsteps * dz / 2
+    return O.grid(xmin, xmin + (n - 1) * dx, dx, zmin, zmin + nsteps * dz, dz, K_01)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fft3d", "fft3d_user", "fft3d_step", "fft3d_abs"])
+def test_dropin_fft3d(tmp_path, mode):
+    n, nsteps = 64, 8
+    out = run(tmp_path, mode, n, nsteps)
+    g = _grid3d(n, nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = O.CSGMaterial(*spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)]))
+    ab = None
+    if mode == "fft3d_abs":
+        npix = int(600.0 / g.dx + 1)
+        ab = O.Absorber3D(npix, npix, g.dx / 300.0, g.dx / 300.0)
+    dsz = nsteps // 2
+    slices = (nsteps + 1) // dsz + 1
+    cube = np.zeros((n // 8, n // 8, slices), complex, order="F")
+    O.cube_store(u0, cube, dsz, 0)
+    ref = O.fft3d_run(g, csg, u0, 1, nsteps, absorber=ab, cube=cube, dsz=dsz)
+    assert rel_l2(load(out, (n, n)), ref) < 1e-11
+    assert rel_l2(load(out + ".cube", cube.shape), cube) < 1e-11
+
+
+@pytest.mark.gpu
+def test_dropin_adi(tmp_path):
+    n, nsteps = 64, 4
+    out = run(tmp_path, "adi", n, nsteps)
+    g = _grid3d(n, nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = O.CSGMaterial(*spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)]))
+    ref = O.adi_run(g, csg, u0, 1, nsteps)
+    assert rel_l2(load(out, (n, n)), ref) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,nx", [("cn", 1024), ("cn_step", 300), ("cn_curved", 777), ("fft2d", 512)])
+def test_dropin_2d(tmp_path, mode, nx):
+    nsteps, dz = 40, 500.0
+    out = run(tmp_path, mode, nx, nsteps)
+    if mode == "cn_curved":
+        R, w = 5e6, 120.0
+        L = nsteps * dz
+        lo, hi = -L * L / (2 * R) - 2 * w - 200, w + 200
+    else:
+        lo, hi = -500.0, 500.0
+    dx = (hi - lo) / (nx - 1)
+    g = O.grid(lo, hi, dx, 0.0, nsteps * dz, dz, K_01)
+    if mode == "cn_curved":
+        mat = O.CurvedMaterial(R=R, width=w, clad_delta=4.5e-5, clad_beta=3.5e-6, dx=dx, dz=dz)
+    else:
+        mat = O.SlabMaterial(-50.0, 50.0, 0.0, 0.0, 4.5e-5, 3.5e-6)
+    u0 = np.ones(nx, complex)
+    sol = np.zeros((nx, nsteps + 1), complex, order="F")
+    sol[:, 0] = u0
+    if mode == "fft2d":
+        ref = O.fft2d_run(g, mat, u0, 1, nsteps, solution=sol)
+    else:
+        ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol)
+    assert rel_l2(load(out, (nx,)), ref) < 1e-10
+    assert rel_l2(load(out + ".sol", sol.shape), sol) < 1e-10
