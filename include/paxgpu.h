@@ -123,6 +123,17 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out);
 int pxg_plan_destroy(pxg_plan plan);
 /* Bytes of device memory the plan reserves (for admission by HBM budget). */
 int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes);
+/* Job plan: n independent problems of one kind in one plan, each with its own
+ * grid, wavenumber and material (uneven Nx, photon energies, waveguides), all
+ * advanced by one launch per pxg_step.  Field/solution index b = job b.
+ * Supported kind: PXG_CN (one work-group per job).  No reference counterpart:
+ * the reference solves one problem per Solver object; this batches many
+ * CrankNicholson::solveStep marches (src/crancNicholson.cpp:17-105). */
+int pxg_plan_create_jobs(pxg_ctx ctx, const pxg_problem *probs, int n, pxg_plan *out);
+/* Device-evaluated CurvedWaveGuideFD material of one job of a job plan. */
+int pxg_set_job_material_curved_wg(pxg_plan plan, int job, const pxg_curved_wg *wg);
+/* Solution dimensions of field/job b (job plans have per-job sizes). */
+int pxg_job_solution_dims(pxg_plan plan, int job, int dims[3]);
 /* Per-field wavenumbers of a batched plan (photon energies); n == batch. */
 int pxg_set_wavenumbers(pxg_plan plan, const double *k, int n);
 

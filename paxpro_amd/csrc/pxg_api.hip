@@ -120,6 +120,17 @@ struct pxg_plan_s {
   // tridiagonal scratch (CN, ADI)
   DBuf scratch;
   bool field_set = false;
+  // per-field geometry: job plans (pxg_plan_create_jobs) and CN job tables
+  bool is_jobs = false;
+  std::vector<pxg_problem> jp;      // one problem per job (job plans only)
+  std::vector<size_t> foff, soff;   // element offsets of each field / stored solution
+  std::vector<size_t> fcells;       // cells of each field
+  std::vector<int> job_mat;         // per-job material mode (job plans)
+  std::vector<DCurved> job_wg;
+  DBuf jobs_dev;
+  bool jobs_dirty = true;
+  int max_P = 1;
+  size_t sol_total = 0;             // elements of solution storage
   // profiling
   std::vector<hipEvent_t> ev;
   struct Rec {
@@ -462,14 +473,62 @@ TriArgs<R> tri_args(pxg_plan P)
 }
 
 template <class R>
+void build_cn_jobs(pxg_plan P)
+{
+  const int nb = P->is_jobs ? static_cast<int>(P->jp.size()) : P->p.batch;
+  std::vector<CnJob> jobs(nb);
+  size_t scoff = 0;
+  P->max_P = 1;
+  const MatDev shared = mat_dev(P);
+  for (int b = 0; b < nb; ++b) {
+    const pxg_problem &q = P->is_jobs ? P->jp[b] : P->p;
+    const CnPartition cp = cn_partition(q.nx);
+    CnJob &J = jobs[b];
+    J.nx = q.nx;
+    J.P = cp.P;
+    J.m = cp.m;
+    J.bc = q.bc;
+    J.nz = q.nz;
+    J.absorb_npix = q.absorb_npix_x;
+    J.absorb_inv = q.absorb_inv_x;
+    J.foff = P->foff[b];
+    J.soff = q.store_dsz > 0 ? P->soff[b] : ~static_cast<size_t>(0);
+    J.scoff = scoff;
+    J.xmin = q.xmin;
+    J.dx = q.dx;
+    J.zmin = q.zmin;
+    J.dz = q.dz;
+    J.k = P->k_host[b];
+    J.mat = shared;
+    if (P->is_jobs && P->job_mat[b] == MAT_CURVED) {
+      J.mat.mode = MAT_CURVED;
+      J.mat.wg = P->job_wg[b];
+    }
+    if (J.mat.mode == MAT_PLANES) J.mat.plane_stride = 2 * static_cast<size_t>(q.nx);
+    scoff += cn_job_scratch(q.nx, sizeof(cx<R>));
+    P->max_P = std::max(P->max_P, cp.P);
+  }
+  if (scoff > P->scratch.bytes) fail(PXG_ERR_STATE, "CN scratch too small (%zu < %zu)", P->scratch.bytes, scoff);
+  P->jobs_dev.alloc(jobs.size() * sizeof(CnJob));
+  hip_check(hipMemcpyAsync(P->jobs_dev.p, jobs.data(), P->jobs_dev.bytes, hipMemcpyHostToDevice, P->stream),
+            "upload CN jobs");
+  hip_check(hipStreamSynchronize(P->stream), "sync");
+  P->jobs_dirty = false;
+}
+
+template <class R>
 void run_cn(pxg_plan P, unsigned first, unsigned nsteps)
 {
-  TriArgs<R> a = tri_args<R>(P);
-  a.first_step = first;
-  a.nsteps = nsteps;
-  const double bytes = 2.0 * P->cells * P->p.batch * sizeof(cx<R>) * nsteps;
+  if (P->jobs_dirty) build_cn_jobs<R>(P);
+  size_t cells = 0;
+  for (size_t c : P->fcells) cells += c;
+  const double bytes = 2.0 * cells * sizeof(cx<R>) * nsteps;
+  const int njobs = static_cast<int>(P->fcells.size());
   const int ev0 = begin(P);
-  record(P, "cn_march", bytes, launch_cn_march<R>(a, P->stream), ev0);
+  record(P, "cn_march", bytes,
+         launch_cn_jobs<R>(P->jobs_dev.as<CnJob>(), njobs, P->max_P, P->field.as<cx<R>>(), P->scratch.p,
+                           P->sol.as<cx<R>>(), first, nsteps, P->stream),
+         ev0);
 }
 
 template <class R>
@@ -668,6 +727,11 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
       const size_t sb = tridiag_scratch_bytes(*prob, P->mixed);
       if (sb) P->scratch.alloc(sb);
     }
+    for (int b = 0; b < prob->batch; ++b) {
+      P->foff.push_back(static_cast<size_t>(b) * P->cells);
+      P->soff.push_back(static_cast<size_t>(b) * P->sol_fstride);
+      P->fcells.push_back(P->cells);
+    }
     rebuild_tables(P);
     hip_check(hipStreamSynchronize(P->stream), "sync");
     *out = P;
@@ -681,6 +745,104 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     // reservation already released with the plan
   }
   return rc;
+}
+
+int pxg_plan_create_jobs(pxg_ctx ctx, const pxg_problem *probs, int n, pxg_plan *out)
+{
+  pxg_plan P = nullptr;
+  int rc = guarded([&] {
+    if (!ctx || !probs || !out || n < 1) fail(PXG_ERR_INVALID, "null argument or no jobs");
+    size_t need = 0, cells = 0, sol = 0, scratch = 0;
+    const int esz = probs[0].precision == PXG_PREC_MIXED ? 8 : 16;
+    for (int i = 0; i < n; ++i) {
+      const pxg_problem &q = probs[i];
+      validate(q);
+      if (q.kind != PXG_CN) fail(PXG_ERR_UNSUPPORTED, "job plans support PXG_CN (job %d)", i);
+      if (q.batch != 1) fail(PXG_ERR_INVALID, "job %d: batch must be 1", i);
+      if (q.precision != probs[0].precision) fail(PXG_ERR_INVALID, "job %d: mixed precisions", i);
+      cells += q.nx;
+      if (q.store_dsz > 0) sol += static_cast<size_t>(q.nx) * q.nz;
+      scratch += cn_job_scratch(q.nx, esz);
+    }
+    need = cells * esz + sol * esz + scratch + n * (sizeof(CnJob) + 64) + 4096;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->reserved.load() + need > ctx->budget)
+        fail(PXG_ERR_BUDGET, "job plan needs %zu bytes, budget %zu has %zu reserved", need, ctx->budget,
+             ctx->reserved.load());
+      ctx->reserved += need;
+    }
+    P = new pxg_plan_s;
+    P->ctx = ctx;
+    P->reserved = need;
+    P->p = probs[0];
+    P->p.batch = n;
+    P->is_jobs = true;
+    P->jp.assign(probs, probs + n);
+    P->mixed = probs[0].precision == PXG_PREC_MIXED;
+    P->esz = esz;
+    P->cells = 0;
+    hip_check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
+    size_t fo = 0, so = 0;
+    for (int i = 0; i < n; ++i) {
+      P->foff.push_back(fo);
+      P->soff.push_back(so);
+      P->fcells.push_back(probs[i].nx);
+      P->k_host.push_back(probs[i].k);
+      fo += probs[i].nx;
+      if (probs[i].store_dsz > 0) so += static_cast<size_t>(probs[i].nx) * probs[i].nz;
+    }
+    P->job_mat.assign(n, MAT_NONE);
+    P->job_wg.assign(n, DCurved{});
+    P->field.alloc(fo * esz);
+    hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
+    if (so) {
+      P->sol.alloc(so * esz);
+      hip_check(hipMemsetAsync(P->sol.p, 0, P->sol.bytes, P->stream), "memset");
+    }
+    P->sol_total = so;
+    P->scratch.alloc(scratch);
+    P->kvec.alloc(sizeof(double) * n);
+    hip_check(hipMemcpyAsync(P->kvec.p, P->k_host.data(), P->kvec.bytes, hipMemcpyHostToDevice, P->stream), "k");
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    *out = P;
+  });
+  if (rc != PXG_OK && P) {
+    std::string keep = g_err;
+    pxg_plan_destroy(P);
+    g_err = keep;
+  }
+  return rc;
+}
+
+int pxg_set_job_material_curved_wg(pxg_plan P, int job, const pxg_curved_wg *wg)
+{
+  return guarded([&] {
+    check_b(P, job);
+    if (!P->is_jobs) fail(PXG_ERR_INVALID, "not a job plan");
+    if (!wg) fail(PXG_ERR_INVALID, "null waveguide");
+    P->job_wg[job] = DCurved{wg->R, wg->width, wg->clad_delta, wg->clad_beta, wg->in_delta, wg->in_beta,
+                             wg->wglength, wg->dx, wg->dz, wg->use_smoothed};
+    P->job_mat[job] = MAT_CURVED;
+    P->jobs_dirty = true;
+  });
+}
+
+int pxg_job_solution_dims(pxg_plan P, int job, int dims[3])
+{
+  return guarded([&] {
+    check_b(P, job);
+    if (!dims) fail(PXG_ERR_INVALID, "null output");
+    const pxg_problem &q = P->is_jobs ? P->jp[job] : P->p;
+    if (P->is_jobs) {
+      dims[0] = q.store_dsz > 0 ? q.nx : 0;
+      dims[1] = q.store_dsz > 0 ? q.nz : 0;
+      dims[2] = q.store_dsz > 0 ? 1 : 0;
+    } else {
+      for (int i = 0; i < 3; ++i) dims[i] = P->sol_dims[i];
+    }
+  });
 }
 
 int pxg_plan_destroy(pxg_plan P)
@@ -703,7 +865,8 @@ int pxg_set_wavenumbers(pxg_plan P, const double *k, int n)
     for (int i = 0; i < n; ++i)
       if (!(k[i] > 0)) fail(PXG_ERR_INVALID, "wavenumber %d must be > 0", i);
     P->k_host.assign(k, k + n);
-    rebuild_tables(P);
+    if (!P->is_jobs) rebuild_tables(P);
+    P->jobs_dirty = true;
   });
 }
 
@@ -712,8 +875,8 @@ int pxg_set_field(pxg_plan P, int b, const double *c128)
   return guarded([&] {
     check_b(P, b);
     if (!c128) fail(PXG_ERR_INVALID, "null field");
-    const size_t n = P->cells;
-    char *dst = P->field.as<char>() + static_cast<size_t>(b) * n * P->esz;
+    const size_t n = P->fcells[b];
+    char *dst = P->field.as<char>() + P->foff[b] * P->esz;
     if (P->mixed) {
       std::vector<cx<float>> tmp(n);
       for (size_t i = 0; i < n; ++i) tmp[i] = {static_cast<float>(c128[2 * i]), static_cast<float>(c128[2 * i + 1])};
@@ -725,7 +888,7 @@ int pxg_set_field(pxg_plan P, int b, const double *c128)
     }
     P->field_set = true;
     // copyCurrentSolution(0): slice / column 0 of the stored solution
-    const pxg_problem &p = P->p;
+    const pxg_problem &p = P->is_jobs ? P->jp[b] : P->p;
     if (p.store_dsz > 0) {
       const bool is3d = p.kind == PXG_FFT3D || p.kind == PXG_ADI;
       if (is3d) {
@@ -752,11 +915,11 @@ int pxg_set_field(pxg_plan P, int b, const double *c128)
           }
           slice[i] = {tx / (d * d), ty / (d * d)};
         }
-        char *sd = P->sol.as<char>() + static_cast<size_t>(b) * P->sol_fstride * sizeof(cx<double>);
+        char *sd = P->sol.as<char>() + P->soff[b] * sizeof(cx<double>);
         hip_check(hipMemcpyAsync(sd, slice.data(), slice.size() * sizeof(cx<double>),
                                  hipMemcpyHostToDevice, P->stream), "store slice 0");
       } else {
-        char *sd = P->sol.as<char>() + static_cast<size_t>(b) * P->sol_fstride * P->esz;
+        char *sd = P->sol.as<char>() + P->soff[b] * P->esz;
         hip_check(hipMemcpyAsync(sd, dst, n * P->esz, hipMemcpyDeviceToDevice, P->stream),
                   "store column 0");
       }
@@ -770,8 +933,8 @@ int pxg_get_field(pxg_plan P, int b, double *c128)
   return guarded([&] {
     check_b(P, b);
     if (!c128) fail(PXG_ERR_INVALID, "null output");
-    const size_t n = P->cells;
-    const char *src = P->field.as<char>() + static_cast<size_t>(b) * n * P->esz;
+    const size_t n = P->fcells[b];
+    const char *src = P->field.as<char>() + P->foff[b] * P->esz;
     hip_check(hipStreamSynchronize(P->stream), "sync");
     if (P->mixed) {
       std::vector<cx<float>> tmp(n);
@@ -792,6 +955,7 @@ int pxg_solution_dims(pxg_plan P, int dims[3])
     check_plan(P);
     if (!dims) fail(PXG_ERR_INVALID, "null output");
     for (int i = 0; i < 3; ++i) dims[i] = P->sol_dims[i];
+    if (P->is_jobs) fail(PXG_ERR_INVALID, "job plans: use pxg_job_solution_dims");
   });
 }
 
@@ -799,18 +963,18 @@ int pxg_get_solution(pxg_plan P, int b, double *c128, size_t n_complex)
 {
   return guarded([&] {
     check_b(P, b);
-    if (P->p.store_dsz <= 0) fail(PXG_ERR_STATE, "plan stores no solution (store_dsz = 0)");
-    if (!c128 || n_complex < P->sol_fstride)
-      fail(PXG_ERR_INVALID, "output needs %zu complex values", P->sol_fstride);
+    const pxg_problem &q = P->is_jobs ? P->jp[b] : P->p;
+    if (q.store_dsz <= 0) fail(PXG_ERR_STATE, "plan stores no solution (store_dsz = 0)");
+    const bool is3d = q.kind == PXG_FFT3D || q.kind == PXG_ADI;
+    const size_t n = P->is_jobs ? static_cast<size_t>(q.nx) * q.nz : P->sol_fstride;
+    if (!c128 || n_complex < n) fail(PXG_ERR_INVALID, "output needs %zu complex values", n);
     hip_check(hipStreamSynchronize(P->stream), "sync");
-    const bool is3d = P->p.kind == PXG_FFT3D || P->p.kind == PXG_ADI;
-    const size_t n = P->sol_fstride;
     if (is3d || !P->mixed) {
-      const char *src = P->sol.as<char>() + static_cast<size_t>(b) * n * 16;
+      const char *src = P->sol.as<char>() + P->soff[b] * 16;
       hip_check(hipMemcpy(c128, src, n * 16, hipMemcpyDeviceToHost), "get_solution");
     } else {
       std::vector<cx<float>> tmp(n);
-      const char *src = P->sol.as<char>() + static_cast<size_t>(b) * n * 8;
+      const char *src = P->sol.as<char>() + P->soff[b] * 8;
       hip_check(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost), "get_solution");
       for (size_t i = 0; i < n; ++i) {
         c128[2 * i] = tmp[i].x;
@@ -825,6 +989,7 @@ int pxg_set_material_none(pxg_plan P)
   return guarded([&] {
     check_plan(P);
     P->mat_mode = MAT_NONE;
+    P->jobs_dirty = true;
   });
 }
 
@@ -886,8 +1051,11 @@ int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, cons
     P->n_parts = static_cast<int>(order.size());
     P->is_ref = is_reference_run;
     P->mat_mode = MAT_CSG;
-    if (P->mixed) build_part_factors<float>(P);
-    else build_part_factors<double>(P);
+    P->jobs_dirty = true;
+    if (P->p.kind == PXG_FFT3D) {
+      if (P->mixed) build_part_factors<float>(P);
+      else build_part_factors<double>(P);
+    }
   });
 }
 
@@ -901,6 +1069,7 @@ int pxg_set_material_curved_wg(pxg_plan P, const pxg_curved_wg *wg)
     P->wg = DCurved{wg->R, wg->width, wg->clad_delta, wg->clad_beta, wg->in_delta, wg->in_beta,
                     wg->wglength, wg->dx, wg->dz, wg->use_smoothed};
     P->mat_mode = MAT_CURVED;
+    P->jobs_dirty = true;
   });
 }
 
@@ -908,6 +1077,7 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
 {
   return guarded([&] {
     check_plan(P);
+    if (P->is_jobs) fail(PXG_ERR_UNSUPPORTED, "job plans take device materials only");
     if (!db || n_planes < 1) fail(PXG_ERR_INVALID, "need at least one plane");
     const size_t per = 2 * P->cells;
     P->planes.alloc(per * n_planes * sizeof(double));
@@ -916,6 +1086,7 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
     P->first_plane = first_plane;
     P->z_invariant = z_invariant;
     P->mat_mode = MAT_PLANES;
+    P->jobs_dirty = true;
   });
 }
 
@@ -1006,7 +1177,7 @@ int pxg_field_device_ptr(pxg_plan P, int b, void **dptr)
   return guarded([&] {
     check_b(P, b);
     if (!dptr) fail(PXG_ERR_INVALID, "null output");
-    *dptr = P->field.as<char>() + static_cast<size_t>(b) * P->cells * P->esz;
+    *dptr = P->field.as<char>() + P->foff[b] * P->esz;
   });
 }
 

@@ -46,7 +46,7 @@ static size_t cn_field_scratch(int n, int esz, const CnPartition &cp)
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed)
 {
   const int esz = mixed ? 8 : 16;
-  if (p.kind == PXG_CN) return cn_field_scratch(p.nx, esz, cn_partition(p.nx)) * p.batch;
+  if (p.kind == PXG_CN) return align256(cn_field_scratch(p.nx, esz, cn_partition(p.nx))) * p.batch;
   if (p.kind == PXG_ADI) {
     const size_t cells = static_cast<size_t>(p.nx) * p.nx;
     return (align256(cells * esz) + 2 * align256(cells * 16)) * p.batch;
@@ -107,16 +107,16 @@ __device__ __forceinline__ cx<double> ld(const cx<R> *f, int i)
 // rhs = i rho/4 (u_{i-1} + u_{i+1}) - i rho/2 u_i + (1 - (r/2)(beta0 + i delta0)) u_i,
 // transparent boundary rows at i = 0 and N-1 (src/crancNicholson.cpp:30-89,107-148).
 template <class R>
-__device__ __forceinline__ void cn_row(const TriArgs<R> &a, const cx<R> *prev, int i, unsigned st,
+__device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i, unsigned st,
                                        double z, double rho, double r, const uint64_t *mask, int nw,
                                        cx<double> &ai, cx<double> &bi, cx<double> &ci,
                                        cx<double> &di)
 {
-  const int N = a.n;
-  const double x = coord(a.xmin, i, a.dx);
+  const int N = J.nx;
+  const double x = coord(J.xmin, i, J.dx);
   double d1, b1, d0, b0;
-  material_2d(a.mat, mask, nw, st, i, x, z, d1, b1);
-  material_2d(a.mat, mask, nw, st - 1, i, x, z - a.dz, d0, b0);
+  material_2d(J.mat, mask, nw, st, i, x, z, d1, b1);
+  material_2d(J.mat, mask, nw, st - 1, i, x, z - J.dz, d0, b0);
   const cx<double> off = {0.0, -0.25 * rho};
   ai = i > 0 ? off : cx<double>{0.0, 0.0};
   ci = i < N - 1 ? off : cx<double>{0.0, 0.0};
@@ -128,7 +128,7 @@ __device__ __forceinline__ void cn_row(const TriArgs<R> &a, const cx<R> *prev, i
   v = v - cx<double>{-0.5 * rho * u.y, 0.5 * rho * u.x};
   v = v + cx<double>{1.0 - 0.5 * (b0 * r), -0.5 * (d0 * r)} * u;
   di = v;
-  if (a.bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
+  if (J.bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
     const int inner = i == 0 ? 1 : N - 2;
     cx<double> e;
     if (tbc_factor(u, ld(prev, inner), 1e-16, e)) {
@@ -149,151 +149,169 @@ __device__ __forceinline__ double absorb_factor(int i, int N, unsigned npix, dou
 }
 
 // ---------------------------------------------------------------- CN march
+// One work-group per job (blockIdx.x); jobs may differ in size, grid,
+// wavenumber and material.  Lanes p >= job.P take part in barriers only.
 template <class R>
-__global__ __launch_bounds__(CN_MAX_P) void k_cn_march(TriArgs<R> a, int P, int m, size_t per_field)
+__global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ jobs, cx<R> *fields,
+                                                      char *scratch, cx<R> *sol, unsigned first,
+                                                      unsigned nsteps)
 {
   __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
-  const int p = threadIdx.x, b = blockIdx.x, N = a.n;
-  char *base = static_cast<char *>(a.scratch) + per_field * b;
-  cx<R> *bufA = a.field + static_cast<size_t>(b) * N;
+  const CnJob &J = jobs[blockIdx.x];
+  const int p = threadIdx.x, N = J.nx, P = J.P, m = J.m;
+  const bool active = p < P;
+  char *base = scratch + J.scoff;
+  cx<R> *bufA = fields + J.foff;
   cx<R> *bufB = reinterpret_cast<cx<R> *>(base);
   const size_t cst = align256(static_cast<size_t>(m) * P * 16);
   cx<double> *ca = reinterpret_cast<cx<double> *>(base + align256(static_cast<size_t>(N) * sizeof(cx<R>)));
   cx<double> *cc = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(ca) + cst);
   cx<double> *cd = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(cc) + cst);
-  const double k = a.kvec[b];
-  const double rho = a.dz / (k * a.dx * a.dx);
-  const double r = k * a.dz;
+  const double k = J.k;
+  const double rho = J.dz / (k * J.dx * J.dx);
+  const double r = k * J.dz;
   const int s = p * m;
-  const int e = min(N, s + m) - 1;
+  const int e = active ? min(N, s + m) - 1 : s - 1;
   const int M = 2 * P;
-  const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
+  const int nw = J.mat.mode == MAT_CSG ? (J.mat.n_parts + 63) >> 6 : 0;
 
-  for (unsigned it = 0; it < a.nsteps; ++it) {
-    const unsigned st = a.first_step + it;
+  for (unsigned it = 0; it < nsteps; ++it) {
+    const unsigned st = first + it;
     const cx<R> *prev = (it & 1) ? bufB : bufA;
     cx<R> *next = (it & 1) ? bufA : bufB;
-    const double z = coord(a.zmin, st, a.dz);
-    if (a.mat.mode == MAT_CSG)
-      csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), 0.0, 0.0, z - a.dz, z);
+    const double z = coord(J.zmin, st, J.dz);
+    if (J.mat.mode == MAT_CSG)
+      csg_cull(J.mat, mask, nw, J.xmin, coord(J.xmin, N - 1, J.dx), 0.0, 0.0, z - J.dz, z);
 
-    // chunk elimination: rows s+1..e expressed through x_s (and x_{e+1})
-    cx<double> as, bs, cs, ds;
-    cn_row(a, prev, s, st, z, rho, r, mask, nw, as, bs, cs, ds);
-    cx<double> ah{}, ch{}, dh{};
-    for (int i = s + 1; i <= e; ++i) {
-      cx<double> ai, bi, ci, di;
-      cn_row(a, prev, i, st, z, rho, r, mask, nw, ai, bi, ci, di);
-      if (i == s + 1) {
-        ah = cdiv(ai, bi);
-        ch = cdiv(ci, bi);
-        dh = cdiv(di, bi);
-      } else {
-        const cx<double> den = bi - ai * ch;
-        const cx<double> ahn = cdiv(cx<double>{0.0, 0.0} - ai * ah, den);
-        ch = cdiv(ci, den);
-        dh = cdiv(di - ai * dh, den);
-        ah = ahn;
+    if (active) {
+      // chunk elimination: rows s+1..e expressed through x_s (and x_{e+1})
+      cx<double> as, bs, cs, ds;
+      cn_row(J, prev, s, st, z, rho, r, mask, nw, as, bs, cs, ds);
+      cx<double> ah{}, ch{}, dh{};
+      for (int i = s + 1; i <= e; ++i) {
+        cx<double> ai, bi, ci, di;
+        cn_row(J, prev, i, st, z, rho, r, mask, nw, ai, bi, ci, di);
+        if (i == s + 1) {
+          ah = cdiv(ai, bi);
+          ch = cdiv(ci, bi);
+          dh = cdiv(di, bi);
+        } else {
+          const cx<double> den = bi - ai * ch;
+          const cx<double> ahn = cdiv(cx<double>{0.0, 0.0} - ai * ah, den);
+          ch = cdiv(ci, den);
+          dh = cdiv(di - ai * dh, den);
+          ah = ahn;
+        }
+        const int j = i - s;
+        ca[j * P + p] = ah;
+        cc[j * P + p] = ch;
+        cd[j * P + p] = dh;
       }
-      const int j = i - s;
-      ca[j * P + p] = ah;
-      cc[j * P + p] = ch;
-      cd[j * P + p] = dh;
+      // last row of the chunk is a reduced equation: ah x_s + x_e + ch x_{e+1} = dh
+      ra[2 * p + 1] = ah;
+      rb[2 * p + 1] = {1.0, 0.0};
+      rc[2 * p + 1] = ch;
+      rd[2 * p + 1] = dh;
+      // backward: x_i = dt_i - at_i x_s - ct_i x_e, base at i = e: (0, 0, -1)
+      cx<double> at{0.0, 0.0}, ct{-1.0, 0.0}, dt{0.0, 0.0};
+      for (int i = e - 1; i >= s + 1; --i) {
+        const int j = i - s;
+        const cx<double> a0 = ca[j * P + p], c0 = cc[j * P + p], d0 = cd[j * P + p];
+        dt = d0 - c0 * dt;
+        at = a0 - c0 * at;
+        ct = cx<double>{0.0, 0.0} - c0 * ct;
+        ca[j * P + p] = at;
+        cc[j * P + p] = ct;
+        cd[j * P + p] = dt;
+      }
+      // first row of the chunk: a_s x_{s-1} + (b_s - c_s at) x_s - c_s ct x_e = d_s - c_s dt
+      ra[2 * p] = as;
+      rb[2 * p] = bs - cs * at;
+      rc[2 * p] = cx<double>{0.0, 0.0} - cs * ct;
+      rd[2 * p] = ds - cs * dt;
     }
-    // last row of the chunk is a reduced equation: ah x_s + x_e + ch x_{e+1} = dh
-    ra[2 * p + 1] = ah;
-    rb[2 * p + 1] = {1.0, 0.0};
-    rc[2 * p + 1] = ch;
-    rd[2 * p + 1] = dh;
-    // backward: x_i = dt_i - at_i x_s - ct_i x_e, base at i = e: (0, 0, -1)
-    cx<double> at{0.0, 0.0}, ct{-1.0, 0.0}, dt{0.0, 0.0};
-    for (int i = e - 1; i >= s + 1; --i) {
-      const int j = i - s;
-      const cx<double> a0 = ca[j * P + p], c0 = cc[j * P + p], d0 = cd[j * P + p];
-      dt = d0 - c0 * dt;
-      at = a0 - c0 * at;
-      ct = cx<double>{0.0, 0.0} - c0 * ct;
-      ca[j * P + p] = at;
-      cc[j * P + p] = ct;
-      cd[j * P + p] = dt;
-    }
-    // first row of the chunk: a_s x_{s-1} + (b_s - c_s at) x_s - c_s ct x_e = d_s - c_s dt
-    ra[2 * p] = as;
-    rb[2 * p] = bs - cs * at;
-    rc[2 * p] = cx<double>{0.0, 0.0} - cs * ct;
-    rd[2 * p] = ds - cs * dt;
     __syncthreads();
 
     // parallel cyclic reduction of the 2P-row reduced system
     for (int h = 1; h < M; h *= 2) {
       cx<double> na[2], nb[2], nc[2], nd[2];
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int row = 2 * p + q;
-        cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
-        cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
-        cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
-        cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
-        if (row - h >= 0) {
-          k1 = cdiv(a_, rb[row - h]);
-          am = ra[row - h];
-          cm = rc[row - h];
-          dm = rd[row - h];
+        for (int q = 0; q < 2; ++q) {
+          const int row = 2 * p + q;
+          cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
+          cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
+          cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
+          cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
+          if (row - h >= 0) {
+            k1 = cdiv(a_, rb[row - h]);
+            am = ra[row - h];
+            cm = rc[row - h];
+            dm = rd[row - h];
+          }
+          if (row + h < M) {
+            k2 = cdiv(c_, rb[row + h]);
+            ap = ra[row + h];
+            cp = rc[row + h];
+            dp = rd[row + h];
+          }
+          na[q] = cx<double>{0.0, 0.0} - am * k1;
+          nc[q] = cx<double>{0.0, 0.0} - cp * k2;
+          nb[q] = b_ - cm * k1 - ap * k2;
+          nd[q] = d_ - dm * k1 - dp * k2;
         }
-        if (row + h < M) {
-          k2 = cdiv(c_, rb[row + h]);
-          ap = ra[row + h];
-          cp = rc[row + h];
-          dp = rd[row + h];
-        }
-        na[q] = cx<double>{0.0, 0.0} - am * k1;
-        nc[q] = cx<double>{0.0, 0.0} - cp * k2;
-        nb[q] = b_ - cm * k1 - ap * k2;
-        nd[q] = d_ - dm * k1 - dp * k2;
       }
       __syncthreads();
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int row = 2 * p + q;
-        ra[row] = na[q];
-        rb[row] = nb[q];
-        rc[row] = nc[q];
-        rd[row] = nd[q];
+        for (int q = 0; q < 2; ++q) {
+          const int row = 2 * p + q;
+          ra[row] = na[q];
+          rb[row] = nb[q];
+          rc[row] = nc[q];
+          rd[row] = nd[q];
+        }
       }
       __syncthreads();
     }
-    const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
-    const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
-
-    cx<R> *sc = (a.sol && static_cast<int>(st) < a.nz)
-                    ? a.sol + b * a.sol_fstride + static_cast<size_t>(st) * N
-                    : nullptr;
-    for (int i = s; i <= e; ++i) {
-      cx<double> x;
-      if (i == s) x = xs;
-      else if (i == e) x = xe;
-      else {
-        const int j = i - s;
-        x = cd[j * P + p] - ca[j * P + p] * xs - cc[j * P + p] * xe;
+    if (active) {
+      const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
+      const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
+      cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
+                      ? sol + J.soff + static_cast<size_t>(st) * N
+                      : nullptr;
+      for (int i = s; i <= e; ++i) {
+        cx<double> x;
+        if (i == s) x = xs;
+        else if (i == e) x = xe;
+        else {
+          const int j = i - s;
+          x = cd[j * P + p] - ca[j * P + p] * xs - cc[j * P + p] * xe;
+        }
+        if (J.absorb_npix) x = scale(x, absorb_factor(i, N, J.absorb_npix, J.absorb_inv));
+        const cx<R> xr = cvt<R>(x);
+        next[i] = xr;
+        if (sc) sc[i] = xr;
       }
-      if (a.absorb_npix) x = scale(x, absorb_factor(i, N, a.absorb_npix, a.absorb_inv));
-      const cx<R> xr = cvt<R>(x);
-      next[i] = xr;
-      if (sc) sc[i] = xr;
     }
     __syncthreads();
   }
-  if (a.nsteps & 1)
+  if ((nsteps & 1) && active)
     for (int i = s; i <= e; ++i) bufA[i] = bufB[i];
 }
 
-template <class R>
-hipError_t launch_cn_march(const TriArgs<R> &a, hipStream_t s)
+size_t cn_job_scratch(int nx, int esz)
 {
-  const CnPartition cp = cn_partition(a.n);
-  const size_t per = cn_field_scratch(a.n, sizeof(cx<R>), cp);
-  hipLaunchKernelGGL(k_cn_march<R>, dim3(a.batch), dim3(cp.P), 0, s, a, cp.P, cp.m, per);
+  return cn_field_scratch(nx, esz, cn_partition(nx));
+}
+
+template <class R>
+hipError_t launch_cn_jobs(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch,
+                          cx<R> *sol, unsigned first, unsigned nsteps, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cn_jobs<R>, dim3(njobs), dim3(max_P), 0, s, jobs_dev, fields,
+                     static_cast<char *>(scratch), sol, first, nsteps);
   return hipGetLastError();
 }
 
@@ -501,8 +519,10 @@ hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
   return hipGetLastError();
 }
 
-template hipError_t launch_cn_march<float>(const TriArgs<float> &, hipStream_t);
-template hipError_t launch_cn_march<double>(const TriArgs<double> &, hipStream_t);
+template hipError_t launch_cn_jobs<float>(const CnJob *, int, int, cx<float> *, void *, cx<float> *,
+                                          unsigned, unsigned, hipStream_t);
+template hipError_t launch_cn_jobs<double>(const CnJob *, int, int, cx<double> *, void *, cx<double> *,
+                                           unsigned, unsigned, hipStream_t);
 template hipError_t launch_adi_half<float>(const TriArgs<float> &, bool, hipStream_t);
 template hipError_t launch_adi_half<double>(const TriArgs<double> &, bool, hipStream_t);
 

@@ -29,8 +29,25 @@ struct TriArgs {
 
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
 
+// One CN job: a field with its own grid, wavenumber and material, solved by
+// one work-group (k_cn_jobs).  Offsets index the plan's field, solution and
+// scratch buffers.
+struct CnJob {
+  int nx, P, m, bc;
+  int nz;
+  unsigned absorb_npix;
+  size_t foff;   // elements into the field buffer
+  size_t soff;   // elements into the solution buffer, ~0 when not stored
+  size_t scoff;  // bytes into the scratch buffer
+  double xmin, dx, zmin, dz, k;
+  double absorb_inv;
+  MatDev mat;
+};
+// Scratch bytes of one CN job of nx rows (256-byte aligned blocks).
+size_t cn_job_scratch(int nx, int esz);
 template <class R>
-hipError_t launch_cn_march(const TriArgs<R> &a, hipStream_t s);
+hipError_t launch_cn_jobs(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch,
+                          cx<R> *sol, unsigned first, unsigned nsteps, hipStream_t s);
 template <class R>
 hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
 hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,

@@ -32,7 +32,8 @@ EXPORTS = [
     "pxg_set_field", "pxg_get_field", "pxg_get_solution", "pxg_solution_dims",
     "pxg_set_material_csg", "pxg_set_material_curved_wg", "pxg_set_material_planes",
     "pxg_set_material_none", "pxg_step", "pxg_sync", "pxg_step_profiled",
-    "pxg_field_device_ptr", "pxg_thomas_batched",
+    "pxg_field_device_ptr", "pxg_thomas_batched", "pxg_plan_create_jobs",
+    "pxg_set_job_material_curved_wg", "pxg_job_solution_dims",
 ]
 
 
@@ -96,6 +97,9 @@ def lib():
         L.pxg_step_profiled.argtypes = [P, U, U, P, I, P, P]
         L.pxg_field_device_ptr.argtypes = [P, I, P]
         L.pxg_thomas_batched.argtypes = [P, I, I, I, P, P, P]
+        L.pxg_plan_create_jobs.argtypes = [P, P, I, P]
+        L.pxg_set_job_material_curved_wg.argtypes = [P, I, P]
+        L.pxg_job_solution_dims.argtypes = [P, I, P]
         _LIB = L
     return _LIB
 
@@ -264,3 +268,43 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class JobPlan(Plan):
+    """Independent CN problems with their own grids in one plan (pxg_plan_create_jobs)."""
+
+    def __init__(self, ctx, probs):
+        self.ctx = ctx
+        self.probs = list(probs)
+        arr = (Problem * len(self.probs))(*self.probs)
+        self.h = C.c_void_p()
+        _check(lib().pxg_plan_create_jobs(ctx.h, arr, len(self.probs), C.byref(self.h)))
+        self.is3d = False
+        self.prob = self.probs[0]
+
+    def shape_of(self, b):
+        return (self.probs[b].nx,)
+
+    def set_field(self, field, b=0):
+        f = np.ascontiguousarray(field, dtype=np.complex128)
+        if f.shape != self.shape_of(b):
+            raise ValueError(f"field shape {f.shape} != {self.shape_of(b)}")
+        _check(lib().pxg_set_field(self.h, b, _p(f)))
+
+    def get_field(self, b=0):
+        out = np.empty(self.shape_of(b), dtype=np.complex128)
+        _check(lib().pxg_get_field(self.h, b, _p(out)))
+        return out
+
+    def material_curved_job(self, b, **kw):
+        d = dict(in_delta=0.0, in_beta=0.0, wglength=1e10, use_smoothed=0)
+        d.update(kw)
+        wg = CurvedWG(**d)
+        _check(lib().pxg_set_job_material_curved_wg(self.h, b, C.byref(wg)))
+
+    def get_solution(self, b=0):
+        d = (C.c_int * 3)()
+        _check(lib().pxg_job_solution_dims(self.h, b, d))
+        out = np.empty((d[0], d[1]), dtype=np.complex128, order="F")
+        _check(lib().pxg_get_solution(self.h, b, _p(out), out.size))
+        return out

@@ -158,3 +158,41 @@ def test_thomas_batched_matches_reference_fixtures(ctx):
         ref = O.thomas(D[i], S[i], B[i])
         assert rel_l2(xc[i], ref) < 1e-12
         assert rel_l2(xi[:, i], ref) < 1e-12
+
+
+def test_cn_job_plan_uneven_curved_guides(ctx):
+    """Config-3 style: jobs of uneven Nx, wavelengths and curved guides in one
+    launch; each equals its own oracle run (FP64) / stays within 1e-5 (mixed)."""
+    from scenes import splitmix64
+    g = splitmix64(0x5EED0003)
+    jobs = []
+    for _ in range(6):
+        nx = int(round(np.exp(np.log(256) + (np.log(2048) - np.log(256)) * next(g))))
+        lam = 0.05 + 0.15 * next(g)
+        R = 1e6 + 49e6 * next(g)
+        w = 50 + 150 * next(g)
+        jobs.append((nx, lam, R, w))
+    nsteps, dz, L = 60, 100.0, 6000.0
+    for prec, tol in ((P.PREC_FP64, 1e-10), (P.PREC_MIXED, 1e-5)):
+        probs, refs = [], []
+        for nx, lam, R, w in jobs:
+            lo, hi = -L * L / (2 * R) - 2 * w - 200, w + 200
+            dx = (hi - lo) / (nx - 1)
+            probs.append(P.problem(P.CN, nx, nsteps + 1, dx, dz, 2 * np.pi / lam, xmin=lo,
+                                   precision=prec, bc=P.BC_TRANSPARENT, store_dsx=1, store_dsz=1))
+            gg = O.grid(lo, hi, dx, 0.0, nsteps * dz, dz, 2 * np.pi / lam)
+            m = O.CurvedMaterial(R=R, width=w, clad_delta=4.5e-5, clad_beta=3.5e-6, dx=dx, dz=dz)
+            refs.append(O.cn_run(gg, m, np.ones(nx, complex), 1, nsteps))
+        plan = P.JobPlan(ctx, probs)
+        for b, (nx, lam, R, w) in enumerate(jobs):
+            plan.material_curved_job(b, R=R, width=w, clad_delta=4.5e-5, clad_beta=3.5e-6,
+                                     dx=probs[b].dx, dz=dz)
+            plan.set_field(np.ones(nx, complex), b)
+        plan.step(1, nsteps)
+        for b in range(len(jobs)):
+            err = rel_l2(plan.get_field(b), refs[b])
+            announce("cn job", b, jobs[b][0], prec, err)
+            assert err < tol
+            sol = plan.get_solution(b)
+            assert sol.shape == (jobs[b][0], nsteps + 1)
+            assert rel_l2(sol[:, -1], refs[b]) < tol
