@@ -121,6 +121,11 @@ const char *pxg_version(void);
  * and solution storage for the problem. */
 int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out);
 int pxg_plan_destroy(pxg_plan plan);
+/* Reconfigures a plan for a new problem of the same kind, precision, nx, batch
+ * and storage shape without reallocating device memory (grids, wavenumber,
+ * absorber and quirks may change); the material resets to none and the field
+ * must be set again.  Extension used by the server to recycle plans. */
+int pxg_plan_reset(pxg_plan plan, const pxg_problem *prob);
 /* Bytes of device memory the plan reserves (for admission by HBM budget). */
 int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes);
 /* Job plan: n independent problems of one kind in one plan, each with its own
@@ -195,6 +200,49 @@ int pxg_field_device_ptr(pxg_plan plan, int b, void **dptr);
 enum { PXG_LAYOUT_CONTIGUOUS = 0, PXG_LAYOUT_INTERLEAVED = 1 };
 int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
                        double *rhs);
+
+/* ---- serving ------------------------------------------------------------------ */
+/* A persistent request loop (no counterpart in the reference, which runs one
+ * simulation per process): jobs of mixed kinds and grid sizes are admitted by
+ * HBM budget into slots (one plan and one stream each), advance in chunks of
+ * z-steps so that the work of different jobs overlaps on the device, and leave
+ * as soon as their result is on the host; queued jobs are admitted as slots
+ * and memory free up.  Latency is measured from submission to result. */
+typedef struct pxg_server_s *pxg_server;
+enum { PXG_JOB_QUEUED = 0, PXG_JOB_RUNNING = 1, PXG_JOB_DONE = 2, PXG_JOB_FAILED = 3, PXG_JOB_CANCELLED = 4 };
+enum { PXG_MAT_NONE = 0, PXG_MAT_CSG = 1, PXG_MAT_CURVED_WG = 2 };
+typedef struct {
+  pxg_problem prob;              /* batch must be 1 */
+  int material;                  /* PXG_MAT_* */
+  const pxg_shape *shapes;       /* CSG arrays, copied at submission */
+  int n_shapes;
+  const pxg_part *parts;
+  int n_parts;
+  const pxg_module *modules;
+  int n_modules;
+  pxg_curved_wg wg;
+  const double *field;           /* initial field (complex128); read at admission */
+  double *result;                /* exit field (complex128); written at completion */
+  unsigned first_step, nsteps;
+  int inject_failure_after_chunks;  /* < 0: none; >= 0 fails the job from the host (tests) */
+} pxg_job;
+typedef struct {
+  uint64_t submitted, completed, failed, cancelled, rejected, running, queued;
+  double p50_ms, p99_ms, mean_ms, max_ms;   /* submission -> result on host, completed jobs */
+  double cell_steps;                        /* sum over completed jobs */
+  double busy_s;                            /* wall time with at least one job running */
+  size_t hbm_reserved, hbm_budget;
+} pxg_server_stats;
+/* slots: jobs running at once; hbm_budget: bytes of plans (0 = 64 GiB);
+ * chunk_steps: z-steps enqueued per job before the next job is served. */
+int pxg_server_create(pxg_ctx ctx, int slots, size_t hbm_budget, unsigned chunk_steps, pxg_server *out);
+int pxg_server_submit(pxg_server s, const pxg_job *job, uint64_t *id);
+int pxg_server_status(pxg_server s, uint64_t id, int *state);
+/* Blocks until the job ends (timeout_s <= 0: no timeout); returns its status. */
+int pxg_server_wait(pxg_server s, uint64_t id, double *latency_ms, double timeout_s);
+int pxg_server_cancel(pxg_server s, uint64_t id);
+int pxg_server_stats_get(pxg_server s, pxg_server_stats *st);
+int pxg_server_destroy(pxg_server s);
 
 #ifdef __cplusplus
 }

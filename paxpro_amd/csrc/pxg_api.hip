@@ -65,21 +65,31 @@ int guarded(F &&f)
 const double PI = 3.14159265358979323846264338327950288;
 
 // Device buffer owned by a plan.
+// Re-allocation to a size within the current capacity reuses the buffer, so a
+// plan that is reconfigured (pxg_plan_reset) does not call hipMalloc/hipFree,
+// which would synchronise the device while other plans run.
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  size_t cap = 0;
   void alloc(size_t n)
   {
+    if (n <= cap && p) {
+      bytes = n;
+      return;
+    }
     release();
     if (n == 0) return;
     hip_check(hipMalloc(&p, n), "hipMalloc");
     bytes = n;
+    cap = n;
   }
   void release()
   {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    cap = 0;
   }
   template <class T>
   T *as() const { return static_cast<T *>(p); }
@@ -625,6 +635,11 @@ DBox shape_box(const pxg_shape &s)
 
 }  // namespace
 
+namespace pxg {
+hipStream_t plan_stream(pxg_plan plan) { return plan ? plan->stream : nullptr; }
+int ctx_device(pxg_ctx ctx) { return ctx ? ctx->device : 0; }
+}  // namespace pxg
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -814,6 +829,30 @@ int pxg_plan_create_jobs(pxg_ctx ctx, const pxg_problem *probs, int n, pxg_plan 
     g_err = keep;
   }
   return rc;
+}
+
+int pxg_plan_reset(pxg_plan P, const pxg_problem *prob)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!prob) fail(PXG_ERR_INVALID, "null problem");
+    validate(*prob);
+    const pxg_problem &o = P->p;
+    if (P->is_jobs || prob->kind != o.kind || prob->precision != o.precision || prob->nx != o.nx ||
+        prob->batch != o.batch || prob->store_dsx != o.store_dsx || prob->store_dsz != o.store_dsz ||
+        (prob->store_dsz > 0 && prob->nz != o.nz))
+      fail(PXG_ERR_INVALID, "pxg_plan_reset needs the same kind, precision, nx, batch and storage shape");
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    P->p = *prob;
+    if (P->p.n_refr == 0) P->p.n_refr = 10;
+    P->k_host.assign(prob->batch, prob->k);
+    P->mat_mode = MAT_NONE;
+    P->field_set = false;
+    P->jobs_dirty = true;
+    rebuild_tables(P);
+    if (P->sol.p) hip_check(hipMemsetAsync(P->sol.p, 0, P->sol.bytes, P->stream), "memset");
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+  });
 }
 
 int pxg_set_job_material_curved_wg(pxg_plan P, int job, const pxg_curved_wg *wg)

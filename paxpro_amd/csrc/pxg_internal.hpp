@@ -2,6 +2,7 @@
 // translation units.  Not part of the public interface.
 #pragma once
 
+#include "../../include/paxgpu.h"
 #include "pxg_common.hpp"
 #include "pxg_material.hpp"
 
@@ -43,6 +44,9 @@ hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube
                              cx<double> *cube, size_t cube_fstride, int slice, int batch,
                              hipStream_t s);
 int max_lines(int n, int elem_bytes);
+// Stream of a plan and device of a context (server internals).
+hipStream_t plan_stream(pxg_plan plan);
+int ctx_device(pxg_ctx ctx);
 bool fft_size_supported(int n);
 
 }  // namespace pxg

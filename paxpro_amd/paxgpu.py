@@ -33,8 +33,12 @@ EXPORTS = [
     "pxg_set_material_csg", "pxg_set_material_curved_wg", "pxg_set_material_planes",
     "pxg_set_material_none", "pxg_step", "pxg_sync", "pxg_step_profiled",
     "pxg_field_device_ptr", "pxg_thomas_batched", "pxg_plan_create_jobs",
-    "pxg_set_job_material_curved_wg", "pxg_job_solution_dims",
+    "pxg_set_job_material_curved_wg", "pxg_job_solution_dims", "pxg_plan_reset",
+    "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
+    "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
 ]
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
 
 
 class PxgError(RuntimeError):
@@ -59,6 +63,22 @@ class CurvedWG(C.Structure):
                 ("clad_beta", C.c_double), ("in_delta", C.c_double), ("in_beta", C.c_double),
                 ("wglength", C.c_double), ("dx", C.c_double), ("dz", C.c_double),
                 ("use_smoothed", C.c_int)]
+
+
+class Job(C.Structure):
+    _fields_ = [("prob", Problem), ("material", C.c_int), ("shapes", C.c_void_p),
+                ("n_shapes", C.c_int), ("parts", C.c_void_p), ("n_parts", C.c_int),
+                ("modules", C.c_void_p), ("n_modules", C.c_int), ("wg", CurvedWG),
+                ("field", C.c_void_p), ("result", C.c_void_p), ("first_step", C.c_uint),
+                ("nsteps", C.c_uint), ("inject_failure_after_chunks", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "cancelled",
+                                          "rejected", "running", "queued")] + \
+               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "cell_steps",
+                                          "busy_s")] + \
+               [("hbm_reserved", C.c_size_t), ("hbm_budget", C.c_size_t)]
 
 
 class KernelStat(C.Structure):
@@ -98,6 +118,14 @@ def lib():
         L.pxg_field_device_ptr.argtypes = [P, I, P]
         L.pxg_thomas_batched.argtypes = [P, I, I, I, P, P, P]
         L.pxg_plan_create_jobs.argtypes = [P, P, I, P]
+        L.pxg_plan_reset.argtypes = [P, P]
+        L.pxg_server_create.argtypes = [P, I, S, U, P]
+        L.pxg_server_submit.argtypes = [P, P, P]
+        L.pxg_server_status.argtypes = [P, C.c_uint64, P]
+        L.pxg_server_wait.argtypes = [P, C.c_uint64, P, D]
+        L.pxg_server_cancel.argtypes = [P, C.c_uint64]
+        L.pxg_server_stats_get.argtypes = [P, P]
+        L.pxg_server_destroy.argtypes = [P]
         L.pxg_set_job_material_curved_wg.argtypes = [P, I, P]
         L.pxg_job_solution_dims.argtypes = [P, I, P]
         _LIB = L
@@ -180,6 +208,10 @@ class Plan:
 
     def shape(self):
         return (self.nx, self.nx) if self.is3d else (self.nx,)
+
+    def reset(self, prob):
+        _check(lib().pxg_plan_reset(self.h, C.byref(prob)))
+        self.prob = prob
 
     def set_field(self, field, b=0):
         order = "F" if self.is3d else "C"
@@ -308,3 +340,75 @@ class JobPlan(Plan):
         out = np.empty((d[0], d[1]), dtype=np.complex128, order="F")
         _check(lib().pxg_get_solution(self.h, b, _p(out), out.size))
         return out
+
+
+class Server:
+    """pxg_server: persistent request loop with HBM-budget admission."""
+
+    def __init__(self, ctx, slots=4, budget=0, chunk_steps=32):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().pxg_server_create(ctx.h, slots, budget, chunk_steps, C.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, prob, field, nsteps, first_step=1, csg=None, curved=None, inject=-1):
+        is3d = prob.kind in (FFT3D, ADI)
+        f = np.asarray(field, dtype=np.complex128)
+        f = np.asfortranarray(f) if is3d else np.ascontiguousarray(f)
+        out = np.empty_like(f, order="F" if is3d else "C")
+        job = Job()
+        job.prob = prob
+        keep = [f, out]
+        if csg is not None:
+            s, pa, m = (np.ascontiguousarray(csg[0], SHAPE_DTYPE), np.ascontiguousarray(csg[1], PART_DTYPE),
+                        np.ascontiguousarray(csg[2], MODULE_DTYPE))
+            keep += [s, pa, m]
+            job.material = MAT_CSG
+            job.shapes, job.n_shapes = _p(s), len(s)
+            job.parts, job.n_parts = _p(pa), len(pa)
+            job.modules, job.n_modules = _p(m), len(m)
+        elif curved is not None:
+            d = dict(in_delta=0.0, in_beta=0.0, wglength=1e10, use_smoothed=0)
+            d.update(curved)
+            job.material = MAT_CURVED_WG
+            job.wg = CurvedWG(**d)
+        else:
+            job.material = MAT_NONE
+        job.field, job.result = _p(f), _p(out)
+        job.first_step, job.nsteps = first_step, nsteps
+        job.inject_failure_after_chunks = inject
+        jid = C.c_uint64()
+        _check(lib().pxg_server_submit(self.h, C.byref(job), C.byref(jid)))
+        self._keep[jid.value] = (keep, out)
+        return jid.value
+
+    def status(self, jid):
+        s = C.c_int()
+        _check(lib().pxg_server_status(self.h, jid, C.byref(s)))
+        return s.value
+
+    def wait(self, jid, timeout=0.0):
+        """Returns (status code, latency ms, result array)."""
+        lat = C.c_double()
+        rc = lib().pxg_server_wait(self.h, jid, C.byref(lat), timeout)
+        keep, out = self._keep.pop(jid)
+        return rc, lat.value, out
+
+    def cancel(self, jid):
+        _check(lib().pxg_server_cancel(self.h, jid))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().pxg_server_stats_get(self.h, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            _check(lib().pxg_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
