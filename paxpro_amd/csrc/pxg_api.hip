@@ -259,7 +259,6 @@ cx<double> screen_factor(double k, double dz, double delta, double beta, double 
   return {std::cos(ph) * amp * inv_norm, std::sin(ph) * amp * inv_norm};
 }
 
-template <class R>
 void build_tables(pxg_plan P)
 {
   const pxg_problem &p = P->p;
@@ -271,7 +270,7 @@ void build_tables(pxg_plan P)
       const double a = 2.0 * PI * (static_cast<double>(m) / n);
       tw[m] = {std::cos(a), -std::sin(a)};
     }
-    upload_cx<R>(P->tw, tw, s);
+    upload_cx<double>(P->tw, tw, s);
     std::vector<cx<double>> kx(static_cast<size_t>(n) * p.batch), ky(kx.size());
     for (int b = 0; b < p.batch; ++b)
       for (unsigned i = 0; i < n; ++i) {
@@ -279,11 +278,11 @@ void build_tables(pxg_plan P)
         ky[b * n + i] = kernel_half(spatial_freq(i, n, p.kind == PXG_FFT3D ? p.dy : p.dx), p.dz,
                                     P->k_host[b]);
       }
-    upload_cx<R>(P->kx, kx, s);
-    upload_cx<R>(P->ky, ky, s);
+    upload_cx<double>(P->kx, kx, s);
+    upload_cx<double>(P->ky, ky, s);
     if (p.kind == PXG_FFT3D && (p.absorb_npix_x || p.absorb_npix_y)) {
-      upload_real<R>(P->absx, absorber_table(n, p.absorb_npix_x, p.absorb_inv_x), s);
-      upload_real<R>(P->absy, absorber_table(n, p.absorb_npix_y, p.absorb_inv_y), s);
+      upload_real<double>(P->absx, absorber_table(n, p.absorb_npix_x, p.absorb_inv_x), s);
+      upload_real<double>(P->absy, absorber_table(n, p.absorb_npix_y, p.absorb_inv_y), s);
     } else {
       P->absx.release();
       P->absy.release();
@@ -295,7 +294,6 @@ void build_tables(pxg_plan P)
   hip_check(hipStreamSynchronize(s), "upload sync");
 }
 
-template <class R>
 void build_part_factors(pxg_plan P)
 {
   const pxg_problem &p = P->p;
@@ -310,18 +308,13 @@ void build_part_factors(pxg_plan P)
     }
     pf[b * (np + 1) + np] = screen_factor(P->k_host[b], p.dz, 0.0, 0.0, inv_norm);
   }
-  upload_cx<R>(P->part_fac, pf, P->stream);
+  upload_cx<double>(P->part_fac, pf, P->stream);
 }
 
 void rebuild_tables(pxg_plan P)
 {
-  if (P->mixed) {
-    build_tables<float>(P);
-    if (P->mat_mode == MAT_CSG) build_part_factors<float>(P);
-  } else {
-    build_tables<double>(P);
-    if (P->mat_mode == MAT_CSG) build_part_factors<double>(P);
-  }
+  build_tables(P);
+  if (P->mat_mode == MAT_CSG) build_part_factors(P);
 }
 
 MatDev mat_dev(pxg_plan P)
@@ -370,11 +363,11 @@ FftArgs<R> fft_args(pxg_plan P)
   FftArgs<R> a{};
   a.field = P->field.as<cx<R>>();
   a.fstride = P->cells;
-  a.tw = P->tw.as<cx<R>>();
-  a.kx = P->kx.as<cx<R>>();
-  a.ky = P->ky.as<cx<R>>();
-  a.absx = P->absx.as<R>();
-  a.absy = P->absy.as<R>();
+  a.tw = P->tw.as<cd>();
+  a.kx = P->kx.as<cd>();
+  a.ky = P->ky.as<cd>();
+  a.absx = P->absx.as<double>();
+  a.absy = P->absy.as<double>();
   a.kvec = P->kvec.as<double>();
   a.mat = mat_dev(P);
   a.xmin = p.xmin;
@@ -479,6 +472,7 @@ TriArgs<R> tri_args(pxg_plan P)
   a.sol = p.store_dsz > 0 ? P->sol.as<cx<R>>() : nullptr;
   a.sol_fstride = P->sol_fstride;
   a.nz = p.nz;
+  a.kmin = *std::min_element(P->k_host.begin(), P->k_host.end());
   return a;
 }
 
@@ -715,9 +709,8 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
     P->k_host.assign(prob->batch, prob->k);
     if (prob->kind == PXG_FFT3D) {
-      const int ml = max_lines(prob->nx, P->esz);
-      P->lines_y = ml;
-      P->lines_x = ml;
+      P->lines_y = max_lines(prob->nx, 81920);    // two work-groups per CU
+      P->lines_x = max_lines(prob->nx, 162816);   // one wide work-group per CU (coalescing)
       // keep enough work-groups in flight for small batches
       while (P->lines_y > 1 && (prob->nx / P->lines_y) * prob->batch < 512) P->lines_y /= 2;
       while (P->lines_x > 4 && (prob->nx / P->lines_x) * prob->batch < 512) P->lines_x /= 2;
@@ -1091,10 +1084,7 @@ int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, cons
     P->is_ref = is_reference_run;
     P->mat_mode = MAT_CSG;
     P->jobs_dirty = true;
-    if (P->p.kind == PXG_FFT3D) {
-      if (P->mixed) build_part_factors<float>(P);
-      else build_part_factors<double>(P);
-    }
+    if (P->p.kind == PXG_FFT3D) build_part_factors(P);
   });
 }
 

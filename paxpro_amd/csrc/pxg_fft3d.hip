@@ -21,22 +21,11 @@ namespace pxg {
 constexpr int MAX_MASK_WORDS = 16;  // CSG parts per material: up to 1024
 
 // Phase-screen factor exp(-k (beta + i delta) dz) / N^2 from FP64 material
-// values; the phase argument stays in FP64 and is reduced mod 2 pi before the
-// sine and cosine (src/fftSolver3D.cpp:154).
-template <class R>
-__device__ __forceinline__ cx<R> screen_from_values(double k, double dz, double delta, double beta,
-                                                    double inv_norm)
+// values, in FP64 (src/fftSolver3D.cpp:154).
+__device__ __forceinline__ cd screen_from_values(double k, double dz, double delta, double beta,
+                                                 double inv_norm)
 {
-  double amp = exp((-k * beta) * dz) * inv_norm;
-  cx<R> e = expi_fp64<R>((-k * delta) * dz);
-  return {static_cast<R>(e.x * amp), static_cast<R>(e.y * amp)};
-}
-
-template <>
-__device__ __forceinline__ cx<double> screen_from_values<double>(double k, double dz, double delta,
-                                                                 double beta, double inv_norm)
-{
-  double amp = exp((-k * beta) * dz);
+  const double amp = exp((-k * beta) * dz);
   double s, c;
   sincos((-k * delta) * dz, &s, &c);
   return {c * amp * inv_norm, s * amp * inv_norm};
@@ -67,6 +56,41 @@ __device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_
   beta /= (2.0 * n_refr);
 }
 
+// FFTSolver3D::refraction for one cell (src/fftSolver3D.cpp:122-156): material at
+// z1 and z0; a changed material (|d delta| or |d beta| > 1e-10) takes the
+// crossing integral, otherwise the part's precomputed factor is used.
+__device__ __forceinline__ cd csg_screen(const MatDev &m, const uint64_t *mask, int nw, const cd *pf,
+                                      double k, double dz, double inv_norm, unsigned n_refr,
+                                      unsigned quirks, double x, double y, double z0, double z1)
+{
+  const int e1 = csg_eval(m, mask, nw, x, y, z1);
+  if (nw == 0) return pf[e1];
+  const int e0 = csg_eval(m, mask, nw, x, y, z0);
+  if (e1 == e0) return pf[e1];
+  double d1, b1, d0, b0;
+  part_values(m, e1, d1, b1);
+  part_values(m, e0, d0, b0);
+  if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
+    crossing_integral(m, mask, nw, n_refr, quirks, x, y, z0, z1, d1, b1);
+    return screen_from_values(k, dz, d1, b1, inv_norm);
+  }
+  return pf[e1];
+}
+
+template <class R>
+__device__ __forceinline__ void load8(cd (&v)[8], const cx<R> *f, size_t stride)
+{
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = cvt<double>(f[m * stride]);
+}
+
+template <class R>
+__device__ __forceinline__ void store8(const cd (&v)[8], cx<R> *f, size_t stride)
+{
+#pragma unroll
+  for (int m = 0; m < 8; ++m) f[m * stride] = cvt<R>(v[m]);
+}
+
 // ---------------------------------------------------------------------------
 // Pass Y: contiguous y-lines (matrix columns); lane t of line `line` holds rows
 // t + m*N/8.  L lines (columns) per work-group, blockIdx.y = field.
@@ -74,55 +98,44 @@ template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 {
   constexpr int T = N / 8;
-  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
   const int c = blockIdx.x * L + line;
   const int b = blockIdx.y;
-  cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N;
-  cx<R> *ll = lds + line * LdsLine<N>::STRIDE;
-  cx<R> v[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = f[t + m * T];
+  cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
+  cd *ll = lds + line * LdsLine<N>::STRIDE;
+  cd v[8];
+  load8(v, f, T);
 
   if (a.mode & Y_INV) {
     fft_line<N, +1>(v, ll, t, a.tw);
     const double k = a.kvec[b];
     const double x = coord(a.xmin, c, a.dx);
     if (a.mat.mode == MAT_NONE) {
-      const R s = static_cast<R>(a.inv_norm);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = scale(v[m], s);
+      for (int m = 0; m < 8; ++m) v[m] = scale(v[m], a.inv_norm);
     } else if (a.mat.mode == MAT_CSG) {
       const double z1 = coord(a.zmin, a.step, a.dz);
       const double z0 = coord(a.zmin, a.step - 1, a.dz);
-      const int nw = (a.mat.n_parts + 63) >> 6;
       const int c0 = blockIdx.x * L;
-      csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx),
-               a.ymin, coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
-      const cx<R> *pf = static_cast<const cx<R> *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
+      int nw = (a.mat.n_parts + 63) >> 6;
+      csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx), a.ymin,
+               coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
+      bool any = false;
+      for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
+      if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
+      const cd *pf = static_cast<const cd *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
+      // Factors go to this lane's own canonical LDS slots (read only by this
+      // lane in the last stage of the transform), then multiply the registers.
 #pragma unroll 1
       for (int m = 0; m < 8; ++m) {
-        const int r = t + m * T;
-        const double y = coord(a.ymin, r, a.dy);
-        const int e1 = csg_eval(a.mat, mask, nw, x, y, z1);
-        const int e0 = csg_eval(a.mat, mask, nw, x, y, z0);
-        cx<R> fac;
-        if (e1 == e0) {
-          fac = pf[e1];
-        } else {
-          double d1, b1, d0, b0;
-          part_values(a.mat, e1, d1, b1);
-          part_values(a.mat, e0, d0, b0);
-          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
-            crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, y, z0, z1, d1, b1);
-            fac = screen_from_values<R>(k, a.dz, d1, b1, a.inv_norm);
-          } else {
-            fac = pf[e1];
-          }
-        }
-        v[m] = v[m] * fac;
+        const double y = coord(a.ymin, t + m * T, a.dy);
+        ll[lds_pos(t + m * T)] =
+            csg_screen(a.mat, mask, nw, pf, k, a.dz, a.inv_norm, a.n_refr, a.quirks, x, y, z0, z1);
       }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = v[m] * ll[lds_pos(t + m * T)];
     } else {  // MAT_PLANES: effective (delta, beta) of this step per cell
       const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
       const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride +
@@ -130,99 +143,95 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int r = t + m * T;
-        v[m] = v[m] * screen_from_values<R>(k, a.dz, pl[2 * r], pl[2 * r + 1], a.inv_norm);
+        v[m] = v[m] * screen_from_values(k, a.dz, pl[2 * r], pl[2 * r + 1], a.inv_norm);
       }
     }
     if (a.absy) {
-      const R ax = a.absx[c];
+      const double ax = a.absx[c];
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = scale(scale(v[m], a.absy[t + m * T]), ax);
     }
   }
   if (a.mode & Y_FWD) fft_line<N, -1>(v, ll, t, a.tw);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) f[t + m * T] = v[m];
+  store8(v, f, T);
 }
 
 // ---------------------------------------------------------------------------
 // Pass X: strided x-lines (matrix rows).  Lanes run line-fastest so that each
-// load/store instruction touches L consecutive rows of a column (L*8 or L*16
-// contiguous bytes).  FFT_x, multiply by Kx(c) Ky(r), IFFT_x.
+// load/store instruction touches L consecutive rows of a column.
+// FFT_x, multiply by Kx(c) Ky(r), IFFT_x.
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 {
   constexpr int T = N / 8;
-  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
   const int tid = threadIdx.x, l = tid % L, t = tid / L;
   const int r = blockIdx.x * L + l;
   const int b = blockIdx.y;
-  cx<R> *f = a.field + b * a.fstride + r;
-  cx<R> *ll = lds + l * LdsLine<N>::STRIDE;
-  cx<R> v[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = f[static_cast<size_t>(t + m * T) * N];
+  cx<R> *f = a.field + b * a.fstride + r + static_cast<size_t>(t) * N;
+  cd *ll = lds + l * LdsLine<N>::STRIDE;
+  cd v[8];
+  load8(v, f, static_cast<size_t>(T) * N);
   fft_line<N, -1>(v, ll, t, a.tw);
-  const cx<R> kyr = a.ky[b * N + r];
-  const cx<R> *kx = a.kx + b * N;
+  const cd kyr = a.ky[b * N + r];
+  const cd *kx = a.kx + b * N;
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[t + m * T] * kyr);
   fft_line<N, +1>(v, ll, t, a.tw);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) f[static_cast<size_t>(t + m * T) * N] = v[m];
+  store8(v, f, static_cast<size_t>(T) * N);
 }
 
 // ---------------------------------------------------------------------------
 // FFTSolver2D step (src/fftSolver2D.cpp:15-74): one field per line,
 // FFT -> x exp(-i dz/(2k) kx^2) -> IFFT -> x exp(-k(beta + i delta) dz)/N with
 // the material at z_step + dz/2; the result is also stored as solution column.
+__device__ __forceinline__ cd screen_2d(const MatDev &m, const uint64_t *mask, int nw, unsigned step,
+                                     int i, double x, double z, double k, double dz, double inv_norm)
+{
+  double d = 0.0, be = 0.0;
+  if (m.mode == MAT_CSG) {
+    part_values(m, csg_eval(m, mask, nw, x, 0.0, z), d, be);
+  } else if (m.mode == MAT_CURVED) {
+    curved_material(m.wg, x, z, d, be);
+  } else if (m.mode == MAT_PLANES) {
+    const int p = m.z_invariant ? 0 : static_cast<int>(step) - m.first_plane;
+    const double *pl = m.planes + static_cast<size_t>(p) * m.plane_stride;
+    d = pl[2 * i];
+    be = pl[2 * i + 1];
+  }
+  return screen_from_values(k, dz, d, be, inv_norm);
+}
+
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_fft2d_step(FftArgs<R> a)
 {
   constexpr int T = N / 8;
-  __shared__ cx<R> lds[L * LdsLine<N>::STRIDE];
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
   const int b = blockIdx.x * L + line;
-  cx<R> *f = a.field + b * a.fstride;
-  cx<R> *ll = lds + line * LdsLine<N>::STRIDE;
-  cx<R> v[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = f[t + m * T];
+  cx<R> *f = a.field + b * a.fstride + t;
+  cd *ll = lds + line * LdsLine<N>::STRIDE;
+  cd v[8];
+  load8(v, f, T);
   fft_line<N, -1>(v, ll, t, a.tw);
-  const cx<R> *kx = a.kx + b * N;
+  const cd *kx = a.kx + b * N;
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = v[m] * kx[t + m * T];
   fft_line<N, +1>(v, ll, t, a.tw);
   const double k = a.kvec[b];
   const double z = coord(a.zmin, a.step, a.dz) + 0.5 * a.dz;
-  if (a.mat.mode == MAT_CSG) {
-    const int nw = (a.mat.n_parts + 63) >> 6;
-    csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), 0.0, 0.0, z, z);
-  }
+  const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
+  if (a.mat.mode == MAT_CSG) csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), 0.0, 0.0, z, z);
 #pragma unroll 1
   for (int m = 0; m < 8; ++m) {
     const int i = t + m * T;
-    const double x = coord(a.xmin, i, a.dx);
-    double d = 0.0, be = 0.0;
-    if (a.mat.mode == MAT_CSG) {
-      part_values(a.mat, csg_eval(a.mat, mask, (a.mat.n_parts + 63) >> 6, x, 0.0, z), d, be);
-    } else if (a.mat.mode == MAT_CURVED) {
-      curved_material(a.mat.wg, x, z, d, be);
-    } else if (a.mat.mode == MAT_PLANES) {
-      const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
-      const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride;
-      d = pl[2 * i];
-      be = pl[2 * i + 1];
-    }
-    v[m] = v[m] * screen_from_values<R>(k, a.dz, d, be, a.inv_norm);
+    ll[lds_pos(i)] = screen_2d(a.mat, mask, nw, a.step, i, coord(a.xmin, i, a.dx), z, k, a.dz, a.inv_norm);
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) f[t + m * T] = v[m];
-  if (a.sol) {
-    cx<R> *sc = a.sol + b * a.sol_fstride + static_cast<size_t>(a.step) * N;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) sc[t + m * T] = v[m];
-  }
+  for (int m = 0; m < 8; ++m) v[m] = v[m] * ll[lds_pos(t + m * T)];
+  store8(v, f, T);
+  if (a.sol) store8(v, a.sol + b * a.sol_fstride + static_cast<size_t>(a.step) * N + t, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,20 +267,19 @@ __global__ void k_cube_store(const cx<R> *field, size_t fstride, int n, int cube
 // ---------------------------------------------------------------------------
 bool fft_size_supported(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
 
-// Lines per work-group: at most 1024 lanes and 80 KiB of line storage, so two
-// work-groups fit in the 160 KiB LDS of a gfx950 CU.
-constexpr int LDS_CAP = 81920;
-int max_lines(int n, int elem_bytes)
+// Lines per work-group: at most 1024 lanes and lds_cap bytes of FP64 line
+// storage (gfx950: 160 KiB of LDS per CU and per work-group).
+int max_lines(int n, int lds_cap)
 {
   int l = 1024 / (n / 8);
-  while (l > 1 && l * (n + n / 8 + 1) * elem_bytes > LDS_CAP) l /= 2;
+  while (l > 1 && l * (n + n / 8 + 1) * static_cast<int>(sizeof(cd)) > lds_cap) l /= 2;
   return l > 16 ? 16 : l;
 }
 
 #define PXG_LINES_CASE(KER, NN, LL)                                                      \
   case LL:                                                                               \
     if constexpr (LL * NN / 8 <= 1024 &&                                                 \
-                  LL * (NN + NN / 8 + 1) * sizeof(cx<R>) <= 81920) {                     \
+                  LL * (NN + NN / 8 + 1) * sizeof(cd) <= 163840 - 1024) {               \
       hipLaunchKernelGGL((KER<NN, R, LL>), grid_of(NN, LL), dim3(LL * NN / 8), 0, s, a); \
       return hipGetLastError();                                                          \
     }                                                                                    \
@@ -322,7 +330,7 @@ template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
   // one field per line; 4 lines per work-group when the batch divides evenly
-  const int lines = (batch % 4 == 0 && max_lines(n, sizeof(cx<R>)) >= 4) ? 4 : 1;
+  const int lines = (batch % 4 == 0 && max_lines(n, 81920) >= 4) ? 4 : 1;
   auto grid_of = [&](int, int ll) { return dim3(batch / ll, 1); };
   PXG_SIZE_SWITCH(k_fft2d_step)
 }

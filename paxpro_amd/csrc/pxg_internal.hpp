@@ -8,16 +8,17 @@
 
 namespace pxg {
 
-// Arguments of the split-step passes (FFT3D and FFT2D).
+// Arguments of the split-step passes (FFT3D and FFT2D).  Fields are stored as
+// cx<R> (complex128 or complex64); every table and all arithmetic are FP64.
 template <class R>
 struct FftArgs {
   cx<R> *field;            // batch of fields
   size_t fstride;          // elements between consecutive fields
-  const cx<R> *tw;         // tw[m] = exp(-2 pi i m / N), m < N
-  const cx<R> *kx;         // [batch][N] exp(-i dz/(2k) kx^2), column (x) index
-  const cx<R> *ky;         // [batch][N] exp(-i dz/(2k) ky^2), row (y) index
-  const R *absx;           // [N] absorber factor along x, or null
-  const R *absy;           // [N] absorber factor along y, or null
+  const cd *tw;            // tw[m] = exp(-2 pi i m / N), m < N
+  const cd *kx;            // [batch][N] exp(-i dz/(2k) kx^2), column (x) index
+  const cd *ky;            // [batch][N] exp(-i dz/(2k) ky^2), row (y) index
+  const double *absx;      // [N] absorber factor along x, or null
+  const double *absy;      // [N] absorber factor along y, or null
   const double *kvec;      // [batch] wavenumbers
   MatDev mat;
   double xmin, dx, ymin, dy, zmin, dz;
@@ -43,7 +44,8 @@ template <class R>
 hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
                              cx<double> *cube, size_t cube_fstride, int slice, int batch,
                              hipStream_t s);
-int max_lines(int n, int elem_bytes);
+// Lines per work-group for an LDS budget of lds_cap bytes (FP64 lines).
+int max_lines(int n, int lds_cap);
 // Stream of a plan and device of a context (server internals).
 hipStream_t plan_stream(pxg_plan plan);
 int ctx_device(pxg_ctx ctx);

@@ -51,6 +51,20 @@ struct MatDev {
   DCurved wg;
 };
 
+// sqrt(s) < r decided without the square root away from the boundary: with a
+// correctly rounded sqrt, s < r^2 (1 - 1e-14) implies sqrt(s) < r and
+// s > r^2 (1 + 1e-14) implies sqrt(s) > r; only the band between them takes
+// the square root, so the decision equals the reference's sqrt(s) < r.
+__device__ __forceinline__ bool radius_less(double s, double r)
+{
+#pragma clang fp contract(off)
+  if (!(r > 0.0)) return sqrt(s) < r;
+  const double r2 = r * r;
+  if (s < r2 * (1.0 - 1e-14)) return true;
+  if (s > r2 * (1.0 + 1e-14)) return false;
+  return sqrt(s) < r;
+}
+
 // Shape membership (src/shape.cpp:126-187), strict inequalities, FP64.
 __device__ __forceinline__ bool shape_inside(const DShape &s, double x, double y, double z)
 {
@@ -58,7 +72,7 @@ __device__ __forceinline__ bool shape_inside(const DShape &s, double x, double y
   double px = s.T[0] * x + s.T[1] * y + s.T[2] * z + s.T[3];
   double py = s.T[4] * x + s.T[5] * y + s.T[6] * z + s.T[7];
   double pz = s.T[8] * x + s.T[9] * y + s.T[10] * z + s.T[11];
-  if (s.type == 0) return sqrt(px * px + py * py + pz * pz) < s.p[0];
+  if (s.type == 0) return radius_less(px * px + py * py + pz * pz, s.p[0]);
   if (s.type == 1)
     return (px > -s.p[0] / 2.0) && (px < s.p[0] / 2.0) && (py > -s.p[1] / 2.0) &&
            (py < s.p[1] / 2.0) && (pz > -s.p[2] / 2.0) && (pz < s.p[2] / 2.0);
@@ -66,7 +80,7 @@ __device__ __forceinline__ bool shape_inside(const DShape &s, double x, double y
   if (!((pz < h / 2.0) && (pz > -h / 2.0))) return false;
   double slope = (s.p[1] - s.p[0]) / h;
   double radius = slope * (pz + h / 2.0) + s.p[0];
-  return sqrt(px * px + py * py) < radius;
+  return radius_less(px * px + py * py, radius);
 }
 
 // Part membership: shapes last to first, first containing shape decides

@@ -438,20 +438,10 @@ hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s)
   const size_t cells = static_cast<size_t>(a.n) * a.n;
   const size_t fbytes = align256(cells * sizeof(cx<R>));
   const size_t per = fbytes + 2 * align256(cells * 16);
-  // scratch per field: [field copy][sup'][rhs']; work pointer of field 0 and stride per field
+  // scratch per field: [field copy][sup'][rhs']
   char *s0 = static_cast<char *>(a.scratch);
-  cx<R> *buf = reinterpret_cast<cx<R> *>(s0);  // field copies laid out with stride `per`
-  cx<double> *work = reinterpret_cast<cx<double> *>(s0 + fbytes);
-  const dim3 grid((a.n + 255) / 256, a.batch);
-  // field copies of different batch entries are `per` bytes apart; pass them
-  // through one contiguous-by-entry view when batch == 1, else per entry.
-  if (a.batch == 1) {
-    if (implicit_x)
-      hipLaunchKernelGGL((k_adi_line<R, true>), grid, dim3(256), 0, s, a, a.field, buf, work, per);
-    else
-      hipLaunchKernelGGL((k_adi_line<R, false>), grid, dim3(256), 0, s, a, buf, a.field, work, per);
-    return hipGetLastError();
-  }
+  // dominant systems: converged tile-Jacobi; otherwise one sequential sweep per line
+  const int iters = adi_jacobi_iters(a.dz, a.kmin, implicit_x ? a.dx : a.dy);
   for (int b = 0; b < a.batch; ++b) {
     TriArgs<R> ab = a;
     ab.batch = 1;
@@ -459,11 +449,18 @@ hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s)
     ab.kvec = a.kvec + b;
     cx<R> *bb = reinterpret_cast<cx<R> *>(s0 + per * b);
     cx<double> *wb = reinterpret_cast<cx<double> *>(s0 + per * b + fbytes);
+    const cx<R> *prev = implicit_x ? ab.field : bb;
+    cx<R> *next = implicit_x ? bb : ab.field;
+    if (iters > 0) {
+      hipError_t e = launch_adi_jacobi<R>(ab, implicit_x, prev, next, iters, s);
+      if (e != hipSuccess) return e;
+      continue;
+    }
     const dim3 g1((a.n + 255) / 256, 1);
     if (implicit_x)
-      hipLaunchKernelGGL((k_adi_line<R, true>), g1, dim3(256), 0, s, ab, ab.field, bb, wb, per);
+      hipLaunchKernelGGL((k_adi_line<R, true>), g1, dim3(256), 0, s, ab, prev, next, wb, per);
     else
-      hipLaunchKernelGGL((k_adi_line<R, false>), g1, dim3(256), 0, s, ab, bb, ab.field, wb, per);
+      hipLaunchKernelGGL((k_adi_line<R, false>), g1, dim3(256), 0, s, ab, prev, next, wb, per);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -25,6 +25,7 @@ struct TriArgs {
   size_t sol_fstride;
   int nz;
   unsigned first_step, nsteps;
+  double kmin;           // smallest wavenumber of the batch (largest Jacobi contraction)
 };
 
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
@@ -50,6 +51,11 @@ hipError_t launch_cn_jobs(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fi
                           cx<R> *sol, unsigned first, unsigned nsteps, hipStream_t s);
 template <class R>
 hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
+// Tile-Jacobi ADI half-step for diagonally dominant systems (pxg_adi.hip).
+int adi_jacobi_iters(double dz, double k, double d);
+template <class R>
+hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
+                             int iters, hipStream_t s);
 hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
                                  const cx<double> *sub, cx<double> *rhs, cx<double> *work,
                                  hipStream_t s);

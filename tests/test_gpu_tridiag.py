@@ -117,17 +117,21 @@ def test_cn_batched_uneven_k(ctx):
 
 
 @pytest.mark.parametrize("quirks", [0, P.FIX_ADI_Y_DIAG])
-def test_adi_vs_oracle(ctx, quirks):
-    n, nsteps = 64, 4
-    announce("adi", n, nsteps, quirks)
+@pytest.mark.parametrize("n,extent,nsteps", [(64, 4000.0, 4), (300, 6000.0, 2), (64, 128.0, 3)])
+def test_adi_vs_oracle(ctx, quirks, n, extent, nsteps):
+    """Dominant grids (dx = 62.5 and 20 nm: tile-Jacobi path, 300 cells spans
+    three tiles) and a fine grid (dx = 2 nm, rho = 0.04: sequential sweep)."""
+    announce("adi", n, extent, nsteps, quirks)
     k, dz = K_01, 20.0
-    dx = 4000.0 / n
-    xmin = -2000.0
+    dx = extent / n
+    xmin = -extent / 2
     g = O.grid(xmin, xmin + (n - 1) * dx, dx, -40.0, 40.0, dz, k)
-    sph = random_spheres(0x5EED0002, 6, (-1500, 1500), (-40, 40), (300, 900), (1e-6, 1e-5))
+    s = extent / 4000.0
+    sph = random_spheres(0x5EED0002, 6, (-1500 * s, 1500 * s), (-40, 40), (300 * s, 900 * s), (1e-6, 1e-5))
     csg = spheres_scene(sph)
     xs = xmin + dx * np.arange(n)
-    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + xs[None, :] ** 2) / 800.0 ** 2).astype(complex))
+    w = 800.0 * s
+    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + xs[None, :] ** 2) / w ** 2).astype(complex))
     for use_tbc in (True, False):
         ref = O.adi_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, use_tbc=use_tbc, quirks=quirks)
         plan = P.Plan(ctx, P.problem(P.ADI, n, nsteps + 1, dx, dz, k, xmin=xmin, zmin=-40.0,
@@ -138,6 +142,26 @@ def test_adi_vs_oracle(ctx, quirks):
         err = rel_l2(plan.get_field(), ref)
         announce("adi rel_l2", use_tbc, err)
         assert err < 1e-10
+
+
+def test_adi_mixed_precision(ctx):
+    n, nsteps, extent = 256, 20, 5000.0
+    k, dz = K_01, 20.0
+    dx = extent / n
+    xmin = -extent / 2
+    csg = spheres_scene(random_spheres(0x5EED0002, 12, (-2000, 2000), (-200, 200), (300, 900), (1e-6, 1e-5)))
+    xs = xmin + dx * np.arange(n)
+    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + xs[None, :] ** 2) / 1000.0 ** 2).astype(complex))
+    out = []
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        plan = P.Plan(ctx, P.problem(P.ADI, n, nsteps + 1, dx, dz, k, xmin=xmin, zmin=-200.0, precision=prec))
+        plan.material_csg(*csg)
+        plan.set_field(u0)
+        plan.step(1, nsteps)
+        out.append(plan.get_field())
+    ef, ei = rel_l2(out[1], out[0]), rel_l2_int(out[1], out[0])
+    announce("adi mixed", ef, ei)
+    assert ef < 1e-5 and ei < 1e-5
 
 
 def test_thomas_batched_matches_reference_fixtures(ctx):
