@@ -89,6 +89,20 @@ def timed(plan, steps):
     return time.perf_counter() - t0
 
 
+def copy_calibration(mib, iters=20):
+    """A plain device-to-device copy of known size; hardware counters of this
+    kernel calibrate FETCH_SIZE/WRITE_SIZE against known bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(iters):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "iters": iters}), flush=True)
+
+
 def cpu_baseline(target_s):
     """The oracle (FP64 C restatement, OpenMP over cells, single-threaded FFT
     as in the reference) on the same 512^2 sphere workload for a bounded number
@@ -122,7 +136,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time only the batched mixed path (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is design-only); run with --gpus 1", file=sys.stderr)
@@ -133,16 +154,17 @@ def main():
     u0 = gaussian_field(N)
 
     # --- plain FP64 single-solve path
-    p64 = make_plan(ctx, P.PREC_FP64, 1, nz)
-    load_fields(p64, u0, 1)
-    if args.warmup:
-        timed(p64, args.warmup)
+    if not args.mixed_only:
+        p64 = make_plan(ctx, P.PREC_FP64, 1, nz)
         load_fields(p64, u0, 1)
-    t64 = timed(p64, args.steps)
-    ref = p64.get_field(0)
-    v64 = N * N * args.steps / t64 / 1e6
-    log(f"fp64 single: {t64 * 1e3:.2f} ms for {args.steps} steps -> {v64:.1f} Mcell*steps/s")
-    p64.close()
+        if args.warmup:
+            timed(p64, args.warmup)
+            load_fields(p64, u0, 1)
+        t64 = timed(p64, args.steps)
+        ref = p64.get_field(0)
+        v64 = N * N * args.steps / t64 / 1e6
+        log(f"fp64 single: {t64 * 1e3:.2f} ms for {args.steps} steps -> {v64:.1f} Mcell*steps/s")
+        p64.close()
 
     # --- batched mixed-precision path (headline)
     pm = make_plan(ctx, P.PREC_MIXED, B, nz)
@@ -153,11 +175,15 @@ def main():
     tm = timed(pm, args.steps)
     mix = pm.get_field(0)
     vm = B * N * N * args.steps / tm / 1e6
+    log(f"mixed batch {B}: {tm * 1e3:.2f} ms -> {vm:.1f} Mcell*steps/s")
+    if args.mixed_only:
+        pm.close()
+        print(json.dumps({"mixed_value": vm, "batch": B, "steps": args.steps}), flush=True)
+        return
     ef = float(np.linalg.norm(mix - ref) / np.linalg.norm(ref))
     ir, im = np.abs(ref) ** 2, np.abs(mix) ** 2
     ei = float(np.linalg.norm(im - ir) / np.linalg.norm(ir))
-    log(f"mixed batch {B}: {tm * 1e3:.2f} ms -> {vm:.1f} Mcell*steps/s; rel L2 field {ef:.3e} "
-        f"intensity {ei:.3e}")
+    log(f"  rel L2 vs fp64: field {ef:.3e} intensity {ei:.3e}")
 
     # --- per-kernel durations with HIP events on the plan's stream
     load_fields(pm, u0, B)
@@ -189,7 +215,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (fp32 complex) with fp64 phase arguments",
+        "dtype": "c64 storage, fp64 arithmetic",
         "data": "synthetic: Gaussian beam + device-evaluated CSG sphere",
         "config": {"workload": "FFTSolver3D 512x512 over 20x20 um, 1000 z-steps of 10 nm, "
                                "lambda 0.1 nm, Gaussian waist 3 um, sphere r 4 um d 5e-6 b 1e-7",
