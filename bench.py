@@ -177,6 +177,11 @@ def main():
     vm = B * N * N * args.steps / tm / 1e6
     log(f"mixed batch {B}: {tm * 1e3:.2f} ms -> {vm:.1f} Mcell*steps/s")
     if args.mixed_only:
+        load_fields(pm, u0, B)
+        _, stats = pm.step_profiled(1, args.profile_steps)
+        for name, s in stats.items():
+            log(f"  {name}: {s['launches']} launches, mean {s['mean_ms'] * 1e3:.1f} us, "
+                f"{s['bytes_per_launch'] / s['mean_ms'] / 1e6:.1f} GB/s alg")
         pm.close()
         print(json.dumps({"mixed_value": vm, "batch": B, "steps": args.steps}), flush=True)
         return
