@@ -135,7 +135,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="z-steps of the HIP-event profile (default: all --steps)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the batched mixed path (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -149,6 +150,8 @@ def main():
               "(sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     nz = max(args.steps, args.warmup) + 1
+    if args.profile_steps <= 0:
+        args.profile_steps = args.steps
     B = args.batch
     ctx = P.Ctx(0)
     u0 = gaussian_field(N)

@@ -133,7 +133,23 @@ __device__ __forceinline__ void load_canon(cx<Rt> (&v)[8], const cx<Rt> *lds, in
   for (int m = 0; m < 8; ++m) v[m] = lds[lds_pos(t + m * (N / 8))];
 }
 
-template <int N, int S, int NS, class Rt>
+// LDS ordering between the stages.  When every lane of a line sits in one
+// wavefront and the line's LDS slice belongs to that wavefront alone (N <= 512
+// with lanes of a line adjacent), completing the wavefront's own LDS accesses
+// (waitcnt, emitted by the work-group-scope fences) orders them: no s_barrier.
+template <bool WAVE_LOCAL>
+__device__ __forceinline__ void lds_sync()
+{
+  if constexpr (WAVE_LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int N, int S, int NS, class Rt, bool WL>
 struct StockhamStages {
   static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t,
                                              const cx<Rt> *__restrict__ tw)
@@ -142,24 +158,25 @@ struct StockhamStages {
     stage_compute<N, R, NS, S>(v, t, tw);
     if constexpr (NS * R < N) {
       stage_store<N, R, NS>(v, lds, t);
-      __syncthreads();
+      lds_sync<WL>();
       load_canon<N>(v, lds, t);
-      __syncthreads();
-      StockhamStages<N, S, NS * R, Rt>::run(v, lds, t, tw);
+      lds_sync<WL>();
+      StockhamStages<N, S, NS * R, Rt, WL>::run(v, lds, t, tw);
     }
   }
 };
 
 // Full length-N transform of the canonical registers (S = -1 forward, +1
 // backward, unnormalised).  `lds` is this line's LdsLine<N>::SIZE slice; every
-// lane of the work-group must call it (it contains work-group barriers).
-// A leading barrier protects LDS still being read by a preceding transform.
-template <int N, int S, class Rt>
+// lane of the work-group must call it (it contains work-group barriers unless
+// WL, see lds_sync).  A leading sync protects LDS still being read by a
+// preceding transform.
+template <int N, int S, class Rt, bool WL = false>
 __device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t,
                                          const cx<Rt> *__restrict__ tw)
 {
-  __syncthreads();
-  StockhamStages<N, S, 1, Rt>::run(v, lds, t, tw);
+  lds_sync<WL>();
+  StockhamStages<N, S, 1, Rt, WL>::run(v, lds, t, tw);
 }
 
 }  // namespace pxg

@@ -108,8 +108,9 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   cd v[8];
   load8(v, f, T);
 
+  constexpr bool WL = (N / 8) <= 64;
   if (a.mode & Y_INV) {
-    fft_line<N, +1>(v, ll, t, a.tw);
+    fft_line<N, +1, double, WL>(v, ll, t, a.tw);
     const double k = a.kvec[b];
     const double x = coord(a.xmin, c, a.dx);
     if (a.mat.mode == MAT_NONE) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
       for (int m = 0; m < 8; ++m) v[m] = scale(scale(v[m], a.absy[t + m * T]), ax);
     }
   }
-  if (a.mode & Y_FWD) fft_line<N, -1>(v, ll, t, a.tw);
+  if (a.mode & Y_FWD) fft_line<N, -1, double, WL>(v, ll, t, a.tw);
   store8(v, f, T);
 }
 
@@ -214,11 +215,12 @@ __global__ __launch_bounds__(L *N / 8) void k_fft2d_step(FftArgs<R> a)
   cd *ll = lds + line * LdsLine<N>::STRIDE;
   cd v[8];
   load8(v, f, T);
-  fft_line<N, -1>(v, ll, t, a.tw);
+  constexpr bool WL = (N / 8) <= 64;
+  fft_line<N, -1, double, WL>(v, ll, t, a.tw);
   const cd *kx = a.kx + b * N;
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = v[m] * kx[t + m * T];
-  fft_line<N, +1>(v, ll, t, a.tw);
+  fft_line<N, +1, double, WL>(v, ll, t, a.tw);
   const double k = a.kvec[b];
   const double z = coord(a.zmin, a.step, a.dz) + 0.5 * a.dz;
   const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;

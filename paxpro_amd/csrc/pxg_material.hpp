@@ -103,7 +103,12 @@ __device__ __forceinline__ int csg_eval(const MatDev &m, const uint64_t *mask, i
 {
   if (m.is_ref) return m.n_parts;
   for (int w = 0; w < nw; ++w) {
-    uint64_t bits = mask[w];
+    // The mask is the same for every lane of the work-group: moving it to
+    // scalar registers makes the part/shape descriptors scalar loads.
+    const uint64_t raw = mask[w];
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(raw));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(raw >> 32));
+    uint64_t bits = (static_cast<uint64_t>(hi) << 32) | lo;
     while (bits) {
       int e = w * 64 + __builtin_ctzll(bits);
       bits &= bits - 1;
