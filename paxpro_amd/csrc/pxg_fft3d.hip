@@ -56,27 +56,6 @@ __device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_
   beta /= (2.0 * n_refr);
 }
 
-// FFTSolver3D::refraction for one cell (src/fftSolver3D.cpp:122-156): material at
-// z1 and z0; a changed material (|d delta| or |d beta| > 1e-10) takes the
-// crossing integral, otherwise the part's precomputed factor is used.
-__device__ __forceinline__ cd csg_screen(const MatDev &m, const uint64_t *mask, int nw, const cd *pf,
-                                      double k, double dz, double inv_norm, unsigned n_refr,
-                                      unsigned quirks, double x, double y, double z0, double z1)
-{
-  const int e1 = csg_eval(m, mask, nw, x, y, z1);
-  if (nw == 0) return pf[e1];
-  const int e0 = csg_eval(m, mask, nw, x, y, z0);
-  if (e1 == e0) return pf[e1];
-  double d1, b1, d0, b0;
-  part_values(m, e1, d1, b1);
-  part_values(m, e0, d0, b0);
-  if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
-    crossing_integral(m, mask, nw, n_refr, quirks, x, y, z0, z1, d1, b1);
-    return screen_from_values(k, dz, d1, b1, inv_norm);
-  }
-  return pf[e1];
-}
-
 template <class R>
 __device__ __forceinline__ void load8(cd (&v)[8], const cx<R> *f, size_t stride)
 {
@@ -127,16 +106,39 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
       for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
       if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
       const cd *pf = static_cast<const cd *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
-      // Factors go to this lane's own canonical LDS slots (read only by this
-      // lane in the last stage of the transform), then multiply the registers.
-#pragma unroll 1
-      for (int m = 0; m < 8; ++m) {
-        const double y = coord(a.ymin, t + m * T, a.dy);
-        ll[lds_pos(t + m * T)] =
-            csg_screen(a.mat, mask, nw, pf, k, a.dz, a.inv_norm, a.n_refr, a.quirks, x, y, z0, z1);
-      }
+      double yv[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = v[m] * ll[lds_pos(t + m * T)];
+      for (int m = 0; m < 8; ++m) yv[m] = coord(a.ymin, t + m * T, a.dy);
+      int e1[8], e0[8];
+      csg_eval8x2(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
+      // Cells whose material changed between z0 and z1 take the crossing
+      // integral below; the others use the part's precomputed factor.
+      unsigned crossing = 0;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        if (e1[m] != e0[m]) {
+          double d1, b1, d0, b0;
+          part_values(a.mat, e1[m], d1, b1);
+          part_values(a.mat, e0[m], d0, b0);
+          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) crossing |= 1u << m;
+        }
+        if (!((crossing >> m) & 1u)) v[m] = v[m] * pf[e1[m]];
+      }
+      if (crossing) {
+        // rare border cells: recompute their factor (the lane's own canonical
+        // LDS slots carry it back into registers without dynamic indexing)
+#pragma unroll 1
+        for (int m = 0; m < 8; ++m) {
+          if (!((crossing >> m) & 1u)) continue;
+          double d1, b1;
+          crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, coord(a.ymin, t + m * T, a.dy), z0,
+                            z1, d1, b1);
+          ll[lds_pos(t + m * T)] = screen_from_values(k, a.dz, d1, b1, a.inv_norm);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          if ((crossing >> m) & 1u) v[m] = v[m] * ll[lds_pos(t + m * T)];
+      }
     } else {  // MAT_PLANES: effective (delta, beta) of this step per cell
       const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
       const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride +
