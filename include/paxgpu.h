@@ -49,8 +49,11 @@ typedef struct pxg_plan_s *pxg_plan;
 /* ---- solver kinds and options --------------------------------------------- */
 enum { PXG_FFT2D = 0, PXG_FFT3D = 1, PXG_CN = 2, PXG_ADI = 3 };
 /* FP64: complex128 fields and FP64 arithmetic throughout (the plain path).
- * MIXED: complex64 fields and FFT/stencil arithmetic; FP64 phase arguments
- * (reduced mod 2*pi before rounding) and FP64 tridiagonal sweeps. */
+ * MIXED: complex64 fields in HBM (half the bytes per pass); every kernel
+ * widens to FP64 on load and computes in FP64 with FP64 tables, rounding to
+ * complex64 once per store.  FP32 arithmetic was measured to accumulate a
+ * coherent error (the rounded twiddles and propagator act as a fixed operator
+ * perturbation, growing linearly with steps), FP64 arithmetic does not. */
 enum { PXG_PREC_FP64 = 0, PXG_PREC_MIXED = 1 };
 enum { PXG_BC_DIRICHLET = 0, PXG_BC_TRANSPARENT = 1 };   /* Solver2D::BC_t */
 /* Quirk switches: 0 reproduces the reference; a set bit selects the physics form. */
@@ -128,6 +131,20 @@ int pxg_plan_destroy(pxg_plan plan);
 int pxg_plan_reset(pxg_plan plan, const pxg_problem *prob);
 /* Bytes of device memory the plan reserves (for admission by HBM budget). */
 int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes);
+/* Work-group shape of the two FFT3D passes: lines per work-group of the x pass
+ * and of the y pass (powers of two 1..16 within the LDS of one CU; 0 keeps the
+ * current value).  Results do not depend on it.  FFT3D plans only.  No
+ * reference counterpart: FFTW chooses its own plan shapes
+ * (src/fftSolver3D.cpp:61-62 plans with FFTW_ESTIMATE). */
+int pxg_plan_set_lines(pxg_plan plan, int lines_x, int lines_y);
+int pxg_plan_get_lines(pxg_plan plan, int *lines_x, int *lines_y);
+/* Autotuner, in the manner of FFTW's measured planning: times every supported
+ * shape of both FFT3D passes on this plan with its current material and keeps
+ * the fastest; the choice is cached in the context for plans of the same nx,
+ * batch, precision and material kind.  Overwrites (zeroes) the field: call it
+ * after the material and before pxg_set_field.  reps <= 0 means 5 timed
+ * launches per shape.  A no-op for the other kinds. */
+int pxg_plan_autotune(pxg_plan plan, int reps);
 /* Job plan: n independent problems of one kind in one plan, each with its own
  * grid, wavenumber and material (uneven Nx, photon energies, waveguides), all
  * advanced by one launch per pxg_step.  Field/solution index b = job b.

@@ -3,11 +3,13 @@
 // pxg_adi.hip.  No exception crosses the boundary: every entry point catches,
 // records a thread-local message and returns a PXG_ERR_* code.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -103,6 +105,8 @@ struct pxg_ctx_s {
   size_t budget = 0;
   std::atomic<size_t> reserved{0};
   std::mutex mu;
+  // autotuned FFT3D work-group shapes: (nx, batch, precision, material) -> (lines_x, lines_y)
+  std::map<std::array<int, 4>, std::pair<int, int>> tuned;
 };
 
 struct pxg_plan_s {
@@ -430,6 +434,60 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
       }
     }
   }
+}
+
+// Lines per work-group the FFT3D passes are built for at size n: powers of two
+// up to 16, at most 1024 lanes, FP64 line storage within the LDS of one CU.
+bool lines_supported(int n, int l)
+{
+  return l >= 1 && l <= 16 && (l & (l - 1)) == 0 && l * (n / 8) <= 1024 && n % l == 0 &&
+         static_cast<size_t>(l) * (n + n / 8 + 1) * sizeof(cd) <= 163840 - 1024;
+}
+
+// Mean duration of `reps` launches of one FFT3D pass with `lines` lines per
+// work-group, on the plan's own field and material (mid-run z-step).
+template <class R>
+double time_pass(pxg_plan P, bool pass_x, int lines, int reps)
+{
+  const pxg_problem &p = P->p;
+  FftArgs<R> a = fft_args<R>(P);
+  a.step = std::max(1, p.nz / 2);
+  a.mode = Y_INV | Y_FWD;
+  auto launch = [&] {
+    return pass_x ? launch_pass_x<R>(p.nx, lines, a, p.batch, P->stream)
+                  : launch_pass_y<R>(p.nx, lines, a, p.batch, P->stream);
+  };
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float ms = 0.0f;
+  hipError_t err = launch();
+  if (err == hipSuccess) err = hipEventRecord(e0, P->stream);
+  for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+  if (err == hipSuccess) err = hipEventRecord(e1, P->stream);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  hip_check(err, "autotune launch");
+  return static_cast<double>(ms) / reps;
+}
+
+template <class R>
+void autotune_fft3d(pxg_plan P, int reps)
+{
+  const int n = P->p.nx;
+  double best_x = 1e30, best_y = 1e30;
+  int lx = P->lines_x, ly = P->lines_y;
+  for (int l = 1; l <= 16; l *= 2) {
+    if (!lines_supported(n, l)) continue;
+    const double tx = time_pass<R>(P, true, l, reps);
+    if (tx < best_x) best_x = tx, lx = l;
+    const double ty = time_pass<R>(P, false, l, reps);
+    if (ty < best_y) best_y = ty, ly = l;
+  }
+  P->lines_x = lx;
+  P->lines_y = ly;
 }
 
 template <class R>
@@ -1116,6 +1174,58 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
     P->z_invariant = z_invariant;
     P->mat_mode = MAT_PLANES;
     P->jobs_dirty = true;
+  });
+}
+
+int pxg_plan_set_lines(pxg_plan P, int lines_x, int lines_y)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (P->p.kind != PXG_FFT3D) fail(PXG_ERR_UNSUPPORTED, "work-group shapes apply to FFT3D plans only");
+    const int n = P->p.nx;
+    if ((lines_x && !lines_supported(n, lines_x)) || (lines_y && !lines_supported(n, lines_y)))
+      fail(PXG_ERR_INVALID, "lines per work-group (%d, %d) not supported at nx=%d", lines_x, lines_y, n);
+    if (lines_x) P->lines_x = lines_x;
+    if (lines_y) P->lines_y = lines_y;
+  });
+}
+
+int pxg_plan_get_lines(pxg_plan P, int *lines_x, int *lines_y)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!lines_x || !lines_y) fail(PXG_ERR_INVALID, "null output");
+    *lines_x = P->lines_x;
+    *lines_y = P->lines_y;
+  });
+}
+
+int pxg_plan_autotune(pxg_plan P, int reps)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (P->p.kind != PXG_FFT3D) return;  // the other kinds have one shape
+    const std::array<int, 4> key = {P->p.nx, P->p.batch, P->p.precision, P->mat_mode};
+    {
+      std::lock_guard<std::mutex> lk(P->ctx->mu);
+      auto it = P->ctx->tuned.find(key);
+      if (it != P->ctx->tuned.end()) {
+        P->lines_x = it->second.first;
+        P->lines_y = it->second.second;
+        return;
+      }
+    }
+    if (reps <= 0) reps = 5;
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    if (P->mixed)
+      autotune_fft3d<float>(P, reps);
+    else
+      autotune_fft3d<double>(P, reps);
+    hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
+    hip_check(hipStreamSynchronize(P->stream), "sync");
+    P->field_set = false;
+    std::lock_guard<std::mutex> lk(P->ctx->mu);
+    P->ctx->tuned[key] = {P->lines_x, P->lines_y};
   });
 }
 

@@ -77,6 +77,8 @@ template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 {
   constexpr int T = N / 8;
+  constexpr bool WL = (N / 8) <= 64;
+  constexpr int CSG_K = 2;
   __shared__ cd lds[L * LdsLine<N>::STRIDE];
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
@@ -84,70 +86,103 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   const int b = blockIdx.y;
   cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
   cd *ll = lds + line * LdsLine<N>::STRIDE;
+  const double x = coord(a.xmin, c, a.dx);
+  const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
+
+  // Material first, before the field occupies registers: the deciding part of
+  // each of the lane's 8 cells at z1 (16-bit indices, packed) and a bit per
+  // cell whose material changed between z0 and z1 (src/fftSolver3D.cpp:122-156).
+  unsigned eidx[4] = {0u, 0u, 0u, 0u};
+  unsigned crossing = 0;
+  int nw = 0;
+  double z0 = 0.0, z1 = 0.0;
+  if (csg) {
+    z1 = coord(a.zmin, a.step, a.dz);
+    z0 = coord(a.zmin, a.step - 1, a.dz);
+    const int c0 = blockIdx.x * L;
+    nw = (a.mat.n_parts + 63) >> 6;
+    csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx), a.ymin,
+             coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
+    bool any = false;
+    for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
+    if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
+    // CSG_K cells at a time (few registers), each cell's code parked in the
+    // lane's own canonical LDS slot until all 8 are known: the deciding part
+    // index, bit 31 set when the material differs between the two planes
+#pragma unroll 1
+    for (int m0 = 0; m0 < 8; m0 += CSG_K) {
+      double yv[CSG_K];
+#pragma unroll
+      for (int q = 0; q < CSG_K; ++q) yv[q] = coord(a.ymin, t + (m0 + q) * T, a.dy);
+      int e1[CSG_K], e0[CSG_K];
+      csg_eval_x2<CSG_K>(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
+#pragma unroll
+      for (int q = 0; q < CSG_K; ++q) {
+        unsigned code = static_cast<unsigned>(e1[q]);
+        if (e1[q] != e0[q]) {
+          double d1, b1, d0, b0;
+          part_values(a.mat, e1[q], d1, b1);
+          part_values(a.mat, e0[q], d0, b0);
+          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) code |= 1u << 31;
+        }
+        ll[lds_pos(t + (m0 + q) * T)] = cd{static_cast<double>(code), 0.0};
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const unsigned code = static_cast<unsigned>(ll[lds_pos(t + m * T)].x);
+      crossing |= (code >> 31) << m;
+      eidx[m >> 1] |= (code & 0xffffu) << (16 * (m & 1));
+    }
+  }
+
   cd v[8];
   load8(v, f, T);
-
-  constexpr bool WL = (N / 8) <= 64;
   if (a.mode & Y_INV) {
     fft_line<N, +1, double, WL>(v, ll, t, a.tw);
     const double k = a.kvec[b];
-    const double x = coord(a.xmin, c, a.dx);
     if (a.mat.mode == MAT_NONE) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = scale(v[m], a.inv_norm);
-    } else if (a.mat.mode == MAT_CSG) {
-      const double z1 = coord(a.zmin, a.step, a.dz);
-      const double z0 = coord(a.zmin, a.step - 1, a.dz);
-      const int c0 = blockIdx.x * L;
-      int nw = (a.mat.n_parts + 63) >> 6;
-      csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx), a.ymin,
-               coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
-      bool any = false;
-      for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
-      if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
+    } else if (csg) {
       const cd *pf = static_cast<const cd *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
-      double yv[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) yv[m] = coord(a.ymin, t + m * T, a.dy);
-      int e1[8], e0[8];
-      csg_eval8x2(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
-      // Cells whose material changed between z0 and z1 take the crossing
-      // integral below; the others use the part's precomputed factor.
-      unsigned crossing = 0;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        if (e1[m] != e0[m]) {
-          double d1, b1, d0, b0;
-          part_values(a.mat, e1[m], d1, b1);
-          part_values(a.mat, e0[m], d0, b0);
-          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) crossing |= 1u << m;
-        }
-        if (!((crossing >> m) & 1u)) v[m] = v[m] * pf[e1[m]];
-      }
+      for (int m = 0; m < 8; ++m)
+        if (!((crossing >> m) & 1u)) v[m] = v[m] * pf[(eidx[m >> 1] >> (16 * (m & 1))) & 0xffffu];
       if (crossing) {
-        // rare border cells: recompute their factor (the lane's own canonical
-        // LDS slots carry it back into registers without dynamic indexing)
+        // rare border cells: the field values go to the lane's own canonical
+        // LDS slots (no other lane touches them between the two FFTs) so that
+        // they are not live in registers while the crossing integral runs
+#pragma unroll
+        for (int m = 0; m < 8; ++m) ll[lds_pos(t + m * T)] = v[m];
 #pragma unroll 1
         for (int m = 0; m < 8; ++m) {
           if (!((crossing >> m) & 1u)) continue;
           double d1, b1;
           crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, coord(a.ymin, t + m * T, a.dy), z0,
                             z1, d1, b1);
-          ll[lds_pos(t + m * T)] = screen_from_values(k, a.dz, d1, b1, a.inv_norm);
+          cd &slot = ll[lds_pos(t + m * T)];
+          slot = slot * screen_from_values(k, a.dz, d1, b1, a.inv_norm);
         }
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
-          if ((crossing >> m) & 1u) v[m] = v[m] * ll[lds_pos(t + m * T)];
+        for (int m = 0; m < 8; ++m) v[m] = ll[lds_pos(t + m * T)];
       }
     } else {  // MAT_PLANES: effective (delta, beta) of this step per cell
       const int p = a.mat.z_invariant ? 0 : static_cast<int>(a.step) - a.mat.first_plane;
       const double *pl = a.mat.planes + static_cast<size_t>(p) * a.mat.plane_stride +
                          2 * static_cast<size_t>(c) * N;
+      // one cell at a time through the lane's own LDS slots, as above: eight
+      // interleaved FP64 exp/sincos chains would double the register count
 #pragma unroll
+      for (int m = 0; m < 8; ++m) ll[lds_pos(t + m * T)] = v[m];
+#pragma unroll 1
       for (int m = 0; m < 8; ++m) {
         const int r = t + m * T;
-        v[m] = v[m] * screen_from_values(k, a.dz, pl[2 * r], pl[2 * r + 1], a.inv_norm);
+        cd &slot = ll[lds_pos(r)];
+        slot = slot * screen_from_values(k, a.dz, pl[2 * r], pl[2 * r + 1], a.inv_norm);
       }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = ll[lds_pos(t + m * T)];
     }
     if (a.absy) {
       const double ax = a.absx[c];

@@ -118,16 +118,16 @@ __device__ __forceinline__ int csg_eval(const MatDev &m, const uint64_t *mask, i
   return m.n_parts;
 }
 
-// csg_eval for the 8 points (x, y[k], za) and (x, y[k], zb) a lane holds: each
-// culled part and each of its shapes is loaded once (scalar loads, the loop is
-// uniform) and tested against all 16 points.  ea/eb receive the same indices
-// csg_eval would return.
-__device__ __forceinline__ void csg_eval8x2(const MatDev &m, const uint64_t *mask, int nw, double x,
-                                            const double (&y)[8], double za, double zb, int (&ea)[8],
-                                            int (&eb)[8])
+// csg_eval for K points (x, y[k], za) and (x, y[k], zb): each culled part and
+// each of its shapes is loaded once (scalar loads, the loop is uniform) and
+// tested against all 2K points.  ea/eb receive the indices csg_eval returns.
+template <int K>
+__device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mask, int nw, double x,
+                                            const double (&y)[K], double za, double zb, int (&ea)[K],
+                                            int (&eb)[K])
 {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) ea[k] = eb[k] = m.n_parts;
+  for (int k = 0; k < K; ++k) ea[k] = eb[k] = m.n_parts;
   if (m.is_ref) return;
   for (int w = 0; w < nw; ++w) {
     const uint64_t raw = mask[w];
@@ -138,20 +138,20 @@ __device__ __forceinline__ void csg_eval8x2(const MatDev &m, const uint64_t *mas
       const int e = w * 64 + __builtin_ctzll(bits);
       bits &= bits - 1;
       const DPart p = m.parts[e];
-      int sa[8], sb[8];  // 0 undecided, 1 inside (union shape), 2 outside (difference)
+      int sa[K], sb[K];  // 0 undecided, 1 inside (union shape), 2 outside (difference)
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sa[k] = sb[k] = 0;
+      for (int k = 0; k < K; ++k) sa[k] = sb[k] = 0;
       for (int i = p.n_shapes - 1; i >= 0; --i) {
         const DShape s = m.shapes[p.first_shape + i];
         const int verdict = s.op == 0 ? 1 : 2;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < K; ++k) {
           if (sa[k] == 0 && shape_inside(s, x, y[k], za)) sa[k] = verdict;
           if (sb[k] == 0 && shape_inside(s, x, y[k], zb)) sb[k] = verdict;
         }
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < K; ++k) {
         if (ea[k] == m.n_parts && sa[k] == 1) ea[k] = e;
         if (eb[k] == m.n_parts && sb[k] == 1) eb[k] = e;
       }

@@ -34,6 +34,7 @@ EXPORTS = [
     "pxg_set_material_none", "pxg_step", "pxg_sync", "pxg_step_profiled",
     "pxg_field_device_ptr", "pxg_thomas_batched", "pxg_plan_create_jobs",
     "pxg_set_job_material_curved_wg", "pxg_job_solution_dims", "pxg_plan_reset",
+    "pxg_plan_set_lines", "pxg_plan_get_lines", "pxg_plan_autotune",
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
 ]
@@ -119,6 +120,9 @@ def lib():
         L.pxg_thomas_batched.argtypes = [P, I, I, I, P, P, P]
         L.pxg_plan_create_jobs.argtypes = [P, P, I, P]
         L.pxg_plan_reset.argtypes = [P, P]
+        L.pxg_plan_set_lines.argtypes = [P, I, I]
+        L.pxg_plan_get_lines.argtypes = [P, P, P]
+        L.pxg_plan_autotune.argtypes = [P, I]
         L.pxg_server_create.argtypes = [P, I, S, U, P]
         L.pxg_server_submit.argtypes = [P, P, P]
         L.pxg_server_status.argtypes = [P, C.c_uint64, P]
@@ -212,6 +216,21 @@ class Plan:
     def reset(self, prob):
         _check(lib().pxg_plan_reset(self.h, C.byref(prob)))
         self.prob = prob
+
+    def set_lines(self, lines_x=0, lines_y=0):
+        """Lines per work-group of the FFT3D x and y passes (0 keeps one)."""
+        _check(lib().pxg_plan_set_lines(self.h, lines_x, lines_y))
+
+    def lines(self):
+        lx, ly = C.c_int(), C.c_int()
+        _check(lib().pxg_plan_get_lines(self.h, C.byref(lx), C.byref(ly)))
+        return lx.value, ly.value
+
+    def autotune(self, reps=0):
+        """Time every supported work-group shape and keep the fastest (FFT3D);
+        zeroes the field, so call it before set_field."""
+        _check(lib().pxg_plan_autotune(self.h, reps))
+        return self.lines()
 
     def set_field(self, field, b=0):
         order = "F" if self.is3d else "C"

@@ -174,3 +174,48 @@ def test_fft2d_vs_oracle(ctx, material):
     plan.step(1, nsteps)
     assert rel_l2(plan.get_field(), ref) < 1e-11
     assert rel_l2(plan.get_solution(), sol_ref) < 1e-11
+
+
+@pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_MIXED])
+def test_fft3d_work_group_shapes_bitwise_and_autotune(ctx, prec):
+    """Every supported lines-per-work-group shape of both passes gives the same
+    bits (a line's arithmetic does not depend on its work-group; CSG culling is
+    conservative), and the autotuner picks a supported shape, caches it in the
+    context and zeroes the field it timed on."""
+    n, nsteps = 128, 6
+    announce("fft3d work-group shapes", n, nsteps, "precision", prec)
+    g = fft3d_case(n, dz=200.0, nsteps=nsteps)
+    sph = random_spheres(0x5EED0003, 12, (-8000, 8000), (-1500, 1500), (500, 3000), (1e-6, 1e-5))
+    csg = spheres_scene(sph)
+    u0 = gaussian_field(g, n, 5000.0)
+    prob = fft3d_problem(g, n, nsteps + 1, precision=prec, batch=2)
+    outs = {}
+    for lines in (1, 2, 4, 8, 16):
+        plan = P.Plan(ctx, prob)
+        plan.material_csg(*csg)
+        plan.set_lines(lines, lines)
+        assert plan.lines() == (lines, lines)
+        plan.set_field(u0, 0)
+        plan.set_field(u0, 1)
+        plan.step(1, nsteps)
+        outs[lines] = plan.get_field(1)
+        plan.close()
+    for lines, out in outs.items():
+        assert np.array_equal(out, outs[1]), lines
+    plan = P.Plan(ctx, prob)
+    plan.material_csg(*csg)
+    with pytest.raises(P.PxgError) as e:
+        plan.set_lines(32, 0)
+    assert e.value.code == P.ERR_INVALID
+    lx, ly = plan.autotune(2)
+    assert lx in outs and ly in outs
+    with pytest.raises(P.PxgError) as e:  # the field was zeroed and must be set again
+        plan.step(1, 1)
+    assert e.value.code == P.ERR_STATE
+    plan.set_field(u0, 0)
+    plan.set_field(u0, 1)
+    plan.step(1, nsteps)
+    assert np.array_equal(plan.get_field(1), outs[1])
+    plan2 = P.Plan(ctx, prob)
+    plan2.material_csg(*csg)
+    assert plan2.autotune(2) == (lx, ly)  # cached for the same shape of problem
