@@ -110,6 +110,8 @@ typedef struct {
 /* Replaces nothing in the reference (it has no device): owns the device choice,
  * the HBM budget used for admission, and the last error string. */
 int pxg_ctx_create(int device, size_t hbm_budget_bytes, pxg_ctx *out);
+/* Plans keep their context alive: a context destroyed while plans of it live
+ * is freed when the last of them is destroyed (no new plans can be made). */
 int pxg_ctx_destroy(pxg_ctx ctx);
 /* Bytes currently reserved by live plans of this context. */
 size_t pxg_ctx_reserved(pxg_ctx ctx);

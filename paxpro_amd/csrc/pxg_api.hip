@@ -105,6 +105,9 @@ struct pxg_ctx_s {
   size_t budget = 0;
   std::atomic<size_t> reserved{0};
   std::mutex mu;
+  // live plans; a destroyed context is freed when its last plan is destroyed
+  int plans = 0;
+  bool closed = false;
   // autotuned FFT3D work-group shapes: (nx, batch, precision, material) -> (lines_x, lines_y)
   std::map<std::array<int, 4>, std::pair<int, int>> tuned;
 };
@@ -723,7 +726,14 @@ int pxg_ctx_destroy(pxg_ctx ctx)
 {
   return guarded([&] {
     if (!ctx) fail(PXG_ERR_INVALID, "null context");
-    delete ctx;
+    bool last;
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->closed) fail(PXG_ERR_STATE, "context already destroyed");
+      ctx->closed = true;
+      last = ctx->plans == 0;
+    }
+    if (last) delete ctx;
   });
 }
 
@@ -748,10 +758,12 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     const size_t need = plan_bytes(*prob);
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->closed) fail(PXG_ERR_STATE, "context destroyed");
       if (ctx->reserved.load() + need > ctx->budget)
         fail(PXG_ERR_BUDGET, "plan needs %zu bytes, budget %zu has %zu reserved", need, ctx->budget,
              ctx->reserved.load());
       ctx->reserved += need;
+      ++ctx->plans;
     }
     P = new pxg_plan_s;
     P->ctx = ctx;
@@ -834,10 +846,12 @@ int pxg_plan_create_jobs(pxg_ctx ctx, const pxg_problem *probs, int n, pxg_plan 
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->closed) fail(PXG_ERR_STATE, "context destroyed");
       if (ctx->reserved.load() + need > ctx->budget)
         fail(PXG_ERR_BUDGET, "job plan needs %zu bytes, budget %zu has %zu reserved", need, ctx->budget,
              ctx->reserved.load());
       ctx->reserved += need;
+      ++ctx->plans;
     }
     P = new pxg_plan_s;
     P->ctx = ctx;
@@ -942,8 +956,15 @@ int pxg_plan_destroy(pxg_plan P)
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     for (auto e : P->ev) (void)hipEventDestroy(e);
     if (P->stream) (void)hipStreamDestroy(P->stream);
-    if (P->ctx) P->ctx->reserved -= P->reserved;
+    pxg_ctx c = P->ctx;
+    bool last = false;
+    if (c) {
+      std::lock_guard<std::mutex> lk(c->mu);
+      c->reserved -= P->reserved;
+      last = --c->plans == 0 && c->closed;
+    }
     delete P;
+    if (last) delete c;
   });
 }
 

@@ -21,8 +21,11 @@ namespace pxg {
 constexpr int MAX_MASK_WORDS = 16;  // CSG parts per material: up to 1024
 
 // Phase-screen factor exp(-k (beta + i delta) dz) / N^2 from FP64 material
-// values, in FP64 (src/fftSolver3D.cpp:154).
-__device__ __forceinline__ cd screen_from_values(double k, double dz, double delta, double beta,
+// values, in FP64 (src/fftSolver3D.cpp:154).  Out of line on purpose: its
+// callers are rare per-cell loops, and inlined into them the exp/sincos
+// polynomial constants are hoisted out of the loop into some 50 VGPRs that
+// then count against the whole kernel (occupancy 3 instead of 4 waves/SIMD).
+__device__ __noinline__ cd screen_from_values(double k, double dz, double delta, double beta,
                                                  double inv_norm)
 {
   const double amp = exp((-k * beta) * dz);
@@ -89,11 +92,11 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   const double x = coord(a.xmin, c, a.dx);
   const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
 
-  // Material first, before the field occupies registers: the deciding part of
-  // each of the lane's 8 cells at z1 (16-bit indices, packed) and a bit per
-  // cell whose material changed between z0 and z1 (src/fftSolver3D.cpp:122-156).
-  unsigned eidx[4] = {0u, 0u, 0u, 0u};
-  unsigned crossing = 0;
+  // Material first, before the field occupies registers: a 16-bit code per
+  // cell of the lane (the deciding part at z1; bit 15 set when the material
+  // differs between z0 and z1, src/fftSolver3D.cpp:122-156), kept in the
+  // lane's pad slot of the line's LDS until the inverse transform is done.
+  uint16_t *codes = reinterpret_cast<uint16_t *>(ll + pad_slot(t));
   int nw = 0;
   double z0 = 0.0, z1 = 0.0;
   if (csg) {
@@ -106,9 +109,6 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
     bool any = false;
     for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
     if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
-    // CSG_K cells at a time (few registers), each cell's code parked in the
-    // lane's own canonical LDS slot until all 8 are known: the deciding part
-    // index, bit 31 set when the material differs between the two planes
 #pragma unroll 1
     for (int m0 = 0; m0 < 8; m0 += CSG_K) {
       double yv[CSG_K];
@@ -123,16 +123,10 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
           double d1, b1, d0, b0;
           part_values(a.mat, e1[q], d1, b1);
           part_values(a.mat, e0[q], d0, b0);
-          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) code |= 1u << 31;
+          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) code |= 0x8000u;
         }
-        ll[lds_pos(t + (m0 + q) * T)] = cd{static_cast<double>(code), 0.0};
+        codes[m0 + q] = static_cast<uint16_t>(code);
       }
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const unsigned code = static_cast<unsigned>(ll[lds_pos(t + m * T)].x);
-      crossing |= (code >> 31) << m;
-      eidx[m >> 1] |= (code & 0xffffu) << (16 * (m & 1));
     }
   }
 
@@ -146,23 +140,37 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
       for (int m = 0; m < 8; ++m) v[m] = scale(v[m], a.inv_norm);
     } else if (csg) {
       const cd *pf = static_cast<const cd *>(a.mat.part_fac) + b * (a.mat.n_parts + 1);
+      unsigned crossing = 0;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const unsigned code = codes[m];
+        if (code & 0x8000u)
+          crossing |= 1u << m;
+        else
+          v[m] = v[m] * pf[code];
+      }
+      // Rare border cells.  The loop runs over the cells that any lane of the
+      // wavefront needs, so the CSG walk inside stays in uniform control flow
+      // (scalar descriptor loads); the field values wait in the lane's own
+      // canonical LDS slots (no other lane touches them between the two FFTs)
+      // so that they are not live in registers meanwhile.
+      unsigned wave_cross = 0;
 #pragma unroll
       for (int m = 0; m < 8; ++m)
-        if (!((crossing >> m) & 1u)) v[m] = v[m] * pf[(eidx[m >> 1] >> (16 * (m & 1))) & 0xffffu];
-      if (crossing) {
-        // rare border cells: the field values go to the lane's own canonical
-        // LDS slots (no other lane touches them between the two FFTs) so that
-        // they are not live in registers while the crossing integral runs
+        if (__ballot((crossing >> m) & 1u) != 0) wave_cross |= 1u << m;
+      if (wave_cross) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) ll[lds_pos(t + m * T)] = v[m];
 #pragma unroll 1
         for (int m = 0; m < 8; ++m) {
-          if (!((crossing >> m) & 1u)) continue;
+          if (!((wave_cross >> m) & 1u)) continue;
           double d1, b1;
           crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, coord(a.ymin, t + m * T, a.dy), z0,
                             z1, d1, b1);
-          cd &slot = ll[lds_pos(t + m * T)];
-          slot = slot * screen_from_values(k, a.dz, d1, b1, a.inv_norm);
+          if ((crossing >> m) & 1u) {
+            cd &slot = ll[lds_pos(t + m * T)];
+            slot = slot * screen_from_values(k, a.dz, d1, b1, a.inv_norm);
+          }
         }
 #pragma unroll
         for (int m = 0; m < 8; ++m) v[m] = ll[lds_pos(t + m * T)];

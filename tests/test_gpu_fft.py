@@ -219,3 +219,22 @@ def test_fft3d_work_group_shapes_bitwise_and_autotune(ctx, prec):
     plan2 = P.Plan(ctx, prob)
     plan2.material_csg(*csg)
     assert plan2.autotune(2) == (lx, ly)  # cached for the same shape of problem
+    plan.close()
+    plan2.close()
+
+
+def test_context_outlived_by_its_plans():
+    """A context destroyed while a plan of it lives stays usable by that plan
+    and is freed with it; no new plan can be made from it."""
+    n = 32
+    g = fft3d_case(n, dz=10.0, nsteps=4)
+    u0 = gaussian_field(g, n, 3000.0)
+    c = P.Ctx(0)
+    plan = P.Plan(c, fft3d_problem(g, n, 5))
+    h = c.h
+    c.close()
+    plan.set_field(u0)
+    plan.step(1, 4)
+    assert rel_l2(plan.get_field(), O.fft3d_run(g, O.NoMaterial, u0, 1, 4)) < 1e-12
+    plan.close()
+    assert h is not None
