@@ -96,11 +96,17 @@ def copy_calibration(mib, iters=20):
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
+    b.copy_(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(iters):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 4, "iters": iters}), flush=True)
+    ms = e0.elapsed_time(e1) / iters
+    print(json.dumps({"copy_bytes": n * 4, "iters": iters + 1, "mean_us": round(ms * 1e3, 2),
+                      "read_plus_write_GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
 
 
 def cpu_baseline(target_s):

@@ -97,6 +97,10 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   // differs between z0 and z1, src/fftSolver3D.cpp:122-156), kept in the
   // lane's pad slot of the line's LDS until the inverse transform is done.
   uint16_t *codes = reinterpret_cast<uint16_t *>(ll + pad_slot(t));
+  // the field loads are issued first so that their latency overlaps the CSG work
+  cx<R> raw[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) raw[m] = f[m * T];
   int nw = 0;
   double z0 = 0.0, z1 = 0.0;
   if (csg) {
@@ -131,7 +135,8 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   }
 
   cd v[8];
-  load8(v, f, T);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = cvt<double>(raw[m]);
   if (a.mode & Y_INV) {
     fft_line<N, +1, double, WL>(v, ll, t, a.tw);
     const double k = a.kvec[b];
