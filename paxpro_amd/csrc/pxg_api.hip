@@ -122,6 +122,7 @@ struct pxg_plan_s {
   size_t reserved = 0;
   int lines_x = 1, lines_y = 1;
   DBuf field, tw, kx, ky, absx, absy, kvec;
+  DBuf mcode, mxdb;  // FFT3D: material map of the current z-step (CSG materials)
   std::vector<double> k_host;
   // material
   int mat_mode = MAT_NONE;
@@ -186,6 +187,7 @@ size_t plan_bytes(const pxg_problem &p)
       b += cells * p.nz * esz * p.batch;
     }
   }
+  if (p.kind == PXG_FFT3D) b += cells * (sizeof(uint16_t) + sizeof(cd));  // CSG material map
   if (p.kind == PXG_ADI) b += cells * p.batch * 16 * 4;   // coefficient planes
   if (p.kind == PXG_CN) b += cells * p.batch * 16 * 6;
   return b;
@@ -386,6 +388,8 @@ FftArgs<R> fft_args(pxg_plan P)
   a.inv_norm = 1.0 / static_cast<double>(P->cells);
   a.n_refr = p.n_refr ? p.n_refr : 10;
   a.quirks = p.quirks;
+  a.mcode = P->mcode.as<uint16_t>();
+  a.mxdb = P->mxdb.as<cd>();
   return a;
 }
 
@@ -424,6 +428,11 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
     ev0 = begin(P);
     record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
     a.step = st;
+    if (P->mat_mode == MAT_CSG) {
+      ev0 = begin(P);
+      record(P, "material_map", static_cast<double>(P->cells) * sizeof(uint16_t),
+             launch_material_map<R>(n, a, P->stream), ev0);
+    }
     a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
     ev0 = begin(P);
     record(P, "pass_y", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
@@ -460,6 +469,7 @@ double time_pass(pxg_plan P, bool pass_x, int lines, int reps)
     return pass_x ? launch_pass_x<R>(p.nx, lines, a, p.batch, P->stream)
                   : launch_pass_y<R>(p.nx, lines, a, p.batch, P->stream);
   };
+  if (!pass_x && P->mat_mode == MAT_CSG) hip_check(launch_material_map<R>(p.nx, a, P->stream), "material map");
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -779,6 +789,8 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
     P->k_host.assign(prob->batch, prob->k);
     if (prob->kind == PXG_FFT3D) {
+      P->mcode.alloc(P->cells * sizeof(uint16_t));
+      P->mxdb.alloc(P->cells * sizeof(cd));
       P->lines_y = max_lines(prob->nx, 81920);    // two work-groups per CU
       P->lines_x = max_lines(prob->nx, 162816);   // one wide work-group per CU (coalescing)
       // keep enough work-groups in flight for small batches
@@ -1281,7 +1293,7 @@ int pxg_step_profiled(pxg_plan P, unsigned first_step, unsigned nsteps, pxg_kern
 {
   return guarded([&] {
     check_plan(P);
-    const size_t need = 2 * (static_cast<size_t>(nsteps) * 4 + 8);
+    const size_t need = 2 * (static_cast<size_t>(nsteps) * 5 + 8);
     while (P->ev.size() < need) {
       hipEvent_t e;
       hip_check(hipEventCreate(&e), "hipEventCreate");

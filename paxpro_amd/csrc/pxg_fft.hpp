@@ -106,9 +106,6 @@ __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const cx<Rt
 // LDS position of logical line index i: one pad slot every 8 entries keeps the
 // radix-8 scatter of the early stages free of bank conflicts.
 __device__ __forceinline__ constexpr int lds_pos(int i) { return i + (i >> 3); }
-// The pad slot after group t (entries 8t..8t+7): never addressed by lds_pos, so
-// lane t may keep 16 bytes of its own there across transforms, without syncs.
-__device__ __forceinline__ constexpr int pad_slot(int t) { return 9 * t + 8; }
 template <int N>
 struct LdsLine {
   static constexpr int SIZE = N + N / 8;      // padded line length

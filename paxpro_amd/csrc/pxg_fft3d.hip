@@ -81,57 +81,24 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 {
   constexpr int T = N / 8;
   constexpr bool WL = (N / 8) <= 64;
-  constexpr int CSG_K = 2;
   __shared__ cd lds[L * LdsLine<N>::STRIDE];
-  __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
   const int c = blockIdx.x * L + line;
   const int b = blockIdx.y;
   cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
   cd *ll = lds + line * LdsLine<N>::STRIDE;
-  const double x = coord(a.xmin, c, a.dx);
   const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
 
-  // Material first, before the field occupies registers: a 16-bit code per
-  // cell of the lane (the deciding part at z1; bit 15 set when the material
-  // differs between z0 and z1, src/fftSolver3D.cpp:122-156), kept in the
-  // lane's pad slot of the line's LDS until the inverse transform is done.
-  uint16_t *codes = reinterpret_cast<uint16_t *>(ll + pad_slot(t));
-  // the field loads are issued first so that their latency overlaps the CSG work
+  // Field loads and, for a CSG material, the cells' material-map codes are
+  // issued together before any arithmetic.
   cx<R> raw[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) raw[m] = f[m * T];
-  int nw = 0;
-  double z0 = 0.0, z1 = 0.0;
+  uint16_t code[8];
   if (csg) {
-    z1 = coord(a.zmin, a.step, a.dz);
-    z0 = coord(a.zmin, a.step - 1, a.dz);
-    const int c0 = blockIdx.x * L;
-    nw = (a.mat.n_parts + 63) >> 6;
-    csg_cull(a.mat, mask, nw, coord(a.xmin, c0, a.dx), coord(a.xmin, c0 + L - 1, a.dx), a.ymin,
-             coord(a.ymin, N - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
-    bool any = false;
-    for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
-    if (!any || a.mat.is_ref) nw = 0;  // no part near this strip: every cell is vacuum
-#pragma unroll 1
-    for (int m0 = 0; m0 < 8; m0 += CSG_K) {
-      double yv[CSG_K];
+    const uint16_t *mc = a.mcode + static_cast<size_t>(c) * N + t;
 #pragma unroll
-      for (int q = 0; q < CSG_K; ++q) yv[q] = coord(a.ymin, t + (m0 + q) * T, a.dy);
-      int e1[CSG_K], e0[CSG_K];
-      csg_eval_x2<CSG_K>(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
-#pragma unroll
-      for (int q = 0; q < CSG_K; ++q) {
-        unsigned code = static_cast<unsigned>(e1[q]);
-        if (e1[q] != e0[q]) {
-          double d1, b1, d0, b0;
-          part_values(a.mat, e1[q], d1, b1);
-          part_values(a.mat, e0[q], d0, b0);
-          if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) code |= 0x8000u;
-        }
-        codes[m0 + q] = static_cast<uint16_t>(code);
-      }
-    }
+    for (int m = 0; m < 8; ++m) code[m] = mc[m * T];
   }
 
   cd v[8];
@@ -148,33 +115,27 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
       unsigned crossing = 0;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const unsigned code = codes[m];
-        if (code & 0x8000u)
+        if (code[m] & MAP_CROSSING)
           crossing |= 1u << m;
         else
-          v[m] = v[m] * pf[code];
+          v[m] = v[m] * pf[code[m]];
       }
-      // Rare border cells.  The loop runs over the cells that any lane of the
-      // wavefront needs, so the CSG walk inside stays in uniform control flow
-      // (scalar descriptor loads); the field values wait in the lane's own
-      // canonical LDS slots (no other lane touches them between the two FFTs)
-      // so that they are not live in registers meanwhile.
-      unsigned wave_cross = 0;
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        if (__ballot((crossing >> m) & 1u) != 0) wave_cross |= 1u << m;
-      if (wave_cross) {
+      // Border cells (rare): the screen of the crossing integral's (delta,
+      // beta).  The field values wait in the lane's own canonical LDS slots
+      // (no other lane touches them between the two FFTs) so that they are
+      // not live in registers across the out-of-line exp/sincos.
+      if (__ballot(crossing != 0) != 0) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) ll[lds_pos(t + m * T)] = v[m];
-#pragma unroll 1
-        for (int m = 0; m < 8; ++m) {
-          if (!((wave_cross >> m) & 1u)) continue;
-          double d1, b1;
-          crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, coord(a.ymin, t + m * T, a.dy), z0,
-                            z1, d1, b1);
-          if ((crossing >> m) & 1u) {
+        const cd *xdb = a.mxdb + static_cast<size_t>(c) * N + t;
+        unsigned pending = crossing;
+        while (__ballot(pending != 0) != 0) {
+          if (pending) {
+            const int m = __builtin_ctz(pending);
+            pending &= pending - 1;
+            const cd db = xdb[m * T];
             cd &slot = ll[lds_pos(t + m * T)];
-            slot = slot * screen_from_values(k, a.dz, d1, b1, a.inv_norm);
+            slot = slot * screen_from_values(k, a.dz, db.x, db.y, a.inv_norm);
           }
         }
 #pragma unroll
@@ -205,6 +166,52 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   }
   if (a.mode & Y_FWD) fft_line<N, -1, double, WL>(v, ll, t, a.tw);
   store8(v, f, T);
+}
+
+// ---------------------------------------------------------------------------
+// Material map of one z-step (src/fftSolver3D.cpp:122-156), computed once for
+// every field of the batch: per cell a 16-bit code holding the deciding part
+// at z1, with MAP_CROSSING set when the material differs between z0 and z1,
+// and for those border cells the crossing integral's (delta, beta) in mxdb.
+// A work-group covers MAP_ROWS rows of one column; parts are culled against
+// that tile, so most work-groups see vacuum and only write codes.
+constexpr int MAP_ROWS = 256;
+
+template <class R>
+__global__ __launch_bounds__(MAP_ROWS) void k_material_map(FftArgs<R> a, int n)
+{
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  const int c = blockIdx.y;
+  const int r0 = blockIdx.x * MAP_ROWS;
+  const int r = r0 + static_cast<int>(threadIdx.x);
+  const double x = coord(a.xmin, c, a.dx);
+  const double z1 = coord(a.zmin, a.step, a.dz);
+  const double z0 = coord(a.zmin, a.step - 1, a.dz);
+  int nw = (a.mat.n_parts + 63) >> 6;
+  csg_cull(a.mat, mask, nw, x, x, coord(a.ymin, r0, a.dy), coord(a.ymin, min(n - 1, r0 + MAP_ROWS - 1), a.dy),
+           fmin(z0, z1), fmax(z0, z1));
+  bool any = false;
+  for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
+  if (!any || a.mat.is_ref) nw = 0;
+  const double yv[1] = {coord(a.ymin, min(r, n - 1), a.dy)};
+  int e1[1], e0[1];
+  csg_eval_x2<1>(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
+  bool cross = false;
+  if (e1[0] != e0[0]) {
+    double d1, b1, d0, b0;
+    part_values(a.mat, e1[0], d1, b1);
+    part_values(a.mat, e0[0], d0, b0);
+    cross = fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10;
+  }
+  cross = cross && r < n;
+  // evaluated by the whole wavefront when any lane needs it, so that the CSG
+  // walk stays in uniform control flow (scalar descriptor loads)
+  if (__ballot(cross) != 0) {
+    double d, b;
+    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, yv[0], z0, z1, d, b);
+    if (cross) a.mxdb[static_cast<size_t>(c) * n + r] = cd{d, b};
+  }
+  if (r < n) a.mcode[static_cast<size_t>(c) * n + r] = static_cast<uint16_t>(e1[0] | (cross ? MAP_CROSSING : 0u));
 }
 
 // ---------------------------------------------------------------------------
@@ -379,6 +386,13 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 }
 
 template <class R>
+hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_material_map<R>, dim3((n + MAP_ROWS - 1) / MAP_ROWS, n), dim3(MAP_ROWS), 0, s, a, n);
+  return hipGetLastError();
+}
+
+template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
   // one field per line; 4 lines per work-group when the batch divides evenly
@@ -402,6 +416,8 @@ template hipError_t launch_pass_y<float>(int, int, const FftArgs<float> &, int, 
 template hipError_t launch_pass_y<double>(int, int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
+template hipError_t launch_material_map<float>(int, const FftArgs<float> &, hipStream_t);
+template hipError_t launch_material_map<double>(int, const FftArgs<double> &, hipStream_t);
 template hipError_t launch_fft2d_step<float>(int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_fft2d_step<double>(int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_cube_store<float>(const cx<float> *, size_t, int, int, cx<double> *,

@@ -29,7 +29,14 @@ struct FftArgs {
   int mode;                // pass-Y mode bits (Y_INV | Y_FWD)
   cx<R> *sol;              // FFT2D: solution columns (Nx x Nz per field) or null
   size_t sol_fstride;
+  // FFT3D CSG material map of the current z-step, shared by the batch:
+  uint16_t *mcode;         // [N*N] deciding part at z1; bit 15: border crossing
+  cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
 };
+
+// Bit of a material-map code marking a border cell (material differs between
+// the two planes of the step); the other bits hold the deciding part index.
+constexpr unsigned MAP_CROSSING = 0x8000u;
 
 enum { Y_INV = 1, Y_FWD = 2 };
 
@@ -38,6 +45,8 @@ template <class R>
 hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
