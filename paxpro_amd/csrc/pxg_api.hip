@@ -791,8 +791,10 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     if (prob->kind == PXG_FFT3D) {
       P->mcode.alloc(P->cells * sizeof(uint16_t));
       P->mxdb.alloc(P->cells * sizeof(cd));
-      P->lines_y = max_lines(prob->nx, 81920);    // two work-groups per CU
-      P->lines_x = max_lines(prob->nx, 162816);   // one wide work-group per CU (coalescing)
+      // untuned defaults (pxg_plan_autotune measures): pass Y one wavefront of
+      // lines per work-group, pass X wide work-groups for coalescing
+      P->lines_y = std::min(max_lines(prob->nx, 81920), std::max(1, 512 / prob->nx));
+      P->lines_x = max_lines(prob->nx, 162816);
       // keep enough work-groups in flight for small batches
       while (P->lines_y > 1 && (prob->nx / P->lines_y) * prob->batch < 512) P->lines_y /= 2;
       while (P->lines_x > 4 && (prob->nx / P->lines_x) * prob->batch < 512) P->lines_x /= 2;
