@@ -38,6 +38,8 @@ LAMBDA = 0.1
 WAIST = 3000.0
 SPHERE = (0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 counter summary of this workload: HBM bytes per launch of each pass
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "fft3d_passes_512x512x64_c64_traffic.txt")
 
 
 def log(*a):
@@ -107,6 +109,24 @@ def copy_calibration(mib, iters=20):
     ms = e0.elapsed_time(e1) / iters
     print(json.dumps({"copy_bytes": n * 4, "iters": iters + 1, "mean_us": round(ms * 1e3, 2),
                       "read_plus_write_GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` (the plan's record name) on this
+    workload, from the committed rocprofv3 summary: FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by that counter's reading for a plain copy of
+    known size.  None when no summary covers this kernel and batch."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            lines = fh.read().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        if line.startswith("{"):
+            d = json.loads(line)
+            if d.get("kernel") == kernel and d.get("batch") == batch and d.get("grid") == N:
+                return d["traffic_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(target_s):
@@ -208,7 +228,8 @@ def main():
     dom = max(stats.items(), key=lambda kv: kv[1]["mean_ms"] * kv[1]["launches"])
     achieved = dom[1]["bytes_per_launch"] / (dom[1]["mean_ms"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom[0], B),
             "alg_bytes_per_launch": dom[1]["bytes_per_launch"],
             "mean_launch_us": round(dom[1]["mean_ms"] * 1e3, 2)}
     pm.close()
