@@ -15,6 +15,12 @@ the mixed result of field 0 (lambda = 0.1 nm) is compared with the FP64 one
 timed region starts.
 
 Prints ONE JSON line on stdout; diagnostics go to stderr.
+
+`--workload` selects another measurement of the same path (not the headline
+line): `cn2d` (CrankNicholson slab guide, Nx 1024, 2000 steps), `adi3d` (ADI
+2048^2, 500 steps, 64 spheres), `cn_jobs` (256 curved-waveguide CN jobs of
+uneven Nx, 1000 steps, one launch) and `served` (the request server on a
+Poisson trace of FFT3D and CN requests: throughput and p50/p99 latency).
 """
 import argparse
 import json
@@ -28,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import paxpro_amd as P  # noqa: E402
+from paxpro_amd import workloads as W  # noqa: E402
 
 N = 512
 DX = 20000.0 / 512
@@ -75,6 +82,7 @@ def make_plan(ctx, precision, batch, nz):
     plan = P.Plan(ctx, prob)
     plan.set_wavenumbers([2 * np.pi / (LAMBDA * (1 + 0.002 * b)) for b in range(batch)])
     plan.material_csg(*sphere_csg())
+    plan.autotune()  # work-group shapes of both passes, timed on this plan (zeroes the field)
     return plan
 
 
@@ -153,10 +161,255 @@ def cpu_baseline(target_s):
                       f"(of 1000) in {dt:.1f} s, FP64, self-written radix-2 FFT (not FFTW)"}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    return O
+
+
+def rel_l2(a, ref):
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+
+
+def rel_l2_int(a, ref):
+    ia, ir = np.abs(a) ** 2, np.abs(ref) ** 2
+    return float(np.linalg.norm(ia - ir) / np.linalg.norm(ir))
+
+
+def roofline_of(stats, steps_per_launch=1):
+    """HBM roofline of the dominant kernel of a pxg_step_profiled run."""
+    dom = max(stats.items(), key=lambda kv: kv[1]["mean_ms"] * kv[1]["launches"])
+    achieved = dom[1]["bytes_per_launch"] / (dom[1]["mean_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "alg_bytes_per_launch": dom[1]["bytes_per_launch"],
+            "mean_launch_us": round(dom[1]["mean_ms"] * 1e3, 2), "launches": dom[1]["launches"]}
+
+
+def line(metric, value, unit, steps, warmup, ms_per_step, dtype, data, config, roof, cpu, **extra):
+    out = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data, "config": config,
+           "roofline": roof, "cpu_baseline": cpu}
+    out.update(extra)
+    if cpu and cpu.get("value"):
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    return out
+
+
+def bench_cn2d(args, ctx):
+    """CrankNicholson slab waveguide: one system of 1024 cells, 2000 steps."""
+    steps = args.steps or 2000
+    res = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        prob, db, u0 = W.cn_slab(prec, nsteps=max(steps, args.warmup))
+        plan = P.Plan(ctx, prob)
+        plan.material_planes(db, z_invariant=True)
+        plan.set_field(u0)
+        if args.warmup:
+            timed(plan, args.warmup)
+            plan.set_field(u0)
+        t = timed(plan, steps)
+        out = plan.get_field()
+        plan.set_field(u0)
+        _, stats = plan.step_profiled(1, steps)
+        res[prec] = (t, out, stats)
+        log(f"cn2d prec {prec}: {t * 1e3:.2f} ms for {steps} steps")
+        plan.close()
+    t64, ref, _ = res[P.PREC_FP64]
+    tm, mix, stats = res[P.PREC_MIXED]
+    nx = 1024
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        prob, _, u0 = W.cn_slab(P.PREC_FP64, nsteps=steps)
+        g = O.grid(prob.xmin, prob.xmin + (nx - 1) * prob.dx, prob.dx, 0.0, steps * prob.dz, prob.dz, prob.k)
+        mat = O.SlabMaterial(-50.0, 50.0, 0.0, 0.0, 4.5e-5, 3.5e-6)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.cn_run(g, mat, u0, 1, steps, bc=O.BC_TRANSPARENT, absorb_npix=50, absorb_inv=0.1)
+            times.append(time.perf_counter() - t0)
+        dt = sorted(times)[1]
+        cpu = {"value": nx * steps / dt / 1e6, "unit": "Mcell*steps/s", "cores": O.num_threads(),
+               "kind": "port", "sample": f"oracle CrankNicholson restatement, the whole workload "
+                                         f"({steps} steps), median of 3: {dt:.3f} s, FP64 serial Thomas"}
+    roof = roofline_of(stats)
+    roof["note"] = "one dependent tridiagonal solve per step: latency-bound, not HBM-bound"
+    return line("Mcell*steps/s mixed-prec vs FP64 at <=1e-5 rel L2 (2D Crank-Nicolson)",
+                nx * steps / tm / 1e6, "Mcell*steps/s", steps, args.warmup, tm * 1e3 / steps,
+                "c64 storage, fp64 sweeps", "synthetic: plane wave into a slab guide",
+                {"workload": "CrankNicholson Nx 1024 over 1 um, 2000 steps of 500 nm, lambda 0.1 nm, "
+                             "slab core |x| < 50 nm, cladding d 4.5e-5 b 3.5e-6, transparent BC, "
+                             "50-cell absorber"},
+                roof, cpu, rel_l2_field_vs_fp64=rel_l2(mix, ref), rel_l2_intensity_vs_fp64=rel_l2_int(mix, ref),
+                fp64_single={"value": round(nx * steps / t64 / 1e6, 2), "ms_per_step": round(t64 * 1e3 / steps, 5)})
+
+
+def bench_adi3d(args, ctx):
+    """ADI on 2048^2 with a 64-sphere phantom, 500 steps."""
+    steps = args.steps or 500
+    csg = W.spheres_csg(W.adi_phantom())
+    res = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        prob, u0 = W.adi_problem(prec, nsteps=500)
+        plan = P.Plan(ctx, prob)
+        plan.material_csg(*csg)
+        plan.set_field(u0)
+        if args.warmup:
+            timed(plan, args.warmup)
+            plan.set_field(u0)
+        t = timed(plan, steps)
+        out = plan.get_field()
+        plan.set_field(u0)
+        _, stats = plan.step_profiled(1, min(steps, 50))
+        res[prec] = (t, out, stats)
+        log(f"adi3d prec {prec}: {t * 1e3:.2f} ms for {steps} steps")
+        plan.close()
+    n = 2048
+    t64, ref, _ = res[P.PREC_FP64]
+    tm, mix, stats = res[P.PREC_MIXED]
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        prob, u0 = W.adi_problem(P.PREC_FP64, nsteps=500)
+        g = O.grid(prob.xmin, prob.xmin + (n - 1) * prob.dx, prob.dx, prob.zmin, prob.zmin + 500 * prob.dz,
+                   prob.dz, prob.k)
+        mat = O.CSGMaterial(*csg)
+        t0 = time.perf_counter()
+        O.adi_run(g, mat, u0, 1, 1)
+        one = time.perf_counter() - t0
+        k = max(1, min(steps, int(args.cpu_seconds / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        O.adi_run(g, mat, u0, 1, k)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n * n * k / dt / 1e6, "unit": "Mcell*steps/s", "cores": O.num_threads(),
+               "kind": "port", "sample": f"oracle ADI restatement, 2048^2 with the 64-sphere phantom, "
+                                         f"first {k} of {steps} steps in {dt:.1f} s, FP64"}
+    return line("Mcell*steps/s mixed-prec vs FP64 at <=1e-5 rel L2 (3D ADI)",
+                n * n * steps / tm / 1e6, "Mcell*steps/s", steps, args.warmup, tm * 1e3 / steps,
+                "c64 storage, fp64 arithmetic", "synthetic: Gaussian beam + 64-sphere CSG phantom",
+                {"workload": "ADI 2048x2048 over 40x40 um, 500 steps of 20 nm, lambda 0.1 nm, Gaussian "
+                             "waist 8 um, 64 spheres (seed 0x5EED0002), transparent BC"},
+                roofline_of(stats), cpu, rel_l2_field_vs_fp64=rel_l2(mix, ref),
+                rel_l2_intensity_vs_fp64=rel_l2_int(mix, ref),
+                fp64_single={"value": round(n * n * steps / t64 / 1e6, 2),
+                             "ms_per_step": round(t64 * 1e3 / steps, 5)})
+
+
+def bench_cn_jobs(args, ctx):
+    """256 curved-waveguide CrankNicholson jobs of uneven Nx in one launch."""
+    steps = args.steps or 1000
+    jobs = W.curved_guide_jobs(count=args.jobs, nsteps=steps)
+    cells = sum(j["nx"] for j in jobs)
+    res = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        plan = P.JobPlan(ctx, [W.curved_guide_problem(j, prec) for j in jobs])
+
+        def load():
+            for b, j in enumerate(jobs):
+                plan.set_field(np.ones(j["nx"], complex), b)
+
+        for b, j in enumerate(jobs):
+            plan.material_curved_job(b, **j["wg"])
+        load()
+        if args.warmup:
+            timed(plan, args.warmup)
+            load()
+        t = timed(plan, steps)
+        outs = [plan.get_field(b) for b in range(len(jobs))]
+        load()
+        _, stats = plan.step_profiled(1, steps)
+        res[prec] = (t, outs, stats)
+        log(f"cn_jobs prec {prec}: {t * 1e3:.2f} ms for {steps} steps")
+        plan.close()
+    t64, refs, _ = res[P.PREC_FP64]
+    tm, mixs, stats = res[P.PREC_MIXED]
+    ef = max(rel_l2(m, r) for m, r in zip(mixs, refs))
+    ei = max(rel_l2_int(m, r) for m, r in zip(mixs, refs))
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        done, dt = 0, 0.0
+        for j in jobs:
+            g = O.grid(j["xmin"], j["xmin"] + (j["nx"] - 1) * j["dx"], j["dx"], 0.0, steps * j["dz"], j["dz"],
+                       2 * np.pi / j["lam"])
+            t0 = time.perf_counter()
+            O.cn_run(g, O.CurvedMaterial(**j["wg"]), np.ones(j["nx"], complex), 1, steps)
+            dt += time.perf_counter() - t0
+            done += 1
+            if dt > args.cpu_seconds:
+                break
+        cs = sum(j["nx"] for j in jobs[:done]) * steps
+        cpu = {"value": cs / dt / 1e6, "unit": "Mcell*steps/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"oracle CrankNicholson restatement, first {done} of {len(jobs)} jobs, all "
+                         f"{steps} steps each, one after another, {dt:.1f} s"}
+    roof = roofline_of(stats)
+    roof["note"] = "one work-group per job marching all steps: latency of the dependent sweeps"
+    return line("Mcell*steps/s mixed-prec vs FP64 at <=1e-5 rel L2 (batched Crank-Nicolson jobs)",
+                cells * steps / tm / 1e6, "Mcell*steps/s", steps, args.warmup, tm * 1e3 / steps,
+                "c64 storage, fp64 sweeps", "synthetic: plane waves into curved waveguides",
+                {"workload": f"{len(jobs)} CrankNicholson jobs, Nx log-uniform 256..8192 (total {cells} cells), "
+                             f"{steps} steps of 100 nm, lambda U[0.05, 0.2] nm, curved guides R U[1e6, 5e7] nm, "
+                             "w U[50, 200] nm (seed 0x5EED0003), transparent BC"},
+                roof, cpu, rel_l2_field_vs_fp64_max=ef, rel_l2_intensity_vs_fp64_max=ei,
+                fp64_batch={"value": round(cells * steps / t64 / 1e6, 2), "ms_per_step": round(t64 * 1e3 / steps, 5)})
+
+
+def bench_served(args, ctx):
+    """The request server on a Poisson trace: throughput and p50/p99 latency."""
+    trace = W.served_trace(args.requests, rate=args.rate)
+    fields = {}
+    for r in trace:
+        key = (r["kind"], r["n"])
+        if key not in fields:
+            fields[key] = (W.gaussian_field(r["n"], 20000.0 / r["n"], -10000.0, 3000.0) if r["kind"] == "fft3d"
+                           else np.ones(r["n"], complex))
+    srv = P.Server(ctx, slots=args.slots, chunk_steps=args.chunk)
+    ids = []
+    t0 = time.perf_counter()
+    for r in trace:
+        delay = t0 + r["t"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(r["prob"], fields[(r["kind"], r["n"])], r["nsteps"], csg=r.get("csg"),
+                              curved=r.get("curved")))
+    t_sub = time.perf_counter() - t0
+    lat = []
+    failed = 0
+    for jid in ids:
+        rc, ms, _ = srv.wait(jid, timeout=600.0)
+        if rc == P.OK:
+            lat.append(ms)
+        else:
+            failed += 1
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    offered = sum(r["cells"] * r["nsteps"] for r in trace)
+    lat = np.array(lat)
+    log(f"served {len(trace)} requests in {wall:.2f} s (submitted over {t_sub:.2f} s), failed {failed}")
+    return line("Mcell*steps/s served, with p50/p99 request latency (mixed FFT3D + CN stream)",
+                st["cell_steps"] / wall / 1e6, "Mcell*steps/s", len(trace), 0, wall * 1e3 / len(trace),
+                "c64 storage, fp64 arithmetic", "synthetic Poisson trace (seed 0x5EED0004)",
+                {"workload": f"{len(trace)} requests at {args.rate} req/s: 70% FFT3D N in {{256..2048}}, "
+                             "200..2000 steps, 64-sphere phantoms; 30% curved-guide CN jobs",
+                 "slots": args.slots, "chunk_steps": args.chunk},
+                None, None, p50_ms=round(float(np.percentile(lat, 50)), 2) if lat.size else None,
+                p99_ms=round(float(np.percentile(lat, 99)), 2) if lat.size else None,
+                mean_ms=round(float(lat.mean()), 2) if lat.size else None, failed=failed,
+                offered_mcell_steps_per_s=round(offered / trace[-1]["t"] / 1e6, 1),
+                utilisation=round(st["busy_s"] / wall, 3), wall_s=round(wall, 3))
+
+
+WORKLOADS = {"cn2d": bench_cn2d, "adi3d": bench_adi3d, "cn_jobs": bench_cn_jobs, "served": bench_served}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="z-steps timed (default: the workload's own, 1000 for fft3d)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -167,6 +420,13 @@ def main():
                     help="time only the batched mixed path (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--workload", default="fft3d", choices=["fft3d"] + sorted(WORKLOADS),
+                    help="fft3d: the headline line; the others measure the rest of the path")
+    ap.add_argument("--jobs", type=int, default=256, help="cn_jobs: number of jobs")
+    ap.add_argument("--requests", type=int, default=100, help="served: requests in the trace")
+    ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
+    ap.add_argument("--slots", type=int, default=8, help="served: jobs running at once")
+    ap.add_argument("--chunk", type=int, default=64, help="served: steps per chunk")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -175,6 +435,11 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "fft3d":
+        print(json.dumps(WORKLOADS[args.workload](args, P.Ctx(0))), flush=True)
+        return
+    if args.steps is None:
+        args.steps = 1000
     nz = max(args.steps, args.warmup) + 1
     if args.profile_steps <= 0:
         args.profile_steps = args.steps
@@ -203,6 +468,7 @@ def main():
         load_fields(pm, u0, B)
     tm = timed(pm, args.steps)
     mix = pm.get_field(0)
+    lines = pm.lines()
     vm = B * N * N * args.steps / tm / 1e6
     log(f"mixed batch {B}: {tm * 1e3:.2f} ms -> {vm:.1f} Mcell*steps/s")
     if args.mixed_only:
@@ -254,7 +520,7 @@ def main():
         "data": "synthetic: Gaussian beam + device-evaluated CSG sphere",
         "config": {"workload": "FFTSolver3D 512x512 over 20x20 um, 1000 z-steps of 10 nm, "
                                "lambda 0.1 nm, Gaussian waist 3 um, sphere r 4 um d 5e-6 b 1e-7",
-                   "batch_fields": B, "grid": N, "z_steps": args.steps},
+                   "batch_fields": B, "grid": N, "z_steps": args.steps, "lines_x_y": list(lines)},
         "rel_l2_field_vs_fp64": ef,
         "rel_l2_intensity_vs_fp64": ei,
         "fp64_single": {"value": round(v64, 1), "unit": "Mcell*steps/s",

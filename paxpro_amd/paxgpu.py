@@ -19,6 +19,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 BC_DIRICHLET, BC_TRANSPARENT = 0, 1
 FIX_ADI_Y_DIAG, FIX_REFR_SAMPLES = 1, 2
 LAYOUT_CONTIGUOUS, LAYOUT_INTERLEAVED = 0, 1
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_CYLINDER = 0, 1, 2
+OP_UNION, OP_DIFFERENCE = 0, 1
 
 SHAPE_DTYPE = np.dtype([("type", np.int32), ("op", np.int32), ("T", np.float64, 16),
                         ("p", np.float64, 3)])
