@@ -53,7 +53,10 @@ enum { PXG_FFT2D = 0, PXG_FFT3D = 1, PXG_CN = 2, PXG_ADI = 3 };
  * widens to FP64 on load and computes in FP64 with FP64 tables, rounding to
  * complex64 once per store.  FP32 arithmetic was measured to accumulate a
  * coherent error (the rounded twiddles and propagator act as a fixed operator
- * perturbation, growing linearly with steps), FP64 arithmetic does not. */
+ * perturbation, growing linearly with steps), FP64 arithmetic does not.
+ * CN plans keep complex128 state in both precisions: their march is
+ * latency-bound, and a slowly varying 1D field rounded to complex64 every step
+ * accumulates a coherent error (measured 5.7e-5 after 1000 steps). */
 enum { PXG_PREC_FP64 = 0, PXG_PREC_MIXED = 1 };
 enum { PXG_BC_DIRICHLET = 0, PXG_BC_TRANSPARENT = 1 };   /* Solver2D::BC_t */
 /* Quirk switches: 0 reproduces the reference; a set bit selects the physics form. */

@@ -220,3 +220,23 @@ def test_cn_job_plan_uneven_curved_guides(ctx):
             sol = plan.get_solution(b)
             assert sol.shape == (jobs[b][0], nsteps + 1)
             assert rel_l2(sol[:, -1], refs[b]) < tol
+
+
+def test_cn_mixed_long_curved_guide_within_bound(ctx):
+    """1000 steps on the curved guide of the batched-jobs workload that drifted
+    to 5.7e-5 when the CN state was stored as complex64: CN plans keep
+    complex128 state, so the mixed plan agrees with FP64 far below 1e-5."""
+    from paxpro_amd import workloads as W
+    jobs = W.curved_guide_jobs()
+    out = []
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        plan = P.JobPlan(ctx, [W.curved_guide_problem(j, prec) for j in jobs[:8]])
+        for b, j in enumerate(jobs[:8]):
+            plan.material_curved_job(b, **j["wg"])
+            plan.set_field(np.ones(j["nx"], complex), b)
+        plan.step(1, 1000)
+        out.append([plan.get_field(b) for b in range(8)])
+        plan.close()
+    worst = max(rel_l2(m, r) for m, r in zip(out[1], out[0]))
+    announce("cn mixed 1000 steps worst rel_l2", worst)
+    assert worst < 1e-5
