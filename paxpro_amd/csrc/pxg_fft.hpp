@@ -83,13 +83,26 @@ __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const cx<Rt
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
     if constexpr (NS > 1) {
+      // One table load per butterfly: w = e^{-2 pi i jm/(NS R)}; its powers
+      // w^2..w^{R-1} are products at most three deep (a few ulp in FP64).
+      // Loading all R-1 powers cost R-1 scattered reads per lane and stage,
+      // which dominated the pass time.
       const int jm = j & (NS - 1);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cx<Rt> w = tw[r * jm * (N / (NS * R))];
-        if constexpr (S > 0) w = conj(w);
-        u[r] = u[r] * w;
+      cx<Rt> w[8];
+      w[1] = tw[jm * (N / (NS * R))];
+      if constexpr (S > 0) w[1] = conj(w[1]);
+      if constexpr (R >= 4) {
+        w[2] = w[1] * w[1];
+        w[3] = w[2] * w[1];
       }
+      if constexpr (R == 8) {
+        w[4] = w[2] * w[2];
+        w[5] = w[4] * w[1];
+        w[6] = w[3] * w[3];
+        w[7] = w[4] * w[3];
+      }
+#pragma unroll
+      for (int r = 1; r < R; ++r) u[r] = u[r] * w[r];
     }
     if constexpr (R == 8) {
       dft8<S>(u);
