@@ -173,7 +173,10 @@ struct StockhamStages {
       stage_store<N, R, NS>(v, lds, t);
       lds_sync<WL>();
       load_canon<N>(v, lds, t);
-      lds_sync<WL>();
+      // Ordering before the next stage's stores.  After the last exchange each
+      // lane has read only its own canonical slots, so nothing is needed
+      // there: a following transform starts with its own sync.
+      if constexpr (NS * R * 8 < N) lds_sync<WL>();
       StockhamStages<N, S, NS * R, Rt, WL>::run(v, lds, t, tw);
     }
   }
@@ -182,13 +185,15 @@ struct StockhamStages {
 // Full length-N transform of the canonical registers (S = -1 forward, +1
 // backward, unnormalised).  `lds` is this line's LdsLine<N>::SIZE slice; every
 // lane of the work-group must call it (it contains work-group barriers unless
-// WL, see lds_sync).  A leading sync protects LDS still being read by a
-// preceding transform.
-template <int N, int S, class Rt, bool WL = false>
+// WL, see lds_sync).  Between transforms a lane may touch only its own
+// canonical slots (lds_pos(t + m N/8)).  The leading sync protects LDS still
+// being read by a preceding transform; LEAD = false skips it where the line's
+// LDS has not been used before in the kernel.
+template <int N, int S, class Rt, bool WL = false, bool LEAD = true>
 __device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t,
                                          const cx<Rt> *__restrict__ tw)
 {
-  lds_sync<WL>();
+  if constexpr (LEAD) lds_sync<WL>();
   StockhamStages<N, S, 1, Rt, WL>::run(v, lds, t, tw);
 }
 

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = cvt<double>(raw[m]);
   if (a.mode & Y_INV) {
-    fft_line<N, +1, double, WL>(v, ll, t, a.tw);
+    fft_line<N, +1, double, WL, false>(v, ll, t, a.tw);
     const double k = a.kvec[b];
     if (a.mat.mode == MAT_NONE) {
 #pragma unroll
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   cd *ll = lds + l * LdsLine<N>::STRIDE;
   cd v[8];
   load8(v, f, static_cast<size_t>(T) * N);
-  fft_line<N, -1>(v, ll, t, a.tw);
+  fft_line<N, -1, double, false, false>(v, ll, t, a.tw);
   const cd kyr = a.ky[b * N + r];
   const cd *kx = a.kx + b * N;
 #pragma unroll
