@@ -12,7 +12,8 @@
 // kernel can load a line from HBM straight into registers, chain transforms and
 // point-wise operations in registers, and store straight back: only the inner
 // stage exchanges go through LDS.  Twiddles come from an FP64-accurate table
-// tw[m] = exp(-2*pi*i*m/N) rounded once to the working precision.
+// tw[m] = exp(-2*pi*i*m/N) rounded once to the working precision: one base per
+// stage and lane, its powers formed by products.
 #pragma once
 
 #include "pxg_common.hpp"
@@ -70,9 +71,51 @@ __device__ __forceinline__ void dft8(cx<R> (&u)[8])
   u[7] = e3 - t3;
 }
 
+// Stages after the first carry twiddles; they are radix 8, so each lane has
+// one butterfly (j = t) and needs one base twiddle w = tw[(t mod Ns) N/(8 Ns)]
+// per stage.  The bases depend only on the lane: a kernel loads them once
+// (tw_base) and passes them down, so no table read sits behind a barrier.
+template <int N>
+constexpr int tw_stages()
+{
+  int n = N / FftShape<N>::FIRST, s = 0;
+  while (n > 1) {
+    n /= 8;
+    ++s;
+  }
+  return s;
+}
+
+// Twiddle slot of the stage whose input spans NS = FIRST * 8^slot.
+template <int N, int NS>
+constexpr int tw_slot()
+{
+  int ns = FftShape<N>::FIRST, s = 0;
+  while (ns < NS) {
+    ns *= 8;
+    ++s;
+  }
+  return s;
+}
+
+template <int N, class Rt>
+struct TwBase {
+  cx<Rt> w[tw_stages<N>() > 0 ? tw_stages<N>() : 1];
+};
+
+template <int N, class Rt>
+__device__ __forceinline__ TwBase<N, Rt> tw_base(const cx<Rt> *__restrict__ tw, int t)
+{
+  TwBase<N, Rt> b;
+#pragma unroll
+  for (int s = 0, ns = FftShape<N>::FIRST; s < tw_stages<N>(); ++s, ns *= 8)
+    b.w[s] = tw[(t & (ns - 1)) * (N / (ns * 8))];
+  return b;
+}
+
 // Radix-R butterflies of one stage on the canonical registers of lane t.
 template <int N, int R, int NS, int S, class Rt>
-__device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const cx<Rt> *__restrict__ tw)
+__device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TwBase<N, Rt> &twb)
 {
   constexpr int Q = 8 / R;
   constexpr int N8 = N / 8;
@@ -83,13 +126,17 @@ __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const cx<Rt
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
     if constexpr (NS > 1) {
-      // One table load per butterfly: w = e^{-2 pi i jm/(NS R)}; its powers
-      // w^2..w^{R-1} are products at most three deep (a few ulp in FP64).
-      // Loading all R-1 powers cost R-1 scattered reads per lane and stage,
-      // which dominated the pass time.
-      const int jm = j & (NS - 1);
+      // The base w = e^{-2 pi i (j mod NS)/(NS R)}; its powers w^2..w^{R-1}
+      // are products at most three deep (a few ulp in FP64).  Loading all R-1
+      // powers cost R-1 scattered reads per lane and stage, which dominated
+      // the pass time.
+      static_assert(R == 8, "twiddle stages are radix 8");
       cx<Rt> w[8];
-      w[1] = tw[jm * (N / (NS * R))];
+      w[1] = twb.w[tw_slot<N, NS>()];
+      // Opaque copy: keeps the powers local to this stage.  Without it the
+      // compiler forms them once per kernel and keeps up to 18 complex values
+      // live across both transforms (spills at 1024 lanes).
+      asm volatile("" : "+v"(w[1].x), "+v"(w[1].y));
       if constexpr (S > 0) w[1] = conj(w[1]);
       if constexpr (R >= 4) {
         w[2] = w[1] * w[1];
@@ -164,11 +211,10 @@ __device__ __forceinline__ void lds_sync()
 
 template <int N, int S, int NS, class Rt, bool WL>
 struct StockhamStages {
-  static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t,
-                                             const cx<Rt> *__restrict__ tw)
+  static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TwBase<N, Rt> &twb)
   {
     constexpr int R = (NS == 1) ? FftShape<N>::FIRST : 8;
-    stage_compute<N, R, NS, S>(v, t, tw);
+    stage_compute<N, R, NS, S>(v, t, twb);
     if constexpr (NS * R < N) {
       stage_store<N, R, NS>(v, lds, t);
       lds_sync<WL>();
@@ -177,10 +223,68 @@ struct StockhamStages {
       // lane has read only its own canonical slots, so nothing is needed
       // there: a following transform starts with its own sync.
       if constexpr (NS * R * 8 < N) lds_sync<WL>();
-      StockhamStages<N, S, NS * R, Rt, WL>::run(v, lds, t, tw);
+      StockhamStages<N, S, NS * R, Rt, WL>::run(v, lds, t, twb);
     }
   }
 };
+
+// Transforms whose first or last exchange also moves each line to another set
+// of lanes.  Mapping A is any lane layout (e.g. lines fastest, for coalesced
+// strided global access); mapping B keeps a line's N/8 lanes inside one
+// wavefront (N <= 512), so that its other exchanges need no work-group
+// barrier.  `la`/`lb` are the line's LDS slices under each mapping.
+//
+// Forward, A -> B: first stage in A, remapping exchange (work-group barrier),
+// remaining stages wave-local in B.  Registers end canonical in B.
+template <int N, int S>
+__device__ __forceinline__ void fft_line_remap_in(cd (&v)[8], cd *la, cd *lb, int ta, int tb,
+                                                  const TwBase<N, double> &twb_b)
+{
+  static_assert(N / 8 <= 64, "mapping B needs a line inside one wavefront");
+  constexpr int F = FftShape<N>::FIRST;
+  const TwBase<N, double> none{};
+  stage_compute<N, F, 1, S>(v, ta, none);
+  stage_store<N, F, 1>(v, la, ta);
+  __syncthreads();
+  load_canon<N>(v, lb, tb);
+  if constexpr (F * 8 < N) lds_sync<true>();
+  StockhamStages<N, S, F, double, true>::run(v, lb, tb, twb_b);
+}
+
+// Inverse, B -> A: stages wave-local in B up to the last exchange, which
+// remaps to A (work-group barrier); the last stage runs in A.  The caller's
+// previous use of lb must be wave-local (a leading wave-local sync is done).
+template <int N, int S, int NS>
+struct RemapOutStages {
+  static __device__ __forceinline__ void run(cd (&v)[8], cd *la, cd *lb, int ta, int tb,
+                                             const TwBase<N, double> &twb_a, const TwBase<N, double> &twb_b)
+  {
+    constexpr int R = (NS == 1) ? FftShape<N>::FIRST : 8;
+    static_assert(NS * R < N, "at least one stage follows");
+    stage_compute<N, R, NS, S>(v, tb, twb_b);
+    stage_store<N, R, NS>(v, lb, tb);
+    if constexpr (NS * R * 8 < N) {
+      lds_sync<true>();
+      load_canon<N>(v, lb, tb);
+      lds_sync<true>();
+      RemapOutStages<N, S, NS * R>::run(v, la, lb, ta, tb, twb_a, twb_b);
+    } else {
+      __syncthreads();
+      load_canon<N>(v, la, ta);
+      stage_compute<N, 8, NS * R, S>(v, ta, twb_a);
+    }
+  }
+};
+
+template <int N, int S>
+__device__ __forceinline__ void fft_line_remap_out(cd (&v)[8], cd *la, cd *lb, int ta, int tb,
+                                                   const TwBase<N, double> &twb_a,
+                                                   const TwBase<N, double> &twb_b)
+{
+  static_assert(N / 8 <= 64, "mapping B needs a line inside one wavefront");
+  lds_sync<true>();
+  RemapOutStages<N, S, 1>::run(v, la, lb, ta, tb, twb_a, twb_b);
+}
 
 // Full length-N transform of the canonical registers (S = -1 forward, +1
 // backward, unnormalised).  `lds` is this line's LdsLine<N>::SIZE slice; every
@@ -190,11 +294,10 @@ struct StockhamStages {
 // being read by a preceding transform; LEAD = false skips it where the line's
 // LDS has not been used before in the kernel.
 template <int N, int S, class Rt, bool WL = false, bool LEAD = true>
-__device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t,
-                                         const cx<Rt> *__restrict__ tw)
+__device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TwBase<N, Rt> &twb)
 {
   if constexpr (LEAD) lds_sync<WL>();
-  StockhamStages<N, S, 1, Rt, WL>::run(v, lds, t, tw);
+  StockhamStages<N, S, 1, Rt, WL>::run(v, lds, t, twb);
 }
 
 }  // namespace pxg

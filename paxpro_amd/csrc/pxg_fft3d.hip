@@ -88,6 +88,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
   cd *ll = lds + line * LdsLine<N>::STRIDE;
   const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
+  const TwBase<N, double> twb = tw_base<N>(a.tw, t);
 
   // Field loads and, for a CSG material, the cells' material-map codes are
   // issued together before any arithmetic.
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = cvt<double>(raw[m]);
   if (a.mode & Y_INV) {
-    fft_line<N, +1, double, WL, false>(v, ll, t, a.tw);
+    fft_line<N, +1, double, WL, false>(v, ll, t, twb);
     const double k = a.kvec[b];
     if (a.mat.mode == MAT_NONE) {
 #pragma unroll
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
       for (int m = 0; m < 8; ++m) v[m] = scale(scale(v[m], a.absy[t + m * T]), ax);
     }
   }
-  if (a.mode & Y_FWD) fft_line<N, -1, double, WL>(v, ll, t, a.tw);
+  if (a.mode & Y_FWD) fft_line<N, -1, double, WL>(v, ll, t, twb);
   store8(v, f, T);
 }
 
@@ -215,27 +216,43 @@ __global__ __launch_bounds__(MAP_ROWS) void k_material_map(FftArgs<R> a, int n)
 }
 
 // ---------------------------------------------------------------------------
-// Pass X: strided x-lines (matrix rows).  Lanes run line-fastest so that each
-// load/store instruction touches L consecutive rows of a column.
-// FFT_x, multiply by Kx(c) Ky(r), IFFT_x.
+// Pass X: strided x-lines (matrix rows).  Global loads and stores run with the
+// lines fastest across lanes (mapping A), so that each instruction touches L
+// consecutive rows of a column.  FFT_x, multiply by Kx(c) Ky(r), IFFT_x.  For
+// N <= 512 the first exchange of FFT_x moves every line into one wavefront
+// (mapping B) and the last exchange of IFFT_x moves it back: the exchanges in
+// between need no work-group barrier.
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 {
   constexpr int T = N / 8;
   __shared__ cd lds[L * LdsLine<N>::STRIDE];
-  const int tid = threadIdx.x, l = tid % L, t = tid / L;
-  const int r = blockIdx.x * L + l;
+  const int tid = threadIdx.x;
+  const int la = tid % L, ta = tid / L;  // mapping A
   const int b = blockIdx.y;
-  cx<R> *f = a.field + b * a.fstride + r + static_cast<size_t>(t) * N;
-  cd *ll = lds + l * LdsLine<N>::STRIDE;
+  const int ra = blockIdx.x * L + la;
+  cx<R> *f = a.field + b * a.fstride + ra + static_cast<size_t>(ta) * N;
+  cd *lla = lds + la * LdsLine<N>::STRIDE;
   cd v[8];
   load8(v, f, static_cast<size_t>(T) * N);
-  fft_line<N, -1, double, false, false>(v, ll, t, a.tw);
-  const cd kyr = a.ky[b * N + r];
   const cd *kx = a.kx + b * N;
+  if constexpr (T <= 64) {
+    const int lb = tid / T, tb = tid % T;  // mapping B
+    cd *llb = lds + lb * LdsLine<N>::STRIDE;
+    const TwBase<N, double> twa = tw_base<N>(a.tw, ta), twb = tw_base<N>(a.tw, tb);
+    fft_line_remap_in<N, -1>(v, lla, llb, ta, tb, twb);
+    const cd kyr = a.ky[b * N + blockIdx.x * L + lb];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[t + m * T] * kyr);
-  fft_line<N, +1>(v, ll, t, a.tw);
+    for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[tb + m * T] * kyr);
+    fft_line_remap_out<N, +1>(v, lla, llb, ta, tb, twa, twb);
+  } else {
+    const TwBase<N, double> twa = tw_base<N>(a.tw, ta);
+    fft_line<N, -1, double, false, false>(v, lla, ta, twa);
+    const cd kyr = a.ky[b * N + ra];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[ta + m * T] * kyr);
+    fft_line<N, +1>(v, lla, ta, twa);
+  }
   store8(v, f, static_cast<size_t>(T) * N);
 }
 
@@ -273,11 +290,12 @@ __global__ __launch_bounds__(L *N / 8) void k_fft2d_step(FftArgs<R> a)
   cd v[8];
   load8(v, f, T);
   constexpr bool WL = (N / 8) <= 64;
-  fft_line<N, -1, double, WL>(v, ll, t, a.tw);
+  const TwBase<N, double> twb = tw_base<N>(a.tw, t);
+  fft_line<N, -1, double, WL>(v, ll, t, twb);
   const cd *kx = a.kx + b * N;
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = v[m] * kx[t + m * T];
-  fft_line<N, +1, double, WL>(v, ll, t, a.tw);
+  fft_line<N, +1, double, WL>(v, ll, t, twb);
   const double k = a.kvec[b];
   const double z = coord(a.zmin, a.step, a.dz) + 0.5 * a.dz;
   const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
