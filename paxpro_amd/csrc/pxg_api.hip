@@ -432,14 +432,9 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
   for (unsigned st = first; st < first + nsteps; ++st) {
     const bool last = st + 1 == first + nsteps;
     const bool store = store_step(P, st);
+    a.step = st;  // pass X also builds this step's material map (CSG)
     ev0 = begin(P);
     record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
-    a.step = st;
-    if (P->mat_mode == MAT_CSG) {
-      ev0 = begin(P);
-      record(P, "material_map", static_cast<double>(P->cells) * sizeof(uint16_t),
-             launch_material_map<R>(n, a, P->stream), ev0);
-    }
     a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
     ev0 = begin(P);
     record(P, "pass_y", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
@@ -476,7 +471,8 @@ double time_pass(pxg_plan P, bool pass_x, int lines, int reps)
     return pass_x ? launch_pass_x<R>(p.nx, lines, a, p.batch, P->stream)
                   : launch_pass_y<R>(p.nx, lines, a, p.batch, P->stream);
   };
-  if (!pass_x && P->mat_mode == MAT_CSG) hip_check(launch_material_map<R>(p.nx, a, P->stream), "material map");
+  // pass Y reads the material map of the step, built by pass X
+  if (!pass_x) hip_check(launch_pass_x<R>(p.nx, P->lines_x, a, p.batch, P->stream), "pass x");
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");

@@ -170,49 +170,66 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 }
 
 // ---------------------------------------------------------------------------
-// Material map of one z-step (src/fftSolver3D.cpp:122-156), computed once for
-// every field of the batch: per cell a 16-bit code holding the deciding part
-// at z1, with MAP_CROSSING set when the material differs between z0 and z1,
-// and for those border cells the crossing integral's (delta, beta) in mxdb.
-// A work-group covers MAP_ROWS rows of one column; parts are culled against
-// that tile, so most work-groups see vacuum and only write codes.
-constexpr int MAP_ROWS = 256;
-
-template <class R>
-__global__ __launch_bounds__(MAP_ROWS) void k_material_map(FftArgs<R> a, int n)
+// Material map of z-step `a.step` (src/fftSolver3D.cpp:122-156), computed once
+// for every field of the batch by the field-0 work-groups of pass X, each for
+// its band of L rows: per cell a 16-bit code holding the deciding part at z1,
+// with MAP_CROSSING set when the material differs between z0 and z1, and for
+// those border cells the crossing integral's (delta, beta) in mxdb.  Pass Y of
+// the same step reads it.  Lane (row r, t) covers columns t + m*N/8.
+template <int N, class R, int L>
+__device__ __forceinline__ void material_map_rows(const FftArgs<R> &a, uint64_t *mask, int r, int t)
 {
-  __shared__ uint64_t mask[MAX_MASK_WORDS];
-  const int c = blockIdx.y;
-  const int r0 = blockIdx.x * MAP_ROWS;
-  const int r = r0 + static_cast<int>(threadIdx.x);
-  const double x = coord(a.xmin, c, a.dx);
+  constexpr int T = N / 8;
+  constexpr int K = 2;
   const double z1 = coord(a.zmin, a.step, a.dz);
   const double z0 = coord(a.zmin, a.step - 1, a.dz);
+  const int r0 = blockIdx.x * L;
   int nw = (a.mat.n_parts + 63) >> 6;
-  csg_cull(a.mat, mask, nw, x, x, coord(a.ymin, r0, a.dy), coord(a.ymin, min(n - 1, r0 + MAP_ROWS - 1), a.dy),
-           fmin(z0, z1), fmax(z0, z1));
+  csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), coord(a.ymin, r0, a.dy),
+           coord(a.ymin, r0 + L - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
   bool any = false;
   for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
-  if (!any || a.mat.is_ref) nw = 0;
-  const double yv[1] = {coord(a.ymin, min(r, n - 1), a.dy)};
-  int e1[1], e0[1];
-  csg_eval_x2<1>(a.mat, mask, nw, x, yv, z1, z0, e1, e0);
-  bool cross = false;
-  if (e1[0] != e0[0]) {
-    double d1, b1, d0, b0;
-    part_values(a.mat, e1[0], d1, b1);
-    part_values(a.mat, e0[0], d0, b0);
-    cross = fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10;
+  if (!any || a.mat.is_ref) nw = 0;  // no part near this band: every cell is vacuum
+  const double y = coord(a.ymin, r, a.dy);
+  unsigned crossing = 0;
+#pragma unroll 1
+  for (int m0 = 0; m0 < 8; m0 += K) {
+    double xv[K], yv[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      xv[q] = coord(a.xmin, t + (m0 + q) * T, a.dx);
+      yv[q] = y;
+    }
+    int e1[K], e0[K];
+    csg_eval_x2<K>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      unsigned code = static_cast<unsigned>(e1[q]);
+      if (e1[q] != e0[q]) {
+        double d1, b1, d0, b0;
+        part_values(a.mat, e1[q], d1, b1);
+        part_values(a.mat, e0[q], d0, b0);
+        if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
+          code |= MAP_CROSSING;
+          crossing |= 1u << (m0 + q);
+        }
+      }
+      a.mcode[static_cast<size_t>(t + (m0 + q) * T) * N + r] = static_cast<uint16_t>(code);
+    }
   }
-  cross = cross && r < n;
-  // evaluated by the whole wavefront when any lane needs it, so that the CSG
-  // walk stays in uniform control flow (scalar descriptor loads)
-  if (__ballot(cross) != 0) {
+  // border cells: the whole wavefront runs the loop while any lane has one
+  // left, so that the CSG walk stays in uniform control flow
+  unsigned pending = crossing;
+  while (__ballot(pending != 0) != 0) {
+    const int m = pending ? __builtin_ctz(pending) : 0;
+    const int c = t + m * T;
     double d, b;
-    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, x, yv[0], z0, z1, d, b);
-    if (cross) a.mxdb[static_cast<size_t>(c) * n + r] = cd{d, b};
+    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, c, a.dx), y, z0, z1, d, b);
+    if (pending) {
+      a.mxdb[static_cast<size_t>(c) * N + r] = cd{d, b};
+      pending &= pending - 1;
+    }
   }
-  if (r < n) a.mcode[static_cast<size_t>(c) * n + r] = static_cast<uint16_t>(e1[0] | (cross ? MAP_CROSSING : 0u));
 }
 
 // ---------------------------------------------------------------------------
@@ -221,7 +238,8 @@ __global__ __launch_bounds__(MAP_ROWS) void k_material_map(FftArgs<R> a, int n)
 // consecutive rows of a column.  FFT_x, multiply by Kx(c) Ky(r), IFFT_x.  For
 // N <= 512 the first exchange of FFT_x moves every line into one wavefront
 // (mapping B) and the last exchange of IFFT_x moves it back: the exchanges in
-// between need no work-group barrier.
+// between need no work-group barrier.  With a CSG material the field-0
+// work-groups then build the material map of the step for their rows.
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 {
@@ -254,6 +272,10 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
     fft_line<N, +1>(v, lla, ta, twa);
   }
   store8(v, f, static_cast<size_t>(T) * N);
+  if (a.mat.mode == MAT_CSG && b == 0) {
+    __shared__ uint64_t mask[MAX_MASK_WORDS];
+    material_map_rows<N, R, L>(a, mask, ra, ta);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,13 +426,6 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 }
 
 template <class R>
-hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_material_map<R>, dim3((n + MAP_ROWS - 1) / MAP_ROWS, n), dim3(MAP_ROWS), 0, s, a, n);
-  return hipGetLastError();
-}
-
-template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
   // one field per line; 4 lines per work-group when the batch divides evenly
@@ -434,8 +449,6 @@ template hipError_t launch_pass_y<float>(int, int, const FftArgs<float> &, int, 
 template hipError_t launch_pass_y<double>(int, int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
-template hipError_t launch_material_map<float>(int, const FftArgs<float> &, hipStream_t);
-template hipError_t launch_material_map<double>(int, const FftArgs<double> &, hipStream_t);
 template hipError_t launch_fft2d_step<float>(int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_fft2d_step<double>(int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_cube_store<float>(const cx<float> *, size_t, int, int, cx<double> *,

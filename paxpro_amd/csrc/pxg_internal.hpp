@@ -29,7 +29,8 @@ struct FftArgs {
   int mode;                // pass-Y mode bits (Y_INV | Y_FWD)
   cx<R> *sol;              // FFT2D: solution columns (Nx x Nz per field) or null
   size_t sol_fstride;
-  // FFT3D CSG material map of the current z-step, shared by the batch:
+  // FFT3D CSG material map of the current z-step, shared by the batch (built
+  // by pass X, read by pass Y):
   uint16_t *mcode;         // [N*N] deciding part at z1; bit 15: border crossing
   cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
 };
@@ -45,8 +46,6 @@ template <class R>
 hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
-template <class R>
-hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>

@@ -118,13 +118,13 @@ __device__ __forceinline__ int csg_eval(const MatDev &m, const uint64_t *mask, i
   return m.n_parts;
 }
 
-// csg_eval for K points (x, y[k], za) and (x, y[k], zb): each culled part and
-// each of its shapes is loaded once (scalar loads, the loop is uniform) and
+// csg_eval for K points (x[k], y[k], za) and (x[k], y[k], zb): each culled part
+// and each of its shapes is loaded once (scalar loads, the loop is uniform) and
 // tested against all 2K points.  ea/eb receive the indices csg_eval returns.
 template <int K>
-__device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mask, int nw, double x,
-                                            const double (&y)[K], double za, double zb, int (&ea)[K],
-                                            int (&eb)[K])
+__device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mask, int nw,
+                                            const double (&x)[K], const double (&y)[K], double za,
+                                            double zb, int (&ea)[K], int (&eb)[K])
 {
 #pragma unroll
   for (int k = 0; k < K; ++k) ea[k] = eb[k] = m.n_parts;
@@ -146,8 +146,8 @@ __device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mas
         const int verdict = s.op == 0 ? 1 : 2;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          if (sa[k] == 0 && shape_inside(s, x, y[k], za)) sa[k] = verdict;
-          if (sb[k] == 0 && shape_inside(s, x, y[k], zb)) sb[k] = verdict;
+          if (sa[k] == 0 && shape_inside(s, x[k], y[k], za)) sa[k] = verdict;
+          if (sb[k] == 0 && shape_inside(s, x[k], y[k], zb)) sb[k] = verdict;
         }
       }
 #pragma unroll
