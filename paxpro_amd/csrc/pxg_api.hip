@@ -372,6 +372,8 @@ int begin(pxg_plan P)
   return ev0;
 }
 
+constexpr int MAP_IN_PASS_X_BATCH = 8;  // fields from which pass X builds the map
+
 template <class R>
 FftArgs<R> fft_args(pxg_plan P)
 {
@@ -397,6 +399,9 @@ FftArgs<R> fft_args(pxg_plan P)
   a.quirks = p.quirks;
   a.mcode = P->mcode.as<uint16_t>();
   a.mxdb = P->mxdb.as<cd>();
+  // the map is computed once per step for the batch: inside pass X when the
+  // batch gives it many work-groups to spread over, else in its own launch
+  a.map_in_pass_x = p.batch >= MAP_IN_PASS_X_BATCH;
   return a;
 }
 
@@ -432,7 +437,12 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
   for (unsigned st = first; st < first + nsteps; ++st) {
     const bool last = st + 1 == first + nsteps;
     const bool store = store_step(P, st);
-    a.step = st;  // pass X also builds this step's material map (CSG)
+    a.step = st;  // the material map of this step (CSG) is built before pass Y
+    if (P->mat_mode == MAT_CSG && !a.map_in_pass_x) {
+      ev0 = begin(P);
+      record(P, "material_map", static_cast<double>(P->cells) * sizeof(uint16_t),
+             launch_material_map<R>(n, a, P->stream), ev0);
+    }
     ev0 = begin(P);
     record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
     a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
@@ -471,8 +481,13 @@ double time_pass(pxg_plan P, bool pass_x, int lines, int reps)
     return pass_x ? launch_pass_x<R>(p.nx, lines, a, p.batch, P->stream)
                   : launch_pass_y<R>(p.nx, lines, a, p.batch, P->stream);
   };
-  // pass Y reads the material map of the step, built by pass X
-  if (!pass_x) hip_check(launch_pass_x<R>(p.nx, P->lines_x, a, p.batch, P->stream), "pass x");
+  // pass Y reads the material map of the step
+  if (!pass_x && P->mat_mode == MAT_CSG) {
+    if (a.map_in_pass_x)
+      hip_check(launch_pass_x<R>(p.nx, P->lines_x, a, p.batch, P->stream), "pass x");
+    else
+      hip_check(launch_material_map<R>(p.nx, a, P->stream), "material map");
+  }
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");

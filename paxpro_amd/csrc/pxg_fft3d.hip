@@ -172,7 +172,8 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 // ---------------------------------------------------------------------------
 // Material map of z-step `a.step` (src/fftSolver3D.cpp:122-156), computed once
 // for every field of the batch by the field-0 work-groups of pass X, each for
-// its band of L rows: per cell a 16-bit code holding the deciding part at z1,
+// its band of L rows (or by k_material_map for plans with few fields): per
+// cell a 16-bit code holding the deciding part at z1,
 // with MAP_CROSSING set when the material differs between z0 and z1, and for
 // those border cells the crossing integral's (delta, beta) in mxdb.  Pass Y of
 // the same step reads it.  Lane (row r, t) covers columns t + m*N/8.
@@ -232,6 +233,16 @@ __device__ __forceinline__ void material_map_rows(const FftArgs<R> &a, uint64_t 
   }
 }
 
+// The same map in a launch of its own, for plans with few fields (inside pass
+// X the field-0 work-groups would carry all of it).  L rows per work-group.
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_material_map(FftArgs<R> a)
+{
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  const int tid = threadIdx.x;
+  material_map_rows<N, R, L>(a, mask, blockIdx.x * L + tid % L, tid / L);
+}
+
 // ---------------------------------------------------------------------------
 // Pass X: strided x-lines (matrix rows).  Global loads and stores run with the
 // lines fastest across lanes (mapping A), so that each instruction touches L
@@ -272,7 +283,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
     fft_line<N, +1>(v, lla, ta, twa);
   }
   store8(v, f, static_cast<size_t>(T) * N);
-  if (a.mat.mode == MAT_CSG && b == 0) {
+  if (a.map_in_pass_x && a.mat.mode == MAT_CSG && b == 0) {
     __shared__ uint64_t mask[MAX_MASK_WORDS];
     material_map_rows<N, R, L>(a, mask, ra, ta);
   }
@@ -426,6 +437,16 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 }
 
 template <class R>
+hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
+{
+  // about 256 lanes per work-group
+  int lines = 256 / (n / 8);
+  lines = lines < 1 ? 1 : (lines > 16 ? 16 : lines);
+  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, 1); };
+  PXG_SIZE_SWITCH(k_material_map)
+}
+
+template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
   // one field per line; 4 lines per work-group when the batch divides evenly
@@ -449,6 +470,8 @@ template hipError_t launch_pass_y<float>(int, int, const FftArgs<float> &, int, 
 template hipError_t launch_pass_y<double>(int, int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
+template hipError_t launch_material_map<float>(int, const FftArgs<float> &, hipStream_t);
+template hipError_t launch_material_map<double>(int, const FftArgs<double> &, hipStream_t);
 template hipError_t launch_fft2d_step<float>(int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_fft2d_step<double>(int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_cube_store<float>(const cx<float> *, size_t, int, int, cx<double> *,

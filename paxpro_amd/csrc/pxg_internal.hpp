@@ -33,6 +33,7 @@ struct FftArgs {
   // by pass X, read by pass Y):
   uint16_t *mcode;         // [N*N] deciding part at z1; bit 15: border crossing
   cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
+  int map_in_pass_x;       // build the map inside pass X (large batches) or in its own launch
 };
 
 // Bit of a material-map code marking a border cell (material differs between
@@ -46,6 +47,8 @@ template <class R>
 hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>

@@ -238,3 +238,31 @@ def test_context_outlived_by_its_plans():
     assert rel_l2(plan.get_field(), O.fft3d_run(g, O.NoMaterial, u0, 1, 4)) < 1e-12
     plan.close()
     assert h is not None
+
+
+@pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_MIXED])
+def test_fft3d_batch_map_built_in_pass_x(ctx, prec):
+    """From 8 fields on the material map is built inside pass X instead of its
+    own launch: a batch of 8 over 24 spheres with border crossings gives, field
+    by field, the same bits as single-field plans (whose map has its own
+    launch), and the FP64 result matches the oracle."""
+    n, nsteps = 128, 10
+    announce("fft3d batch 8, map in pass X", n, nsteps, "precision", prec)
+    g = fft3d_case(n, dz=200.0, nsteps=nsteps)
+    csg = spheres_scene(random_spheres(0x5EED0002, 24, (-8000, 8000), (-1500, 1500), (500, 3000), (1e-6, 1e-5)))
+    ks = [K_01 * (1 + 0.01 * b) for b in range(8)]
+    u0 = gaussian_field(g, n, 5000.0)
+    plan = P.Plan(ctx, fft3d_problem(g, n, nsteps + 1, precision=prec, batch=8))
+    plan.set_wavenumbers(ks)
+    plan.material_csg(*csg)
+    for b in range(8):
+        plan.set_field(u0, b)
+    plan.step(1, nsteps)
+    for b in (0, 5):
+        gb = fft3d_case(n, dz=200.0, nsteps=nsteps, k=ks[b])
+        _, (single,) = _run_gpu(ctx, fft3d_problem(gb, n, nsteps + 1, precision=prec), u0, nsteps, csg)
+        assert np.array_equal(plan.get_field(b), single), b
+        if prec == P.PREC_FP64:
+            ref = O.fft3d_run(gb, O.CSGMaterial(*csg), u0, 1, nsteps)
+            assert rel_l2(single, ref) < 1e-11
+    plan.close()
