@@ -171,35 +171,36 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 
 // ---------------------------------------------------------------------------
 // Material map of z-step `a.step` (src/fftSolver3D.cpp:122-156), computed once
-// for every field of the batch by the field-0 work-groups of pass X, each for
-// its band of L rows (or by k_material_map for plans with few fields): per
-// cell a 16-bit code holding the deciding part at z1,
-// with MAP_CROSSING set when the material differs between z0 and z1, and for
-// those border cells the crossing integral's (delta, beta) in mxdb.  Pass Y of
-// the same step reads it.  Lane (row r, t) covers columns t + m*N/8.
-template <int N, class R, int L>
-__device__ __forceinline__ void material_map_rows(const FftArgs<R> &a, uint64_t *mask, int r, int t)
+// for every field of the batch: per cell a 16-bit code holding the deciding
+// part at z1, with MAP_CROSSING set when the material differs between z0 and
+// z1, and for those border cells the crossing integral's (delta, beta) in
+// mxdb.  Pass Y of the same step reads it.  A work-group covers the cells of
+// the block [xlo, xhi] x [ylo, yhi] (column and row indices, parts culled
+// against it); lane cell m is cell(m) = (column, row), m < CPL.  The field-0
+// work-groups of pass X build it for their rows; k_material_map does it in a
+// launch of its own for plans with few fields.
+template <int CPL, class R, class CellOf>
+__device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t *mask, int n, int xlo, int xhi,
+                                                   int ylo, int yhi, CellOf cell)
 {
-  constexpr int T = N / 8;
-  constexpr int K = 2;
+  constexpr int K = CPL < 2 ? CPL : 2;
   const double z1 = coord(a.zmin, a.step, a.dz);
   const double z0 = coord(a.zmin, a.step - 1, a.dz);
-  const int r0 = blockIdx.x * L;
   int nw = (a.mat.n_parts + 63) >> 6;
-  csg_cull(a.mat, mask, nw, a.xmin, coord(a.xmin, N - 1, a.dx), coord(a.ymin, r0, a.dy),
-           coord(a.ymin, r0 + L - 1, a.dy), fmin(z0, z1), fmax(z0, z1));
+  csg_cull(a.mat, mask, nw, coord(a.xmin, xlo, a.dx), coord(a.xmin, xhi, a.dx), coord(a.ymin, ylo, a.dy),
+           coord(a.ymin, yhi, a.dy), fmin(z0, z1), fmax(z0, z1));
   bool any = false;
   for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
-  if (!any || a.mat.is_ref) nw = 0;  // no part near this band: every cell is vacuum
-  const double y = coord(a.ymin, r, a.dy);
+  if (!any || a.mat.is_ref) nw = 0;  // no part near this block: every cell is vacuum
   unsigned crossing = 0;
 #pragma unroll 1
-  for (int m0 = 0; m0 < 8; m0 += K) {
+  for (int m0 = 0; m0 < CPL; m0 += K) {
     double xv[K], yv[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      xv[q] = coord(a.xmin, t + (m0 + q) * T, a.dx);
-      yv[q] = y;
+      const int2 cr = cell(m0 + q);
+      xv[q] = coord(a.xmin, cr.x, a.dx);
+      yv[q] = coord(a.ymin, cr.y, a.dy);
     }
     int e1[K], e0[K];
     csg_eval_x2<K>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
@@ -215,32 +216,36 @@ __device__ __forceinline__ void material_map_rows(const FftArgs<R> &a, uint64_t 
           crossing |= 1u << (m0 + q);
         }
       }
-      a.mcode[static_cast<size_t>(t + (m0 + q) * T) * N + r] = static_cast<uint16_t>(code);
+      const int2 cr = cell(m0 + q);
+      a.mcode[static_cast<size_t>(cr.x) * n + cr.y] = static_cast<uint16_t>(code);
     }
   }
   // border cells: the whole wavefront runs the loop while any lane has one
   // left, so that the CSG walk stays in uniform control flow
   unsigned pending = crossing;
   while (__ballot(pending != 0) != 0) {
-    const int m = pending ? __builtin_ctz(pending) : 0;
-    const int c = t + m * T;
+    const int2 cr = cell(pending ? __builtin_ctz(pending) : 0);
     double d, b;
-    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, c, a.dx), y, z0, z1, d, b);
+    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, cr.x, a.dx), coord(a.ymin, cr.y, a.dy),
+                      z0, z1, d, b);
     if (pending) {
-      a.mxdb[static_cast<size_t>(c) * N + r] = cd{d, b};
+      a.mxdb[static_cast<size_t>(cr.x) * n + cr.y] = cd{d, b};
       pending &= pending - 1;
     }
   }
 }
 
 // The same map in a launch of its own, for plans with few fields (inside pass
-// X the field-0 work-groups would carry all of it).  L rows per work-group.
-template <int N, class R, int L>
-__global__ __launch_bounds__(L *N / 8) void k_material_map(FftArgs<R> a)
+// X the field-0 work-groups would carry all of it): one cell per lane, a
+// work-group covers up to 256 rows of one column.
+template <class R>
+__global__ __launch_bounds__(256) void k_material_map(FftArgs<R> a, int n)
 {
   __shared__ uint64_t mask[MAX_MASK_WORDS];
-  const int tid = threadIdx.x;
-  material_map_rows<N, R, L>(a, mask, blockIdx.x * L + tid % L, tid / L);
+  const int c = blockIdx.y, r0 = blockIdx.x * blockDim.x;
+  const int r = r0 + static_cast<int>(threadIdx.x);
+  material_map_cells<1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
+                        [&](int) { return make_int2(c, r); });
 }
 
 // ---------------------------------------------------------------------------
@@ -285,7 +290,9 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   store8(v, f, static_cast<size_t>(T) * N);
   if (a.map_in_pass_x && a.mat.mode == MAT_CSG && b == 0) {
     __shared__ uint64_t mask[MAX_MASK_WORDS];
-    material_map_rows<N, R, L>(a, mask, ra, ta);
+    const int r0 = blockIdx.x * L;
+    material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
+                          [&](int m) { return make_int2(ta + m * T, ra); });
   }
 }
 
@@ -439,11 +446,9 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 {
-  // about 256 lanes per work-group
-  int lines = 256 / (n / 8);
-  lines = lines < 1 ? 1 : (lines > 16 ? 16 : lines);
-  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, 1); };
-  PXG_SIZE_SWITCH(k_material_map)
+  const int rows = n < 256 ? n : 256;  // n is a power of two >= 16
+  hipLaunchKernelGGL(k_material_map<R>, dim3(n / rows, n), dim3(rows), 0, s, a, n);
+  return hipGetLastError();
 }
 
 template <class R>
