@@ -5,7 +5,7 @@
 // CN: one work-group owns one field for the whole run (a persistent march over
 // all z-steps, no launch per step).  Each step builds the complex tridiagonal
 // system row by row in FP64 and solves it with a partitioned method: P lanes
-// each eliminate a contiguous chunk of m rows so that every interior unknown is
+// each eliminate a contiguous chunk of about n/P rows so that every interior unknown is
 // expressed through the chunk's first and last unknowns; those 2P unknowns form
 // a tridiagonal system solved by parallel cyclic reduction in LDS; the lanes
 // then recover their interiors.  The arithmetic is Gaussian elimination without
@@ -24,16 +24,14 @@ namespace pxg {
 constexpr int CN_MAX_P = 512;
 constexpr int TRI_MASK_WORDS = 16;
 
+// P lanes (a power of two, at most CN_MAX_P and n/4) own balanced chunks: the
+// first n mod P chunks have n/P + 1 rows, the others n/P (at least 4), so
+// every chunk has distinct first and last rows.  m is the largest chunk.
 CnPartition cn_partition(int n)
 {
   int P = 1;
   while (P * 2 <= CN_MAX_P && P * 2 <= n / 4) P *= 2;
-  int m = (n + P - 1) / P;
-  while (P > 1 && (P - 1) * m > n - 2) {
-    P /= 2;
-    m = (n + P - 1) / P;
-  }
-  return {P, m};
+  return {P, (n + P - 1) / P};
 }
 
 __host__ __device__ static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -171,8 +169,10 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
   const double k = J.k;
   const double rho = J.dz / (k * J.dx * J.dx);
   const double r = k * J.dz;
-  const int s = p * m;
-  const int e = active ? min(N, s + m) - 1 : s - 1;
+  // balanced chunk [s, e] of lane p (cn_partition)
+  const int base = N / P, rem = N % P;
+  const int s = p * base + min(p, rem);
+  const int e = active ? s + base + (p < rem ? 1 : 0) - 1 : s - 1;
   const int M = 2 * P;
   const int nw = J.mat.mode == MAT_CSG ? (J.mat.n_parts + 63) >> 6 : 0;
 
