@@ -170,9 +170,9 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
   const double rho = J.dz / (k * J.dx * J.dx);
   const double r = k * J.dz;
   // balanced chunk [s, e] of lane p (cn_partition)
-  const int base = N / P, rem = N % P;
-  const int s = p * base + min(p, rem);
-  const int e = active ? s + base + (p < rem ? 1 : 0) - 1 : s - 1;
+  const int len0 = N / P, rem = N % P;
+  const int s = p * len0 + min(p, rem);
+  const int e = active ? s + len0 + (p < rem ? 1 : 0) - 1 : s - 1;
   const int M = 2 * P;
   const int nw = J.mat.mode == MAT_CSG ? (J.mat.n_parts + 63) >> 6 : 0;
 
