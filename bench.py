@@ -158,7 +158,29 @@ def cpu_baseline(target_s):
     return {"value": N * N * steps / dt / 1e6, "unit": "Mcell*steps/s", "cores": O.num_threads(),
             "kind": "port",
             "sample": f"oracle FFTSolver3D restatement, 512x512 sphere workload, {steps} z-steps "
-                      f"(of 1000) in {dt:.1f} s, FP64, self-written radix-2 FFT (not FFTW)"}
+                      f"(of 1000) in {dt:.1f} s, FP64, self-written radix-2 FFT (not FFTW)",
+            "fft_calibration": fft_calibration(O, u0)}
+
+
+def fft_calibration(O, a, reps=5):
+    """The oracle's self-written 2D FFT beside pocketfft (numpy, one thread;
+    scipy with one and with all workers) on the same 512^2 complex128 field, in
+    ms per 2D transform: how far the CPU restatement is from a tuned FFT."""
+    import scipy.fft
+
+    def best(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return round(min(ts) * 1e3, 3)
+
+    return {"oracle_fft2d_ms": best(lambda: O.fft2d_colmajor(a)),
+            "numpy_pocketfft_1thread_ms": best(lambda: np.fft.fft2(a)),
+            "scipy_pocketfft_1worker_ms": best(lambda: scipy.fft.fft2(a, workers=1)),
+            "scipy_pocketfft_all_workers_ms": best(lambda: scipy.fft.fft2(a, workers=-1)),
+            "workers": os.cpu_count()}
 
 
 def _oracle():
