@@ -37,23 +37,38 @@ __device__ __noinline__ cd screen_from_values(double k, double dz, double delta,
 // Border-crossing integral of FFTSolver3D::refractionIntegral
 // (src/fftSolver3D.cpp:253-273): f(z0) + f(z1) + 2 sum_{i=1}^{n-2} f(z0 + i dz/n),
 // divided by 2n; the reference skips sample n-1 (PXG_FIX_REFR_SAMPLES restores it).
+// The points (z0, z1, then the interior samples, summed in that order) are
+// classified CK at a time in one walk over the culled parts each.
 __device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_t *mask, int nw,
                                                   unsigned n_refr, unsigned quirks, double x,
                                                   double y, double za, double zb, double &delta,
                                                   double &beta)
 {
-  double dt, bt;
-  part_values(m, csg_eval(m, mask, nw, x, y, za), delta, beta);
-  part_values(m, csg_eval(m, mask, nw, x, y, zb), dt, bt);
-  delta += dt;
-  beta += bt;
+  constexpr int CK = 4;
   const double h = (zb - za) / n_refr;
   const unsigned last = (quirks & 2u) ? n_refr : n_refr - 1;
-  for (unsigned i = 1; i < last; ++i) {
-    const double z = za + i * h;
-    part_values(m, csg_eval(m, mask, nw, x, y, z), dt, bt);
-    delta += 2.0 * dt;
-    beta += 2.0 * bt;
+  const unsigned npts = last + 1;  // z0, z1 and samples 1 .. last-1
+  delta = 0.0;
+  beta = 0.0;
+  for (unsigned j0 = 0; j0 < npts; j0 += CK) {
+    double zs[CK];
+#pragma unroll
+    for (int q = 0; q < CK; ++q) {
+      const unsigned j = j0 + q;
+      zs[q] = j == 0 ? za : (j == 1 ? zb : za + (j - 1) * h);
+    }
+    int es[CK];
+    csg_eval_column<CK>(m, mask, nw, x, y, zs, es);
+#pragma unroll
+    for (int q = 0; q < CK; ++q) {
+      const unsigned j = j0 + q;
+      if (j >= npts) break;
+      double dt, bt;
+      part_values(m, es[q], dt, bt);
+      const double wgt = j < 2 ? 1.0 : 2.0;
+      delta += wgt * dt;
+      beta += wgt * bt;
+    }
   }
   delta /= (2.0 * n_refr);
   beta /= (2.0 * n_refr);

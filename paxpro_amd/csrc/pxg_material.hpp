@@ -159,6 +159,41 @@ __device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mas
   }
 }
 
+// csg_eval for K points on one (x, y) column at heights z[k], in one walk over
+// the culled parts (each part's shapes loaded once for all K points).
+template <int K>
+__device__ __forceinline__ void csg_eval_column(const MatDev &m, const uint64_t *mask, int nw, double x, double y,
+                                                const double (&z)[K], int (&e)[K])
+{
+#pragma unroll
+  for (int k = 0; k < K; ++k) e[k] = m.n_parts;
+  if (m.is_ref) return;
+  for (int w = 0; w < nw; ++w) {
+    const uint64_t raw = mask[w];
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(raw));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(raw >> 32));
+    uint64_t bits = (static_cast<uint64_t>(hi) << 32) | lo;
+    while (bits) {
+      const int p = w * 64 + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const DPart part = m.parts[p];
+      int s[K];  // 0 undecided, 1 inside (union shape), 2 outside (difference)
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k] = 0;
+      for (int i = part.n_shapes - 1; i >= 0; --i) {
+        const DShape sh = m.shapes[part.first_shape + i];
+        const int verdict = sh.op == 0 ? 1 : 2;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (s[k] == 0 && shape_inside(sh, x, y, z[k])) s[k] = verdict;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (e[k] == m.n_parts && s[k] == 1) e[k] = p;
+    }
+  }
+}
+
 __device__ __forceinline__ void part_values(const MatDev &m, int e, double &d, double &b)
 {
   if (e >= m.n_parts || m.parts[e].op != 0) {
