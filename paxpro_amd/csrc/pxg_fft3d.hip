@@ -279,7 +279,14 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   const int tid = threadIdx.x;
   const int la = tid % L, ta = tid / L;  // mapping A
   const int b = blockIdx.y;
-  const int ra = blockIdx.x * L + la;
+  // Row group of this work-group.  Work-groups are dealt to the 8 XCDs in
+  // turn; the groups that share a 128-byte line of a column (128 / (L s)
+  // neighbours) are given to work-groups of one XCD, so that one L2 serves
+  // them.  Every group is still covered exactly once.
+  constexpr int G = N / L;
+  const int bx = blockIdx.x;
+  const int g = (G % 8 == 0) ? (bx % 8) * (G / 8) + bx / 8 : bx;
+  const int ra = g * L + la;
   cx<R> *f = a.field + b * a.fstride + ra + static_cast<size_t>(ta) * N;
   cd *lla = lds + la * LdsLine<N>::STRIDE;
   cd v[8];
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
     cd *llb = lds + lb * LdsLine<N>::STRIDE;
     const TwBase<N, double> twa = tw_base<N>(a.tw, ta), twb = tw_base<N>(a.tw, tb);
     fft_line_remap_in<N, -1>(v, lla, llb, ta, tb, twb);
-    const cd kyr = a.ky[b * N + blockIdx.x * L + lb];
+    const cd kyr = a.ky[b * N + g * L + lb];
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = v[m] * (kx[tb + m * T] * kyr);
     fft_line_remap_out<N, +1>(v, lla, llb, ta, tb, twa, twb);
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   store8(v, f, static_cast<size_t>(T) * N);
   if (a.map_in_pass_x && a.mat.mode == MAT_CSG && b == 0) {
     __shared__ uint64_t mask[MAX_MASK_WORDS];
-    const int r0 = blockIdx.x * L;
+    const int r0 = g * L;
     material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
                           [&](int m) { return make_int2(ta + m * T, ra); });
   }
