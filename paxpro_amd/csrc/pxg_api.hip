@@ -123,6 +123,7 @@ struct pxg_plan_s {
   int lines_x = 1, lines_y = 1;
   DBuf field, tw, kx, ky, absx, absy, kvec;
   DBuf mcode, mxdb;  // FFT3D: material map of the current z-step (CSG materials)
+  long map_step = -1;  // step whose map mcode holds (-1: none)
   std::vector<double> k_host;
   // material
   int mat_mode = MAT_NONE;
@@ -442,6 +443,8 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
     const bool last = st + 1 == first + nsteps;
     const bool store = store_step(P, st);
     a.step = st;  // the material map of this step (CSG) is built before pass Y
+    a.map_prev_valid = P->map_step >= 0 && P->map_step + 1 == static_cast<long>(st);
+    P->map_step = -1;  // until this step's map is launched
     if (P->mat_mode == MAT_CSG && !a.map_in_pass_x) {
       ev0 = begin(P);
       record(P, "material_map", static_cast<double>(P->cells) * sizeof(uint16_t),
@@ -449,6 +452,7 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
     }
     ev0 = begin(P);
     record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
+    if (P->mat_mode == MAT_CSG) P->map_step = st;
     a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
     ev0 = begin(P);
     record(P, "pass_y", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
@@ -1015,6 +1019,7 @@ int pxg_plan_reset(pxg_plan P, const pxg_problem *prob)
     if (P->p.n_refr == 0) P->p.n_refr = 10;
     P->k_host.assign(prob->batch, prob->k);
     P->mat_mode = MAT_NONE;
+    P->map_step = -1;
     P->field_set = false;
     P->jobs_dirty = true;
     rebuild_tables(P);
@@ -1189,6 +1194,7 @@ int pxg_set_material_none(pxg_plan P)
   return guarded([&] {
     check_plan(P);
     P->mat_mode = MAT_NONE;
+    P->map_step = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1251,6 +1257,7 @@ int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, cons
     P->n_parts = static_cast<int>(order.size());
     P->is_ref = is_reference_run;
     P->mat_mode = MAT_CSG;
+    P->map_step = -1;
     P->jobs_dirty = true;
     if (P->p.kind == PXG_FFT3D) build_part_factors(P);
   });
@@ -1266,6 +1273,7 @@ int pxg_set_material_curved_wg(pxg_plan P, const pxg_curved_wg *wg)
     P->wg = DCurved{wg->R, wg->width, wg->clad_delta, wg->clad_beta, wg->in_delta, wg->in_beta,
                     wg->wglength, wg->dx, wg->dz, wg->use_smoothed};
     P->mat_mode = MAT_CURVED;
+    P->map_step = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1283,6 +1291,7 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
     P->first_plane = first_plane;
     P->z_invariant = z_invariant;
     P->mat_mode = MAT_PLANES;
+    P->map_step = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1334,6 +1343,7 @@ int pxg_plan_autotune(pxg_plan P, int reps)
     hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
     hip_check(hipStreamSynchronize(P->stream), "sync");
     P->field_set = false;
+    P->map_step = -1;  // the timing runs wrote maps of another step
     std::lock_guard<std::mutex> lk(P->ctx->mu);
     P->ctx->tuned[key] = {P->lines_x, P->lines_y};
   });

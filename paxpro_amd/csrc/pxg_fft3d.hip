@@ -218,7 +218,17 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
       yv[q] = coord(a.ymin, cr.y, a.dy);
     }
     int e1[K], e0[K];
-    csg_eval_x2<K>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
+    if (a.map_prev_valid) {
+      // the z0 plane is the previous step's z1 plane: its parts are in the map
+      csg_eval_x2<K, false>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int2 cr = cell(m0 + q);
+        e0[q] = a.mcode[static_cast<size_t>(cr.x) * n + cr.y] & (MAP_CROSSING - 1u);
+      }
+    } else {
+      csg_eval_x2<K>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
+    }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       unsigned code = static_cast<unsigned>(e1[q]);

@@ -34,6 +34,7 @@ struct FftArgs {
   uint16_t *mcode;         // [N*N] deciding part at z1; bit 15: border crossing
   cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
   int map_in_pass_x;       // build the map inside pass X (large batches) or in its own launch
+  int map_prev_valid;      // mcode holds the map of step - 1: reuse its parts as the z0 plane
 };
 
 // Bit of a material-map code marking a border cell (material differs between

@@ -121,7 +121,8 @@ __device__ __forceinline__ int csg_eval(const MatDev &m, const uint64_t *mask, i
 // csg_eval for K points (x[k], y[k], za) and (x[k], y[k], zb): each culled part
 // and each of its shapes is loaded once (scalar loads, the loop is uniform) and
 // tested against all 2K points.  ea/eb receive the indices csg_eval returns.
-template <int K>
+// With ZB = false only the za points are classified (eb left at n_parts).
+template <int K, bool ZB = true>
 __device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mask, int nw,
                                             const double (&x)[K], const double (&y)[K], double za,
                                             double zb, int (&ea)[K], int (&eb)[K])
@@ -147,7 +148,7 @@ __device__ __forceinline__ void csg_eval_x2(const MatDev &m, const uint64_t *mas
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (sa[k] == 0 && shape_inside(s, x[k], y[k], za)) sa[k] = verdict;
-          if (sb[k] == 0 && shape_inside(s, x[k], y[k], zb)) sb[k] = verdict;
+          if (ZB && sb[k] == 0 && shape_inside(s, x[k], y[k], zb)) sb[k] = verdict;
         }
       }
 #pragma unroll
