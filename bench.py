@@ -119,20 +119,22 @@ def copy_calibration(mib, iters=20):
                       "read_plus_write_GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
 
 
-def measured_traffic(kernel, batch):
-    """HBM bytes per launch of `kernel` (the plan's record name) on this
-    workload, from the committed rocprofv3 summary: FETCH_SIZE and WRITE_SIZE
-    per dispatch, each scaled by that counter's reading for a plain copy of
-    known size.  None when no summary covers this kernel and batch."""
+def measured_traffic(kernel, batch, lines):
+    """HBM bytes per launch of `kernel` (the plan's record name) with `lines`
+    lines per work-group on this workload, from the committed rocprofv3
+    summary: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by that
+    counter's reading for a plain copy of known size.  None when no summary
+    covers this kernel, shape and batch."""
     try:
         with open(TRAFFIC_SUMMARY) as fh:
-            lines = fh.read().splitlines()
+            text = fh.read().splitlines()
     except OSError:
         return None
-    for line in lines:
-        if line.startswith("{"):
-            d = json.loads(line)
-            if d.get("kernel") == kernel and d.get("batch") == batch and d.get("grid") == N:
+    for row in text:
+        if row.startswith("{"):
+            d = json.loads(row)
+            if (d.get("kernel") == kernel and d.get("batch") == batch and d.get("grid") == N
+                    and d.get("lines") == lines):
                 return d["traffic_bytes_per_launch"]
     return None
 
@@ -517,7 +519,8 @@ def main():
     achieved = dom[1]["bytes_per_launch"] / (dom[1]["mean_ms"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(dom[0], B),
+            "traffic": measured_traffic(dom[0], B, lines[0] if dom[0] == "pass_x" else lines[1]),
+            "lines": lines[0] if dom[0] == "pass_x" else lines[1],
             "alg_bytes_per_launch": dom[1]["bytes_per_launch"],
             "mean_launch_us": round(dom[1]["mean_ms"] * 1e3, 2)}
     pm.close()
