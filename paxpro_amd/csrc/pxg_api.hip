@@ -146,7 +146,12 @@ struct pxg_plan_s {
   std::vector<size_t> fcells;       // cells of each field
   std::vector<int> job_mat;         // per-job material mode (job plans)
   std::vector<DCurved> job_wg;
-  DBuf jobs_dev;
+  DBuf jobs_dev;      // CN jobs marched by k_cn_jobs (chunks above cn_reg_rows() rows)
+  DBuf jobs_reg_dev;  // CN jobs marched by k_cn_reg
+  int njobs_glob = 0, njobs_reg = 0, max_P_reg = 1;
+  // second stream: the two CN job sets march concurrently
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool jobs_dirty = true;
   int max_P = 1;
   size_t sol_total = 0;             // elements of solution storage
@@ -358,22 +363,24 @@ MatDev mat_dev(pxg_plan P)
 }
 
 // ------------------------------------------------------------------ launches
-void record(pxg_plan P, const char *name, double bytes, hipError_t e, int ev0)
+// Profiling records: begin() before a launch, record() after it, on the
+// stream the kernel is launched on (the plan's stream unless given).
+void record(pxg_plan P, const char *name, double bytes, hipError_t e, int ev0, hipStream_t s = nullptr)
 {
   hip_check(e, name);
   if (P->profiling) {
     const int ev1 = static_cast<int>(P->recs.size()) * 2 + 1;
-    hip_check(hipEventRecord(P->ev[ev1], P->stream), "event");
+    hip_check(hipEventRecord(P->ev[ev1], s ? s : P->stream), "event");
     P->recs.push_back({name, ev0, ev1, bytes});
   }
 }
 
-int begin(pxg_plan P)
+int begin(pxg_plan P, hipStream_t s = nullptr)
 {
   if (!P->profiling) return -1;
   const int ev0 = static_cast<int>(P->recs.size()) * 2;
   if (static_cast<size_t>(ev0 + 2) > P->ev.size()) fail(PXG_ERR_STATE, "event pool exhausted");
-  hip_check(hipEventRecord(P->ev[ev0], P->stream), "event");
+  hip_check(hipEventRecord(P->ev[ev0], s ? s : P->stream), "event");
   return ev0;
 }
 
@@ -610,9 +617,31 @@ void build_cn_jobs(pxg_plan P)
     P->max_P = std::max(P->max_P, cp.P);
   }
   if (scoff > P->scratch.bytes) fail(PXG_ERR_STATE, "CN scratch too small (%zu < %zu)", P->scratch.bytes, scoff);
-  P->jobs_dev.alloc(jobs.size() * sizeof(CnJob));
-  hip_check(hipMemcpyAsync(P->jobs_dev.p, jobs.data(), P->jobs_dev.bytes, hipMemcpyHostToDevice, P->stream),
-            "upload CN jobs");
+  // jobs whose chunks fit the register-resident march, and the others
+  std::vector<CnJob> reg, glob;
+  P->max_P = P->max_P_reg = 1;
+  for (const CnJob &J : jobs) {
+    if (J.m <= cn_reg_rows()) {
+      reg.push_back(J);
+      P->max_P_reg = std::max(P->max_P_reg, J.P);
+    } else {
+      glob.push_back(J);
+      P->max_P = std::max(P->max_P, J.P);
+    }
+  }
+  P->njobs_reg = static_cast<int>(reg.size());
+  P->njobs_glob = static_cast<int>(glob.size());
+  if (!glob.empty()) {
+    P->jobs_dev.alloc(glob.size() * sizeof(CnJob));
+    hip_check(hipMemcpyAsync(P->jobs_dev.p, glob.data(), P->jobs_dev.bytes, hipMemcpyHostToDevice, P->stream),
+              "upload CN jobs");
+  }
+  if (!reg.empty()) {
+    P->jobs_reg_dev.alloc(reg.size() * sizeof(CnJob));
+    hip_check(hipMemcpyAsync(P->jobs_reg_dev.p, reg.data(), P->jobs_reg_dev.bytes, hipMemcpyHostToDevice,
+                             P->stream),
+              "upload CN jobs");
+  }
   hip_check(hipStreamSynchronize(P->stream), "sync");
   P->jobs_dirty = false;
 }
@@ -624,12 +653,37 @@ void run_cn(pxg_plan P, unsigned first, unsigned nsteps)
   size_t cells = 0;
   for (size_t c : P->fcells) cells += c;
   const double bytes = 2.0 * cells * sizeof(cx<R>) * nsteps;
-  const int njobs = static_cast<int>(P->fcells.size());
-  const int ev0 = begin(P);
-  record(P, "cn_march", bytes,
-         launch_cn_jobs<R>(P->jobs_dev.as<CnJob>(), njobs, P->max_P, P->field.as<cx<R>>(), P->scratch.p,
-                           P->sol.as<cx<R>>(), first, nsteps, P->stream),
-         ev0);
+  // the jobs with long chunks march on the second stream, concurrently with
+  // the register-resident ones on the plan's stream
+  hipStream_t gs = P->stream;
+  if (P->njobs_glob && P->njobs_reg) {
+    if (!P->aux) {
+      hip_check(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming), "event");
+      hip_check(hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming), "event");
+    }
+    hip_check(hipEventRecord(P->fork_ev, P->stream), "fork");
+    hip_check(hipStreamWaitEvent(P->aux, P->fork_ev, 0), "fork");
+    gs = P->aux;
+  }
+  if (P->njobs_glob) {
+    const int ev0 = begin(P, gs);
+    record(P, "cn_march", bytes,
+           launch_cn_jobs<R>(P->jobs_dev.as<CnJob>(), P->njobs_glob, P->max_P, P->field.as<cx<R>>(),
+                             P->scratch.p, P->sol.as<cx<R>>(), first, nsteps, gs),
+           ev0, gs);
+  }
+  if (P->njobs_reg) {
+    const int ev0 = begin(P);
+    record(P, "cn_march_reg", bytes,
+           launch_cn_reg<R>(P->jobs_reg_dev.as<CnJob>(), P->njobs_reg, P->max_P_reg, P->field.as<cx<R>>(),
+                            P->sol.as<cx<R>>(), first, nsteps, P->stream),
+           ev0);
+  }
+  if (gs != P->stream) {
+    hip_check(hipEventRecord(P->join_ev, gs), "join");
+    hip_check(hipStreamWaitEvent(P->stream, P->join_ev, 0), "join");
+  }
 }
 
 template <class R>
@@ -1067,6 +1121,12 @@ int pxg_plan_destroy(pxg_plan P)
       if (P->stage[i]) (void)hipHostFree(P->stage[i]);
       if (P->stage_ev[i]) (void)hipEventDestroy(P->stage_ev[i]);
     }
+    if (P->aux) {
+      (void)hipStreamSynchronize(P->aux);
+      (void)hipStreamDestroy(P->aux);
+    }
+    if (P->fork_ev) (void)hipEventDestroy(P->fork_ev);
+    if (P->join_ev) (void)hipEventDestroy(P->join_ev);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     pxg_ctx c = P->ctx;
     bool last = false;

@@ -262,9 +262,9 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
 
 // The same map in a launch of its own, for plans with few fields (inside pass
 // X the field-0 work-groups would carry all of it): one cell per lane, a
-// work-group covers up to 256 rows of one column.
+// work-group covers up to 64 rows of one column.
 template <class R>
-__global__ __launch_bounds__(256) void k_material_map(FftArgs<R> a, int n)
+__global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
 {
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int c = blockIdx.y, r0 = blockIdx.x * blockDim.x;
@@ -478,7 +478,9 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 {
-  const int rows = n < 256 ? n : 256;  // n is a power of two >= 16
+  // one wavefront per work-group: short column strips keep the culled part
+  // lists (and the CSG walks of border cells) short
+  const int rows = n < 64 ? n : 64;  // n is a power of two >= 16
   hipLaunchKernelGGL(k_material_map<R>, dim3(n / rows, n), dim3(rows), 0, s, a, n);
   return hipGetLastError();
 }

@@ -22,6 +22,7 @@
 namespace pxg {
 
 constexpr int CN_MAX_P = 512;
+constexpr int CN_REG_ROWS = 4;  // chunk rows held in registers by k_cn_reg
 constexpr int TRI_MASK_WORDS = 16;
 
 // P lanes (a power of two, at most CN_MAX_P and n/4) own balanced chunks: the
@@ -59,8 +60,10 @@ __device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) { return 
 // Re(kd) clamped at 0 (src/crancNicholson.cpp:131-141,
 // src/alternatingDirectionSolver.cpp:157-234).  Returns false when
 // |u_in| <= thresh, in which case the reference leaves the row unchanged.
-__device__ __forceinline__ bool tbc_factor(cx<double> u_out, cx<double> u_in, double thresh,
-                                           cx<double> &e)
+// Out of line (like absorb_factor): boundary rows only, and inlined into the
+// unrolled row loops its log/atan2/exp/sincos constants would be hoisted into
+// registers that count against the whole kernel.
+__device__ __noinline__ bool tbc_factor(cx<double> u_out, cx<double> u_in, double thresh, cx<double> &e)
 {
   if (!(sqrt(norm2(u_in)) > thresh)) return false;
   const cx<double> ratio = cdiv(u_out, u_in);
@@ -104,11 +107,12 @@ __device__ __forceinline__ cx<double> ld(const cx<R> *f, int i)
 // diag = 1 + i rho/2 + (r/2)(beta + i delta), off-diagonals -i rho/4,
 // rhs = i rho/4 (u_{i-1} + u_{i+1}) - i rho/2 u_i + (1 - (r/2)(beta0 + i delta0)) u_i,
 // transparent boundary rows at i = 0 and N-1 (src/crancNicholson.cpp:30-89,107-148).
-template <class R>
-__device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i, unsigned st,
-                                       double z, double rho, double r, const uint64_t *mask, int nw,
-                                       cx<double> &ai, cx<double> &bi, cx<double> &ci,
-                                       cx<double> &di)
+// ul, u, ur are the previous field at i-1, i, i+1 (zero outside); uin is the
+// inner neighbour of a boundary row (i = 1 or N-2), used only at i = 0, N-1.
+__device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, double z, double rho, double r,
+                                         const uint64_t *mask, int nw, cx<double> ul, cx<double> u,
+                                         cx<double> ur, cx<double> uin, cx<double> &ai, cx<double> &bi,
+                                         cx<double> &ci, cx<double> &di)
 {
   const int N = J.nx;
   const double x = coord(J.xmin, i, J.dx);
@@ -119,17 +123,13 @@ __device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i,
   ai = i > 0 ? off : cx<double>{0.0, 0.0};
   ci = i < N - 1 ? off : cx<double>{0.0, 0.0};
   bi = {1.0 + 0.5 * (b1 * r), 0.5 * rho + 0.5 * (d1 * r)};
-  const cx<double> u = ld(prev, i);
-  const cx<double> ul = i > 0 ? ld(prev, i - 1) : cx<double>{0.0, 0.0};
-  const cx<double> ur = i < N - 1 ? ld(prev, i + 1) : cx<double>{0.0, 0.0};
   cx<double> v = (ul + ur) * cx<double>{0.0, 0.25 * rho};
   v = v - cx<double>{-0.5 * rho * u.y, 0.5 * rho * u.x};
   v = v + cx<double>{1.0 - 0.5 * (b0 * r), -0.5 * (d0 * r)} * u;
   di = v;
   if (J.bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
-    const int inner = i == 0 ? 1 : N - 2;
     cx<double> e;
-    if (tbc_factor(u, ld(prev, inner), 1e-16, e)) {
+    if (tbc_factor(u, uin, 1e-16, e)) {
       const cx<double> q = cx<double>{0.0, 0.25 * rho} * e;
       bi = bi - q;
       di = di + q * u;
@@ -137,13 +137,76 @@ __device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i,
   }
 }
 
-__device__ __forceinline__ double absorb_factor(int i, int N, unsigned npix, double inv)
+template <class R>
+__device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i, unsigned st,
+                                       double z, double rho, double r, const uint64_t *mask, int nw,
+                                       cx<double> &ai, cx<double> &bi, cx<double> &ci,
+                                       cx<double> &di)
+{
+  const int N = J.nx;
+  const cx<double> u = ld(prev, i);
+  const cx<double> ul = i > 0 ? ld(prev, i - 1) : cx<double>{0.0, 0.0};
+  const cx<double> ur = i < N - 1 ? ld(prev, i + 1) : cx<double>{0.0, 0.0};
+  const cx<double> uin = (i == 0 && N >= 2) ? ld(prev, 1) : ((i == N - 1 && N >= 2) ? ld(prev, N - 2) : u);
+  cn_row_v(J, i, st, z, rho, r, mask, nw, ul, u, ur, uin, ai, bi, ci, di);
+}
+
+__device__ __noinline__ double absorb_factor(int i, int N, unsigned npix, double inv)
 {
   double f = 1.0;
   if (i < static_cast<int>(npix)) f *= exp(-static_cast<double>(npix - i - 1) * inv);
   const int j = N - 1 - i;
   if (j < static_cast<int>(npix)) f *= exp(-static_cast<double>(npix - j - 1) * inv);
   return f;
+}
+
+// Parallel cyclic reduction of the M-row tridiagonal system (ra, rb, rc, rd)
+// in LDS; lane p owns rows 2p and 2p+1.  On return row i reads rb[i] x_i = rd[i].
+// Every lane of the work-group calls it (work-group barriers).
+__device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rb, cx<double> *rc, cx<double> *rd, int M,
+                                       int p, bool active)
+{
+  for (int h = 1; h < M; h *= 2) {
+    cx<double> na[2], nb[2], nc[2], nd[2];
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = 2 * p + q;
+        cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
+        cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
+        cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
+        cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
+        if (row - h >= 0) {
+          k1 = cdiv(a_, rb[row - h]);
+          am = ra[row - h];
+          cm = rc[row - h];
+          dm = rd[row - h];
+        }
+        if (row + h < M) {
+          k2 = cdiv(c_, rb[row + h]);
+          ap = ra[row + h];
+          cp = rc[row + h];
+          dp = rd[row + h];
+        }
+        na[q] = cx<double>{0.0, 0.0} - am * k1;
+        nc[q] = cx<double>{0.0, 0.0} - cp * k2;
+        nb[q] = b_ - cm * k1 - ap * k2;
+        nd[q] = d_ - dm * k1 - dp * k2;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = 2 * p + q;
+        ra[row] = na[q];
+        rb[row] = nb[q];
+        rc[row] = nc[q];
+        rd[row] = nd[q];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- CN march
@@ -233,48 +296,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
     }
     __syncthreads();
 
-    // parallel cyclic reduction of the 2P-row reduced system
-    for (int h = 1; h < M; h *= 2) {
-      cx<double> na[2], nb[2], nc[2], nd[2];
-      if (active) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int row = 2 * p + q;
-          cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
-          cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
-          cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
-          cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
-          if (row - h >= 0) {
-            k1 = cdiv(a_, rb[row - h]);
-            am = ra[row - h];
-            cm = rc[row - h];
-            dm = rd[row - h];
-          }
-          if (row + h < M) {
-            k2 = cdiv(c_, rb[row + h]);
-            ap = ra[row + h];
-            cp = rc[row + h];
-            dp = rd[row + h];
-          }
-          na[q] = cx<double>{0.0, 0.0} - am * k1;
-          nc[q] = cx<double>{0.0, 0.0} - cp * k2;
-          nb[q] = b_ - cm * k1 - ap * k2;
-          nd[q] = d_ - dm * k1 - dp * k2;
-        }
-      }
-      __syncthreads();
-      if (active) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int row = 2 * p + q;
-          ra[row] = na[q];
-          rb[row] = nb[q];
-          rc[row] = nc[q];
-          rd[row] = nd[q];
-        }
-      }
-      __syncthreads();
-    }
+    cn_pcr(ra, rb, rc, rd, M, p, active);
     if (active) {
       const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
       const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
@@ -301,9 +323,160 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
     for (int i = s; i <= e; ++i) bufA[i] = bufB[i];
 }
 
+// CN march with the field in registers.  For jobs whose chunks have at most
+// MM rows (N <= MM * CN_MAX_P): lane p keeps its rows of the field and its
+// forward-elimination coefficients in registers for the whole run; only the
+// chunk-edge values of the field (for the neighbouring rows) and the reduced
+// system go through LDS, and the field goes back to memory once, at the end.
+// The elimination is the same as in k_cn_jobs (Gaussian elimination without
+// pivoting over the chunk, PCR over the 2P chunk ends); the interior is
+// recovered by back substitution x_i = dh_i - ah_i x_s - ch_i x_{i+1}.
+template <class R, int MM>
+__global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ jobs, cx<R> *fields, cx<R> *sol,
+                                                     unsigned first, unsigned nsteps)
+{
+  __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
+  __shared__ uint64_t mask[TRI_MASK_WORDS];
+  const CnJob &J = jobs[blockIdx.x];
+  const int p = threadIdx.x, N = J.nx, P = J.P;
+  const bool active = p < P;
+  cx<R> *fld = fields + J.foff;
+  const double k = J.k;
+  const double rho = J.dz / (k * J.dx * J.dx);
+  const double r = k * J.dz;
+  const int len0 = N / P, rem = N % P;
+  const int s = p * len0 + min(p, rem);
+  const int len = active ? len0 + (p < rem ? 1 : 0) : 0;  // 2 <= len <= MM for active lanes
+  const int M = 2 * P;
+  const int nw = J.mat.mode == MAT_CSG ? (J.mat.n_parts + 63) >> 6 : 0;
+  const cx<double> zero{0.0, 0.0};
+
+  cx<double> u[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) u[j] = j < len ? cvt<double>(fld[s + j]) : zero;
+
+  for (unsigned it = 0; it < nsteps; ++it) {
+    const unsigned st = first + it;
+    const double z = coord(J.zmin, st, J.dz);
+    if (J.mat.mode == MAT_CSG)
+      csg_cull(J.mat, mask, nw, J.xmin, coord(J.xmin, N - 1, J.dx), 0.0, 0.0, z - J.dz, z);
+    // chunk-edge values for the neighbouring lanes
+    cx<double> ulast = u[0], u2last = u[0];
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+      if (j == len - 1) ulast = u[j];
+      if (j == len - 2) u2last = u[j];
+    }
+    if (active) {
+      edge_lo[p] = u[0];
+      edge_hi[p] = ulast;
+    }
+    __syncthreads();
+    const cx<double> ul = (active && p > 0) ? edge_hi[p - 1] : zero;
+    const cx<double> ur = (active && p < P - 1) ? edge_lo[p + 1] : zero;
+
+    cx<double> ah[MM], ch[MM], dh[MM];
+    if (active) {
+      // first row of the chunk
+      cx<double> as, bs, cs, ds;
+      cn_row_v(J, s, st, z, rho, r, mask, nw, ul, u[0], u[1], u[1], as, bs, cs, ds);
+      // forward elimination of rows s+1 .. s+len-1
+#pragma unroll
+      for (int j = 1; j < MM; ++j) {
+        if (j < len) {
+          const int i = s + j;
+          const cx<double> un = (j + 1 < MM && j + 1 < len) ? u[j + 1 < MM ? j + 1 : 0] : ur;
+          const cx<double> uin = (i == N - 1) ? u2last : u[j];
+          cx<double> ai, bi, ci, di;
+          cn_row_v(J, i, st, z, rho, r, mask, nw, u[j - 1], u[j], un, uin, ai, bi, ci, di);
+          if (j == 1) {
+            ah[j] = cdiv(ai, bi);
+            ch[j] = cdiv(ci, bi);
+            dh[j] = cdiv(di, bi);
+          } else {
+            const cx<double> den = bi - ai * ch[j - 1];
+            ah[j] = cdiv(zero - ai * ah[j - 1], den);
+            ch[j] = cdiv(ci, den);
+            dh[j] = cdiv(di - ai * dh[j - 1], den);
+          }
+        }
+      }
+      // the chunk's last row is a reduced equation: ah x_s + x_e + ch x_{e+1} = dh
+      cx<double> ahl = zero, chl = zero, dhl = zero;
+#pragma unroll
+      for (int j = 1; j < MM; ++j)
+        if (j == len - 1) {
+          ahl = ah[j];
+          chl = ch[j];
+          dhl = dh[j];
+        }
+      ra[2 * p + 1] = ahl;
+      rb[2 * p + 1] = {1.0, 0.0};
+      rc[2 * p + 1] = chl;
+      rd[2 * p + 1] = dhl;
+      // backward: row s+1 through x_s and x_e (running values, base (0, 0, -1) at e)
+      cx<double> at = zero, ct{-1.0, 0.0}, dt = zero;
+#pragma unroll
+      for (int j = MM - 2; j >= 1; --j) {
+        if (j <= len - 2) {
+          dt = dh[j] - ch[j] * dt;
+          at = ah[j] - ch[j] * at;
+          ct = zero - ch[j] * ct;
+        }
+      }
+      ra[2 * p] = as;
+      rb[2 * p] = bs - cs * at;
+      rc[2 * p] = zero - cs * ct;
+      rd[2 * p] = ds - cs * dt;
+    }
+    __syncthreads();
+    cn_pcr(ra, rb, rc, rd, M, p, active);
+    if (active) {
+      const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
+      const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
+      // back substitution from the chunk's end
+      cx<double> xn = xe;
+#pragma unroll
+      for (int j = MM - 1; j >= 1; --j) {
+        if (j == len - 1) {
+          u[j] = xe;
+        } else if (j < len - 1) {
+          u[j] = dh[j] - ah[j] * xs - ch[j] * xn;
+          xn = u[j];
+        }
+      }
+      u[0] = xs;
+      cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
+                      ? sol + J.soff + static_cast<size_t>(st) * N
+                      : nullptr;
+#pragma unroll
+      for (int j = 0; j < MM; ++j) {
+        if (j < len) {
+          if (J.absorb_npix) u[j] = scale(u[j], absorb_factor(s + j, N, J.absorb_npix, J.absorb_inv));
+          if (sc) sc[s + j] = cvt<R>(u[j]);
+        }
+      }
+    }
+    __syncthreads();  // edge arrays and the reduced system are reused next step
+  }
+#pragma unroll
+  for (int j = 0; j < MM; ++j)
+    if (j < len) fld[s + j] = cvt<R>(u[j]);
+}
+
 size_t cn_job_scratch(int nx, int esz)
 {
   return cn_field_scratch(nx, esz, cn_partition(nx));
+}
+
+template <class R>
+hipError_t launch_cn_reg(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
+                         unsigned first, unsigned nsteps, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_cn_reg<R, CN_REG_ROWS>), dim3(njobs), dim3(max_P), 0, s, jobs_dev, fields, sol, first,
+                     nsteps);
+  return hipGetLastError();
 }
 
 template <class R>
@@ -516,6 +689,11 @@ hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
   return hipGetLastError();
 }
 
+int cn_reg_rows() { return CN_REG_ROWS; }
+template hipError_t launch_cn_reg<float>(const CnJob *, int, int, cx<float> *, cx<float> *, unsigned, unsigned,
+                                         hipStream_t);
+template hipError_t launch_cn_reg<double>(const CnJob *, int, int, cx<double> *, cx<double> *, unsigned,
+                                          unsigned, hipStream_t);
 template hipError_t launch_cn_jobs<float>(const CnJob *, int, int, cx<float> *, void *, cx<float> *,
                                           unsigned, unsigned, hipStream_t);
 template hipError_t launch_cn_jobs<double>(const CnJob *, int, int, cx<double> *, void *, cx<double> *,

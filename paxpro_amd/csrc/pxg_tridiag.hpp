@@ -49,6 +49,12 @@ size_t cn_job_scratch(int nx, int esz);
 template <class R>
 hipError_t launch_cn_jobs(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch,
                           cx<R> *sol, unsigned first, unsigned nsteps, hipStream_t s);
+// Register-resident march for jobs whose largest chunk has at most
+// cn_reg_rows() rows (CnJob.m); needs no scratch.
+template <class R>
+hipError_t launch_cn_reg(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
+                         unsigned first, unsigned nsteps, hipStream_t s);
+int cn_reg_rows();
 template <class R>
 hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
 // Tile-Jacobi ADI half-step for diagonally dominant systems (pxg_adi.hip).
