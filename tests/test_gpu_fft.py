@@ -266,3 +266,20 @@ def test_fft3d_batch_map_built_in_pass_x(ctx, prec):
             ref = O.fft3d_run(gb, O.CSGMaterial(*csg), u0, 1, nsteps)
             assert rel_l2(single, ref) < 1e-11
     plan.close()
+
+
+@pytest.mark.parametrize("n,nsteps", [(16, 6), (4096, 2)])
+def test_fft3d_smallest_and_largest_grid(ctx, n, nsteps):
+    """The smallest (16^2) and largest (4096^2) supported grids: FP64 against
+    the oracle, mixed against FP64."""
+    announce("fft3d edge size", n, nsteps)
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7), (2500.0, -1500.0, 0.0, 1200.0, 8e-6, 2e-7)])
+    ref = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps)
+    _, (out,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1), u0, nsteps, csg)
+    err = rel_l2(out, ref)
+    announce("rel_l2", err)
+    assert err < 1e-11
+    _, (mix,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1, precision=P.PREC_MIXED), u0, nsteps, csg)
+    assert rel_l2(mix, out) < 1e-5
