@@ -1,42 +1,88 @@
 // ADI half-step for the diagonally dominant regime: each tridiagonal line is
-// solved by a converged Jacobi (Neumann-series) iteration inside a tile held in
-// LDS, instead of a sequential sweep per line.
+// solved by a converged Jacobi (Neumann-series) iteration over a tile of the
+// line held in registers, instead of a sequential sweep per line.
 //
 // Every row of an ADI half-step system (src/alternatingDirectionSolver.cpp:8-134)
 // has Re(diag) >= 1/h, h = dz/2: the material terms k*beta and the transparent
 // boundary terms only add non-negative real parts (Im e >= 0 because Re(kd) is
 // clamped at 0).  The two off-diagonals are -i/(2 k d^2), so the Jacobi
 // iteration x <- D^-1 (b - O x) contracts by rho = 2|off|/|diag| <= h/(k d^2)
-// per sweep.  With rho <= 5e-3 (config 2: rho = 4.2e-4) K sweeps give an error
-// below rho^K relative, and K is chosen so that rho^K < 1e-17: the result is the
-// exact solution to FP64 rounding, like the reference's Thomas sweep.  A tile
-// covers 8 lines x 128 positions; the 8 positions at each tile edge are halo,
-// recomputed by the neighbouring tile, and K <= 8 keeps truncation at the tile
-// edge below the same bound.  Less dominant systems use the sequential kernel.
+// per sweep.  With rho <= 5e-3 (the 2048^2 / 20 nm grid: rho = 4.2e-4) K sweeps
+// give an error below rho^K relative, and K is chosen so that rho^K < 1e-17: the
+// result is the exact solution to FP64 rounding, like the reference's Thomas
+// sweep.  Less dominant systems use the sequential kernel.
 //
-// Lanes map to cells so that global accesses coalesce: along y (contiguous) in
-// the y-implicit pass, across the 8 consecutive lines (rows) in the x-implicit
-// pass.
+// Tiles.  A work-group holds TL lines x TP positions; each lane holds ADI_C
+// consecutive positions of one line in registers.  The K-sweep iterate at a
+// position depends only on the K positions either side, so the K positions at
+// each tile edge are a halo, recomputed by the neighbouring tile, and every
+// other position comes out exactly as from an untiled sweep.  A sweep updates
+// a lane's positions from its own registers; only the first and last positions
+// cross lanes, through LDS slots indexed by lane (conflict-free).
+//  - y-implicit lines are contiguous in memory: one wavefront per line tile
+//    (TL = 4, TP = 256), so the exchange needs no work-group barrier.
+//  - x-implicit lines run across memory (stride N): 16 lines side by side make
+//    every load 16 consecutive elements (TL = 16, TP = 128, a barrier per sweep).
+// Work-groups of neighbouring line groups run on one XCD, where the explicit
+// direction's neighbour lines they share are still in that XCD's L2.
+#include <type_traits>
+
 #include "pxg_tridiag.hpp"
 
 namespace pxg {
 
-constexpr int ADI_LINES = 8;
-constexpr int ADI_TILE = 128;  // positions per line held by a work-group
-constexpr int ADI_HALO = 8;    // recomputed positions at each tile edge
+constexpr int ADI_C = 4;            // consecutive positions per lane
+constexpr int ADI_HALO_MAX = 8;     // largest sweep count (and halo) of the tile path
 constexpr int ADI_MASK_WORDS = 16;
 
+template <bool IMPX>
+struct AdiTile {
+  static constexpr int TL = IMPX ? 16 : 4;    // lines per tile
+  static constexpr int TP = IMPX ? 128 : 256; // positions per line
+  static constexpr int G = TP / ADI_C;        // lanes per line
+  static constexpr int LANES = TL * G;
+  static constexpr int DG = IMPX ? TL : 1;    // lane distance of the next position group
+  static constexpr bool WAVE_LOCAL = !IMPX;   // a line tile is one wavefront
+};
+static_assert(AdiTile<false>::G == 64, "y-implicit line tile must be one wavefront");
+
+// Transparent boundary factor of the reference, e = exp(i kd) with
+// kd = -i log(r), r = u_out / u_in, and Re(kd) = arg(r) clamped at 0
+// (src/alternatingDirectionSolver.cpp:166-230).  exp(i kd) = |r| e^{i arg r} is
+// r itself when arg r >= 0, and |r| when arg r < 0 (the clamp removes the
+// phase), so no logarithm or exponential is needed: arg r < 0 exactly when
+// atan2(Im r, Re r) < 0, i.e. Im r < 0, or Im r = -0 with Re r < 0.  Returns
+// false when |u_in| <= 1e-6 (the reference then leaves the row unchanged).
 __device__ __forceinline__ bool tbc_e(cx<double> u_out, cx<double> u_in, cx<double> &e)
 {
-  if (!(sqrt(norm2(u_in)) > 1e-6)) return false;
-  const cx<double> ratio = cdiv(u_out, u_in);
-  double kr = atan2(ratio.y, ratio.x);
-  const double amp = hypot(ratio.x, ratio.y);
-  if (kr < 0.0) kr = 0.0;
-  double s, c;
-  sincos(kr, &s, &c);
-  e = {amp * c, amp * s};
+  const double n2 = norm2(u_in);
+  if (!(sqrt(n2) > 1e-6)) return false;
+  const double in2 = 1.0 / n2;
+  const cx<double> r = {(u_out.x * u_in.x + u_out.y * u_in.y) * in2, (u_out.y * u_in.x - u_out.x * u_in.y) * in2};
+  const bool clamp = r.y < 0.0 || (r.y == 0.0 && signbit(r.y) && r.x < 0.0);
+  e = clamp ? cx<double>{sqrt(norm2(r)), 0.0} : r;
   return true;
+}
+
+// Transparent boundary at the ends of the implicit line: diag correction from
+// the cell's value and its inward neighbour along the line.
+__device__ __forceinline__ cx<double> tbc_diag(cx<double> diag, cx<double> u, cx<double> uin, double dimp)
+{
+  cx<double> e;
+  if (tbc_e(u, uin, e)) diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
+  return diag;
+}
+
+// Transparent boundary at the first and last line (explicit direction): rhs
+// correction from the cell's value and its neighbour in the next line inward.
+__device__ __forceinline__ cx<double> tbc_rhs(cx<double> rhs, cx<double> u, cx<double> uin, double dexp)
+{
+  cx<double> e;
+  if (tbc_e(u, uin, e)) {
+    const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
+    rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
+  }
+  return rhs;
 }
 
 template <class R>
@@ -45,32 +91,63 @@ __device__ __forceinline__ cx<double> ldd(const cx<R> *f, size_t i)
   return cvt<double>(f[i]);
 }
 
-template <class R, bool IMPX>
-__global__ __launch_bounds__(ADI_LINES *ADI_TILE) void k_adi_jacobi(TriArgs<R> a, const cx<R> *__restrict__ prev,
-                                                                   cx<R> *__restrict__ next, int iters)
+// Launch constants of one half-step (one field): computed once on the host.
+struct AdiCoef {
+  double k;           // wavenumber
+  double z;           // height of the material evaluation
+  double ih;          // 1/h, h = dz/2
+  double dimp, dexp;  // k d^2 of the implicit and explicit directions
+  double idimp, iexp; // their inverses
+  double hexp;        // 0.5 / dexp
+  double o;           // off-diagonal i*o, o = -0.5 / dimp
+  double ddy;         // k dy^2 (y-implicit diagonal of the reference)
+};
+
+// 1/d for a finite positive normal d: the hardware reciprocal estimate refined
+// by two Newton-Raphson steps r <- r + r (1 - d r), each of which squares the
+// relative error; the result is within an ulp or two of the rounded quotient.
+__device__ __forceinline__ double recip(double d)
 {
-  __shared__ cx<double> xs[2][ADI_LINES][ADI_TILE];
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Linear work-group index -> tile index such that the work-groups the hardware
+// places on one XCD (index mod 8) take a contiguous range of tiles.
+__device__ __forceinline__ int xcd_tile(int b, int total)
+{
+  const int q = total >> 3, r = total & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
+template <class R, bool IMPX>
+__global__ __launch_bounds__(AdiTile<IMPX>::LANES) void k_adi_jacobi(TriArgs<R> a, AdiCoef c,
+                                                                    const cx<R> *__restrict__ prev,
+                                                                    cx<R> *__restrict__ next, int iters, int nseg)
+{
+  using T = AdiTile<IMPX>;
+  constexpr int C = ADI_C;
+  __shared__ cx<double> edge[2][2][T::LANES];  // [buffer][first, last position][lane]
   __shared__ uint64_t mask[ADI_MASK_WORDS];
   const int N = a.n;
   const int tid = threadIdx.x;
-  // lane -> (line l, tile position t); coalesced direction fastest
-  const int l = IMPX ? tid % ADI_LINES : tid / ADI_TILE;
-  const int t = IMPX ? tid / ADI_LINES : tid % ADI_TILE;
-  const int line = blockIdx.y * ADI_LINES + l;
-  const int j0 = blockIdx.x * (ADI_TILE - 2 * ADI_HALO) - ADI_HALO;
-  const int j = j0 + t;                      // position along the line
-  const bool live = line < N && j >= 0 && j < N;
-  const double k = a.kvec[0];
-  const double h = a.dz / 2.0;
+  const int l = IMPX ? tid % T::TL : tid / T::G;
+  const int g = IMPX ? tid / T::TL : tid % T::G;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int seg = tile % nseg, grp = tile / nseg;
+  const int halo = iters;
+  const int j0 = seg * (T::TP - 2 * halo) - halo;  // position of tile slot 0
+  const int jb = j0 + g * C;                        // this lane's first position
+  const int line = grp * T::TL + l;
+  const double k = c.k, z = c.z;
   const unsigned st = a.first_step;
-  const double z = IMPX ? coord(a.zmin, st, a.dz) : coord(a.zmin, st, a.dz) + h;
-  const double ddx = k * a.dx * a.dx, ddy = k * a.dy * a.dy;
-  const double dimp = IMPX ? ddx : ddy, dexp = IMPX ? ddy : ddx;
-  const cx<double> off = {0.0, -0.5 / dimp};
   const int nw = a.mat.mode == MAT_CSG ? (a.mat.n_parts + 63) >> 6 : 0;
   if (a.mat.mode == MAT_CSG) {
-    const int l0 = blockIdx.y * ADI_LINES, l1 = min(N - 1, l0 + ADI_LINES - 1);
-    const int p0 = max(0, j0), p1 = min(N - 1, j0 + ADI_TILE - 1);
+    const int l0 = grp * T::TL, l1 = min(N - 1, l0 + T::TL - 1);
+    const int p0 = max(0, j0), p1 = min(N - 1, j0 + T::TP - 1);
     if (IMPX)
       csg_cull(a.mat, mask, nw, coord(a.xmin, p0, a.dx), coord(a.xmin, p1, a.dx), coord(a.ymin, l0, a.dy),
                coord(a.ymin, l1, a.dy), z, z);
@@ -78,99 +155,178 @@ __global__ __launch_bounds__(ADI_LINES *ADI_TILE) void k_adi_jacobi(TriArgs<R> a
       csg_cull(a.mat, mask, nw, coord(a.xmin, l0, a.dx), coord(a.xmin, l1, a.dx), coord(a.ymin, p0, a.dy),
                coord(a.ymin, p1, a.dy), z, z);
   }
-  // element index of (line + dl, position jj)
-  auto at = [&](int jj, int dl) -> size_t {
-    const int ln = line + dl;
-    return IMPX ? static_cast<size_t>(jj) * N + ln : static_cast<size_t>(ln) * N + jj;
+  // Clamped cell indices: every load below stays inside the field, and cells
+  // outside the grid (dead) are masked when the coefficients are formed.  The
+  // field loads follow the material evaluation: issued before it, they held
+  // 24-48 registers through the CSG walk and cut occupancy to 3-4 waves.
+  const int ln = min(line, N - 1), lm = max(ln - 1, 0), lp = min(ln + 1, N - 1);
+  auto at = [&](int jj, int li) -> size_t {
+    return IMPX ? static_cast<size_t>(jj) * N + li : static_cast<size_t>(li) * N + jj;
   };
-  cx<double> inv{0.0, 0.0}, rhs{0.0, 0.0}, x{0.0, 0.0};
-  if (live) {
-    double d = 0.0, be = 0.0;
-    if (a.mat.mode == MAT_CSG) {
-      const double xx = IMPX ? coord(a.xmin, j, a.dx) : coord(a.xmin, line, a.dx);
-      const double yy = IMPX ? coord(a.ymin, line, a.dy) : coord(a.ymin, j, a.dy);
-      part_values(a.mat, csg_eval(a.mat, mask, nw, xx, yy, z), d, be);
-    } else if (a.mat.mode == MAT_PLANES) {
-      const int pi = a.mat.z_invariant ? 0 : static_cast<int>(2 * st + (IMPX ? 0 : 1)) - a.mat.first_plane;
-      const double *pl = a.mat.planes + static_cast<size_t>(pi) * a.mat.plane_stride;
-      const size_t idx = at(j, 0);
-      d = pl[2 * idx];
-      be = pl[2 * idx + 1];
-    }
-    cx<double> diag;
-    if (IMPX || (a.quirks & PXG_FIX_ADI_Y_DIAG))
-      diag = {1.0 / h + be * k, 1.0 / dimp + d * k};
-    else
-      diag = cx<double>{1.0 / h, 0.0} + cdiv(cx<double>{0.0, 1.0}, cx<double>{ddy + be * k, d * k});
-    const cx<double> u = ldd(prev, at(j, 0));
-    rhs = cx<double>{1.0 / h, -1.0 / dexp} * u;
-    if (line > 0) {
-      const cx<double> un = ldd(prev, at(j, -1));
-      rhs = rhs + cx<double>{-0.5 * un.y / dexp, 0.5 * un.x / dexp};
-    }
-    if (line < N - 1) {
-      const cx<double> un = ldd(prev, at(j, +1));
-      rhs = rhs + cx<double>{-0.5 * un.y / dexp, 0.5 * un.x / dexp};
-    }
-    if (a.use_tbc) {
-      cx<double> e;
-      if (j == N - 1 && tbc_e(u, ldd(prev, at(N - 2, 0)), e))
-        diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
-      if (j == 0 && tbc_e(u, ldd(prev, at(1, 0)), e))
-        diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
-      if (line == N - 1 && tbc_e(u, ldd(prev, at(j, -1)), e)) {
-        const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
-        rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
-      }
-      if (line == 0 && tbc_e(u, ldd(prev, at(j, +1)), e)) {
-        const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
-        rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
-      }
-    }
-    inv = cdiv(cx<double>{1.0, 0.0}, diag);
-    x = rhs * inv;
+  bool live[C];
+  int jc[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    live[m] = line < N && jb + m >= 0 && jb + m < N;
+    jc[m] = min(max(jb + m, 0), N - 1);
   }
-  xs[0][l][t] = x;
-  __syncthreads();
-  int cur = 0;
-  for (int it = 0; it < iters; ++it) {
-    cx<double> xl{0.0, 0.0}, xr{0.0, 0.0};
-    if (t > 0 && j - 1 >= 0) xl = xs[cur][l][t - 1];
-    if (t < ADI_TILE - 1 && j + 1 < N) xr = xs[cur][l][t + 1];
-    const cx<double> xn = live ? (rhs - off * (xl + xr)) * inv : cx<double>{0.0, 0.0};
-    xs[cur ^ 1][l][t] = xn;
-    cur ^= 1;
-    __syncthreads();
+  // material of the lane's cells
+  double dd[C], bb[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) dd[m] = bb[m] = 0.0;
+  if (a.mat.mode == MAT_CSG) {
+    double xs[C], ys[C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+      xs[m] = IMPX ? coord(a.xmin, jc[m], a.dx) : coord(a.xmin, ln, a.dx);
+      ys[m] = IMPX ? coord(a.ymin, ln, a.dy) : coord(a.ymin, jc[m], a.dy);
+    }
+    int e[C], eb[C];
+    csg_eval_x2<C, false>(a.mat, mask, nw, xs, ys, z, z, e, eb);
+#pragma unroll
+    for (int m = 0; m < C; ++m) part_values(a.mat, e[m], dd[m], bb[m]);
+  } else if (a.mat.mode == MAT_PLANES) {
+    const int pi = a.mat.z_invariant ? 0 : static_cast<int>(2 * st + (IMPX ? 0 : 1)) - a.mat.first_plane;
+    const double *pl = a.mat.planes + static_cast<size_t>(pi) * a.mat.plane_stride;
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+      const size_t idx = at(jc[m], ln);
+      dd[m] = pl[2 * idx];
+      bb[m] = pl[2 * idx + 1];
+    }
   }
-  if (live && t >= ADI_HALO && t < ADI_TILE - ADI_HALO) next[at(j, 0)] = cvt<R>(xs[cur][l][t]);
+
+  cx<R> ru[C], rm[C], rp[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    ru[m] = prev[at(jc[m], ln)];
+    rm[m] = prev[at(jc[m], lm)];
+    rp[m] = prev[at(jc[m], lp)];
+  }
+
+  // Rows of the lane's cells: diag and rhs (src/alternatingDirectionSolver.cpp:8-134)
+  cx<double> dg[C], rh[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    if (IMPX || (a.quirks & PXG_FIX_ADI_Y_DIAG)) {
+      dg[m] = {c.ih + bb[m] * k, c.idimp + dd[m] * k};
+    } else {
+      // 1/h + i / (k dy^2 + k beta + i k delta)
+      const double p = c.ddy + bb[m] * k, q = dd[m] * k, r = recip(p * p + q * q);
+      dg[m] = {c.ih + q * r, p * r};
+    }
+    const cx<double> u = cvt<double>(ru[m]);
+    cx<double> un{0.0, 0.0};
+    if (line > 0) un = cvt<double>(rm[m]);
+    if (line < N - 1) un = un + cvt<double>(rp[m]);
+    rh[m] = cx<double>{c.ih, -c.iexp} * u + cx<double>{-un.y * c.hexp, un.x * c.hexp};
+  }
+  // Transparent boundary rows, after the loaded values are consumed: the few
+  // lanes holding boundary cells read their neighbours again.
+  if (a.use_tbc) {
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+      const int j = jb + m;
+      if (!live[m] || (j != 0 && j != N - 1 && line != 0 && line != N - 1)) continue;
+      const cx<double> u = ldd(prev, at(j, ln));
+      if (j == N - 1) dg[m] = tbc_diag(dg[m], u, ldd(prev, at(N - 2, ln)), c.dimp);
+      if (j == 0) dg[m] = tbc_diag(dg[m], u, ldd(prev, at(1, ln)), c.dimp);
+      if (line == N - 1) rh[m] = tbc_rhs(rh[m], u, ldd(prev, at(j, lm)), c.dexp);
+      if (line == 0) rh[m] = tbc_rhs(rh[m], u, ldd(prev, at(j, lp)), c.dexp);
+    }
+  }
+  // x0 = D^-1 b and the sweep factor w = D^-1 (i o): x <- x0 - w (x_left + x_right);
+  // dead cells (outside the grid) stay exactly zero: x0 = 0 and w = 0
+  cx<double> x0[C], w[C], x[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    const double r = recip(norm2(dg[m]));
+    const double lr = live[m] ? r : 0.0, lo = live[m] ? c.o * r : 0.0;
+    const cx<double> inv = {dg[m].x * lr, -dg[m].y * lr};
+    x0[m] = x[m] = rh[m] * inv;
+    w[m] = {dg[m].y * lo, dg[m].x * lo};  // i o conj(diag) / |diag|^2
+  }
+
+  // One Jacobi sweep; B selects the LDS buffer (alternating buffers need one
+  // sync per sweep: a lane cannot rewrite a buffer before every lane has
+  // passed the sync that follows the reads of it).
+  auto sweep = [&](auto bsel) {
+    constexpr int B = decltype(bsel)::value;
+    edge[B][0][tid] = x[0];
+    edge[B][1][tid] = x[C - 1];
+    lds_sync<T::WAVE_LOCAL>();
+    const cx<double> xl = g > 0 ? edge[B][1][tid - T::DG] : cx<double>{0.0, 0.0};
+    const cx<double> xr = g < T::G - 1 ? edge[B][0][tid + T::DG] : cx<double>{0.0, 0.0};
+    cx<double> left = xl;
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+      const cx<double> right = m < C - 1 ? x[m + 1] : xr;
+      const cx<double> sum = left + right;
+      left = x[m];
+      // x0 - w sum, as two fused steps per component
+      x[m].x = fma(-w[m].x, sum.x, fma(w[m].y, sum.y, x0[m].x));
+      x[m].y = fma(-w[m].x, sum.y, fma(-w[m].y, sum.x, x0[m].y));
+    }
+  };
+  for (int it = 0; it + 1 < iters; it += 2) {
+    sweep(std::integral_constant<int, 0>{});
+    sweep(std::integral_constant<int, 1>{});
+  }
+  if (iters & 1) sweep(std::integral_constant<int, 0>{});
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    const int t = g * C + m;  // tile slot
+    if (live[m] && t >= halo && t < T::TP - halo) next[at(jb + m, line)] = cvt<R>(x[m]);
+  }
 }
 
-// Sweeps needed for rho^K < 1e-17, or 0 when the system is not dominant
-// enough for the tile halo (the caller then uses the sequential kernel).
-int adi_jacobi_iters(double dz, double k, double d)
+// Sweeps needed for rho^K < tol, or 0 when the system is not dominant enough
+// for the tile halo (the caller then uses the sequential kernel).
+int adi_jacobi_iters(double dz, double k, double d, double tol)
 {
   const double rho = (dz / 2.0) / (k * d * d);
   if (!(rho > 0.0) || rho > 5e-3) return 0;
   int K = 1;
   double e = rho;
-  while (e > 1e-17 && K <= ADI_HALO) {
+  while (e > tol && K <= ADI_HALO_MAX) {
     e *= rho;
     ++K;
   }
-  return K <= ADI_HALO ? K : 0;
+  return K <= ADI_HALO_MAX ? K : 0;
+}
+
+template <class R, bool IMPX>
+static hipError_t launch_tiles(const TriArgs<R> &a, const cx<R> *prev, cx<R> *next, int iters, hipStream_t s)
+{
+  using T = AdiTile<IMPX>;
+  const int stride = T::TP - 2 * iters;
+  const int nseg = (a.n + stride - 1) / stride;
+  const int ngrp = (a.n + T::TL - 1) / T::TL;
+  // the same expressions as the reference's coefficients, once per launch
+  AdiCoef c;
+  c.k = a.khost[0];
+  const double h = a.dz / 2.0;
+  c.z = IMPX ? coord(a.zmin, a.first_step, a.dz) : coord(a.zmin, a.first_step, a.dz) + h;
+  const double ddx = c.k * a.dx * a.dx, ddy = c.k * a.dy * a.dy;
+  c.dimp = IMPX ? ddx : ddy;
+  c.dexp = IMPX ? ddy : ddx;
+  c.ih = 1.0 / h;
+  c.idimp = 1.0 / c.dimp;
+  c.iexp = 1.0 / c.dexp;
+  c.hexp = 0.5 / c.dexp;
+  c.o = -0.5 / c.dimp;
+  c.ddy = ddy;
+  hipLaunchKernelGGL((k_adi_jacobi<R, IMPX>), dim3(nseg * ngrp), dim3(T::LANES), 0, s, a, c, prev, next, iters,
+                     nseg);
+  return hipGetLastError();
 }
 
 template <class R>
 hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
                              int iters, hipStream_t s)
 {
-  const int interior = ADI_TILE - 2 * ADI_HALO;
-  const dim3 grid((a.n + interior - 1) / interior, (a.n + ADI_LINES - 1) / ADI_LINES);
-  if (implicit_x)
-    hipLaunchKernelGGL((k_adi_jacobi<R, true>), grid, dim3(ADI_LINES * ADI_TILE), 0, s, a, prev, next, iters);
-  else
-    hipLaunchKernelGGL((k_adi_jacobi<R, false>), grid, dim3(ADI_LINES * ADI_TILE), 0, s, a, prev, next, iters);
-  return hipGetLastError();
+  if (iters < 1 || iters > ADI_HALO_MAX) return hipErrorInvalidValue;
+  return implicit_x ? launch_tiles<R, true>(a, prev, next, iters, s) : launch_tiles<R, false>(a, prev, next, iters, s);
 }
 
 template hipError_t launch_adi_jacobi<float>(const TriArgs<float> &, bool, const cx<float> *, cx<float> *, int,

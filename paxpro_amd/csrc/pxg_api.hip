@@ -577,6 +577,7 @@ TriArgs<R> tri_args(pxg_plan P)
   a.sol_fstride = P->sol_fstride;
   a.nz = p.nz;
   a.kmin = *std::min_element(P->k_host.begin(), P->k_host.end());
+  a.khost = P->k_host.data();
   return a;
 }
 

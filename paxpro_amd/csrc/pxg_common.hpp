@@ -74,4 +74,20 @@ __host__ __device__ __forceinline__ double coord(double min, unsigned i, double 
   return min + static_cast<double>(i) * step;
 }
 
+// LDS ordering between the phases of a computation.  When the LDS slots a
+// wavefront reads were written by that wavefront alone, completing the
+// wavefront's own LDS accesses (waitcnt, emitted by the work-group-scope
+// fences) orders them: no s_barrier.
+template <bool WAVE_LOCAL>
+__device__ __forceinline__ void lds_sync()
+{
+  if constexpr (WAVE_LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
 }  // namespace pxg

@@ -118,15 +118,14 @@ template <int N, int R, int NS, int S, class Rt>
 __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TwBase<N, Rt> &twb)
 {
   constexpr int Q = 8 / R;
-  constexpr int N8 = N / 8;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int j = t + q * N8;
     cx<Rt> u[8];
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
     if constexpr (NS > 1) {
-      // The base w = e^{-2 pi i (j mod NS)/(NS R)}; its powers w^2..w^{R-1}
+      // The base w = e^{-2 pi i (j mod NS)/(NS R)} of butterfly j = t + q N/8
+      // (held in twb); its powers w^2..w^{R-1}
       // are products at most three deep (a few ulp in FP64).  Loading all R-1
       // powers cost R-1 scattered reads per lane and stage, which dominated
       // the pass time.
@@ -193,22 +192,9 @@ __device__ __forceinline__ void load_canon(cx<Rt> (&v)[8], const cx<Rt> *lds, in
   for (int m = 0; m < 8; ++m) v[m] = lds[lds_pos(t + m * (N / 8))];
 }
 
-// LDS ordering between the stages.  When every lane of a line sits in one
-// wavefront and the line's LDS slice belongs to that wavefront alone (N <= 512
-// with lanes of a line adjacent), completing the wavefront's own LDS accesses
-// (waitcnt, emitted by the work-group-scope fences) orders them: no s_barrier.
-template <bool WAVE_LOCAL>
-__device__ __forceinline__ void lds_sync()
-{
-  if constexpr (WAVE_LOCAL) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  } else {
-    __syncthreads();
-  }
-}
-
+// WL: every lane of a line sits in one wavefront and the line's LDS slice
+// belongs to that wavefront alone (N <= 512 with lanes of a line adjacent), so
+// lds_sync<true> orders the stages.
 template <int N, int S, int NS, class Rt, bool WL>
 struct StockhamStages {
   static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TwBase<N, Rt> &twb)

@@ -613,13 +613,18 @@ hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s)
   const size_t per = fbytes + 2 * align256(cells * 16);
   // scratch per field: [field copy][sup'][rhs']
   char *s0 = static_cast<char *>(a.scratch);
-  // dominant systems: converged tile-Jacobi; otherwise one sequential sweep per line
-  const int iters = adi_jacobi_iters(a.dz, a.kmin, implicit_x ? a.dx : a.dy);
+  // Dominant systems: converged tile-Jacobi; otherwise one sequential sweep per
+  // line.  FP64 fields take the sweeps that reach FP64 rounding; complex64
+  // fields stop where the truncation is 1e-12 relative, four orders of
+  // magnitude below the complex64 rounding of the stored result.
+  const double tol = sizeof(R) == sizeof(double) ? 1e-17 : 1e-12;
+  const int iters = adi_jacobi_iters(a.dz, a.kmin, implicit_x ? a.dx : a.dy, tol);
   for (int b = 0; b < a.batch; ++b) {
     TriArgs<R> ab = a;
     ab.batch = 1;
     ab.field = a.field + b * cells;
     ab.kvec = a.kvec + b;
+    ab.khost = a.khost + b;
     cx<R> *bb = reinterpret_cast<cx<R> *>(s0 + per * b);
     cx<double> *wb = reinterpret_cast<cx<double> *>(s0 + per * b + fbytes);
     const cx<R> *prev = implicit_x ? ab.field : bb;

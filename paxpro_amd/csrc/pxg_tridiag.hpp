@@ -26,6 +26,7 @@ struct TriArgs {
   int nz;
   unsigned first_step, nsteps;
   double kmin;           // smallest wavenumber of the batch (largest Jacobi contraction)
+  const double *khost;   // [batch] wavenumbers in host memory (launch-time constants)
 };
 
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
@@ -58,7 +59,8 @@ int cn_reg_rows();
 template <class R>
 hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
 // Tile-Jacobi ADI half-step for diagonally dominant systems (pxg_adi.hip).
-int adi_jacobi_iters(double dz, double k, double d);
+// Sweeps so that rho^K < tol (0 = not dominant enough for the tile path).
+int adi_jacobi_iters(double dz, double k, double d, double tol);
 template <class R>
 hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
                              int iters, hipStream_t s);

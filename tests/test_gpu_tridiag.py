@@ -117,10 +117,12 @@ def test_cn_batched_uneven_k(ctx):
 
 
 @pytest.mark.parametrize("quirks", [0, P.FIX_ADI_Y_DIAG])
-@pytest.mark.parametrize("n,extent,nsteps", [(64, 4000.0, 4), (300, 6000.0, 2), (64, 128.0, 3)])
+@pytest.mark.parametrize("n,extent,nsteps", [(64, 4000.0, 4), (300, 6000.0, 2), (1000, 20000.0, 2),
+                                             (64, 128.0, 3)])
 def test_adi_vs_oracle(ctx, quirks, n, extent, nsteps):
-    """Dominant grids (dx = 62.5 and 20 nm: tile-Jacobi path, 300 cells spans
-    three tiles) and a fine grid (dx = 2 nm, rho = 0.04: sequential sweep)."""
+    """Dominant grids (dx = 62.5 and 20 nm: tile-Jacobi path; 300 and 1000
+    cells span several tiles along the lines and a partial last group of
+    lines) and a fine grid (dx = 2 nm, rho = 0.04: sequential sweep)."""
     announce("adi", n, extent, nsteps, quirks)
     k, dz = K_01, 20.0
     dx = extent / n
@@ -141,6 +143,40 @@ def test_adi_vs_oracle(ctx, quirks, n, extent, nsteps):
         plan.step(1, nsteps)
         err = rel_l2(plan.get_field(), ref)
         announce("adi rel_l2", use_tbc, err)
+        assert err < 1e-10
+
+
+def test_adi_planes_batch_vs_oracle(ctx):
+    """Host-sampled planes (one per half-step, the CSG scene evaluated at the
+    half-step's z) on a batch of two wavenumbers equal the oracle's CSG run of
+    each field: checks the plane index of each half-step and the per-field
+    wavenumber of the tile path."""
+    announce("adi planes batch")
+    n, nsteps, dz, extent = 300, 3, 20.0, 6000.0
+    dx = extent / n
+    xmin, zmin = -extent / 2, -40.0
+    ks = np.array([K_01, 2 * np.pi / 0.15])
+    sph = random_spheres(0x5EED0005, 6, (-1500, 1500), (-40, 40), (300, 900), (1e-6, 1e-5))
+    csg = O.CSGMaterial(*spheres_scene(sph))
+    xs = xmin + dx * np.arange(n)
+    planes = np.empty((2 * nsteps, n * n, 2))
+    for st in range(1, nsteps + 1):
+        for half in (0, 1):
+            z = zmin + st * dz + half * dz / 2
+            vals = np.array([[csg.eval(xs[c], xs[r], z) for r in range(n)] for c in range(n)])
+            planes[2 * (st - 1) + half] = vals.reshape(n * n, 2)
+    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + xs[None, :] ** 2) / 800.0 ** 2).astype(complex))
+    plan = P.Plan(ctx, P.problem(P.ADI, n, nsteps + 1, dx, dz, K_01, xmin=xmin, zmin=zmin, batch=2))
+    plan.set_wavenumbers(ks)
+    plan.material_planes(planes, first_plane=2)
+    for b in range(2):
+        plan.set_field(u0, b)
+    plan.step(1, nsteps)
+    for b in range(2):
+        g = O.grid(xmin, xmin + (n - 1) * dx, dx, zmin, zmin + nsteps * dz, dz, ks[b])
+        ref = O.adi_run(g, csg, u0, 1, nsteps)
+        err = rel_l2(plan.get_field(b), ref)
+        announce("adi planes batch", b, err)
         assert err < 1e-10
 
 
