@@ -46,30 +46,12 @@ struct AdiTile {
 };
 static_assert(AdiTile<false>::G == 64, "y-implicit line tile must be one wavefront");
 
-// Transparent boundary factor of the reference, e = exp(i kd) with
-// kd = -i log(r), r = u_out / u_in, and Re(kd) = arg(r) clamped at 0
-// (src/alternatingDirectionSolver.cpp:166-230).  exp(i kd) = |r| e^{i arg r} is
-// r itself when arg r >= 0, and |r| when arg r < 0 (the clamp removes the
-// phase), so no logarithm or exponential is needed: arg r < 0 exactly when
-// atan2(Im r, Re r) < 0, i.e. Im r < 0, or Im r = -0 with Re r < 0.  Returns
-// false when |u_in| <= 1e-6 (the reference then leaves the row unchanged).
-__device__ __forceinline__ bool tbc_e(cx<double> u_out, cx<double> u_in, cx<double> &e)
-{
-  const double n2 = norm2(u_in);
-  if (!(sqrt(n2) > 1e-6)) return false;
-  const double in2 = 1.0 / n2;
-  const cx<double> r = {(u_out.x * u_in.x + u_out.y * u_in.y) * in2, (u_out.y * u_in.x - u_out.x * u_in.y) * in2};
-  const bool clamp = r.y < 0.0 || (r.y == 0.0 && signbit(r.y) && r.x < 0.0);
-  e = clamp ? cx<double>{sqrt(norm2(r)), 0.0} : r;
-  return true;
-}
-
 // Transparent boundary at the ends of the implicit line: diag correction from
 // the cell's value and its inward neighbour along the line.
 __device__ __forceinline__ cx<double> tbc_diag(cx<double> diag, cx<double> u, cx<double> uin, double dimp)
 {
   cx<double> e;
-  if (tbc_e(u, uin, e)) diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
+  if (tbc_exp_ikd(u, uin, 1e-6, e)) diag = diag - cx<double>{-0.5 * e.y / dimp, 0.5 * e.x / dimp};
   return diag;
 }
 
@@ -78,7 +60,7 @@ __device__ __forceinline__ cx<double> tbc_diag(cx<double> diag, cx<double> u, cx
 __device__ __forceinline__ cx<double> tbc_rhs(cx<double> rhs, cx<double> u, cx<double> uin, double dexp)
 {
   cx<double> e;
-  if (tbc_e(u, uin, e)) {
+  if (tbc_exp_ikd(u, uin, 1e-6, e)) {
     const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
     rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
   }
@@ -102,18 +84,6 @@ struct AdiCoef {
   double o;           // off-diagonal i*o, o = -0.5 / dimp
   double ddy;         // k dy^2 (y-implicit diagonal of the reference)
 };
-
-// 1/d for a finite positive normal d: the hardware reciprocal estimate refined
-// by two Newton-Raphson steps r <- r + r (1 - d r), each of which squares the
-// relative error; the result is within an ulp or two of the rounded quotient.
-__device__ __forceinline__ double recip(double d)
-{
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
 
 // Linear work-group index -> tile index such that the work-groups the hardware
 // places on one XCD (index mod 8) take a contiguous range of tiles.

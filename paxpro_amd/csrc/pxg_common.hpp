@@ -66,6 +66,27 @@ __device__ __forceinline__ cx<R> expi_fp64(double phi)
   return {static_cast<R>(c), static_cast<R>(s)};
 }
 
+// 1/d for a finite, normal, nonzero d: the hardware reciprocal estimate refined
+// by two Newton-Raphson steps r <- r + r (1 - d r), each of which squares the
+// relative error; the result is within an ulp or two of the rounded quotient.
+// Used where a quotient feeds FP64 arithmetic whose tolerance is far above an
+// ulp (tridiagonal eliminations), in place of the IEEE division sequence.
+__device__ __forceinline__ double recip(double d)
+{
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/b = conj(b) / |b|^2 for |b|^2 in the normal range.
+__device__ __forceinline__ cx<double> crecip(cx<double> b)
+{
+  const double r = recip(b.x * b.x + b.y * b.y);
+  return {b.x * r, -b.y * r};
+}
+
 // Grid coordinate min + i*step, evaluated without FMA contraction like the
 // reference's getX/getY/getZ (src/paraxialSimulation.cpp:250-263).
 __host__ __device__ __forceinline__ double coord(double min, unsigned i, double step)

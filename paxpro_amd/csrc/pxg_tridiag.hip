@@ -56,26 +56,11 @@ size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed)
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) { return a * b; }
 
-// Transparent-boundary factor exp(i kd) with kd = -i log(u_out/u_in) and
-// Re(kd) clamped at 0 (src/crancNicholson.cpp:131-141,
-// src/alternatingDirectionSolver.cpp:157-234).  Returns false when
-// |u_in| <= thresh, in which case the reference leaves the row unchanged.
-// Out of line (like absorb_factor): boundary rows only, and inlined into the
-// unrolled row loops its log/atan2/exp/sincos constants would be hoisted into
-// registers that count against the whole kernel.
-__device__ __noinline__ bool tbc_factor(cx<double> u_out, cx<double> u_in, double thresh, cx<double> &e)
+// Transparent-boundary factor of the CN and sequential ADI rows: the closed
+// form tbc_exp_ikd (no logarithm or exponential).
+__device__ __forceinline__ bool tbc_factor(cx<double> u_out, cx<double> u_in, double thresh, cx<double> &e)
 {
-  if (!(sqrt(norm2(u_in)) > thresh)) return false;
-  const cx<double> ratio = cdiv(u_out, u_in);
-  const double lr = log(hypot(ratio.x, ratio.y));
-  double kr = atan2(ratio.y, ratio.x);  // Re(kd) = arg(ratio)
-  const double ki = -lr;                 // Im(kd) = -log|ratio|
-  if (kr < 0.0) kr = 0.0;
-  const double amp = exp(-ki);
-  double s, c;
-  sincos(kr, &s, &c);
-  e = {amp * c, amp * s};
-  return true;
+  return tbc_exp_ikd(u_out, u_in, thresh, e);
 }
 
 __device__ __forceinline__ void material_2d(const MatDev &m, const uint64_t *mask, int nw,
@@ -151,6 +136,12 @@ __device__ __forceinline__ void cn_row(const CnJob &J, const cx<R> *prev, int i,
   cn_row_v(J, i, st, z, rho, r, mask, nw, ul, u, ur, uin, ai, bi, ci, di);
 }
 
+// Rows inside the absorbing layer of npix cells at either edge.
+__device__ __forceinline__ bool absorbing(int i, int N, unsigned npix)
+{
+  return i < static_cast<int>(npix) || N - 1 - i < static_cast<int>(npix);
+}
+
 __device__ __noinline__ double absorb_factor(int i, int N, unsigned npix, double inv)
 {
   double f = 1.0;
@@ -177,13 +168,13 @@ __device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rb, cx<double
         cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
         cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
         if (row - h >= 0) {
-          k1 = cdiv(a_, rb[row - h]);
+          k1 = a_ * crecip(rb[row - h]);
           am = ra[row - h];
           cm = rc[row - h];
           dm = rd[row - h];
         }
         if (row + h < M) {
-          k2 = cdiv(c_, rb[row + h]);
+          k2 = c_ * crecip(rb[row + h]);
           ap = ra[row + h];
           cp = rc[row + h];
           dp = rd[row + h];
@@ -219,7 +210,9 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
 {
   __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
-  const CnJob &J = jobs[blockIdx.x];
+  // a copy: the descriptor stays in registers instead of being re-read after
+  // every barrier (barriers order global memory, so a reference reloads it)
+  const CnJob J = jobs[blockIdx.x];
   const int p = threadIdx.x, N = J.nx, P = J.P, m = J.m;
   const bool active = p < P;
   char *base = scratch + J.scoff;
@@ -256,14 +249,15 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
         cx<double> ai, bi, ci, di;
         cn_row(J, prev, i, st, z, rho, r, mask, nw, ai, bi, ci, di);
         if (i == s + 1) {
-          ah = cdiv(ai, bi);
-          ch = cdiv(ci, bi);
-          dh = cdiv(di, bi);
+          const cx<double> ib = crecip(bi);
+          ah = ai * ib;
+          ch = ci * ib;
+          dh = di * ib;
         } else {
-          const cx<double> den = bi - ai * ch;
-          const cx<double> ahn = cdiv(cx<double>{0.0, 0.0} - ai * ah, den);
-          ch = cdiv(ci, den);
-          dh = cdiv(di - ai * dh, den);
+          const cx<double> iden = crecip(bi - ai * ch);
+          const cx<double> ahn = (cx<double>{0.0, 0.0} - ai * ah) * iden;
+          ch = ci * iden;
+          dh = (di - ai * dh) * iden;
           ah = ahn;
         }
         const int j = i - s;
@@ -298,8 +292,8 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
 
     cn_pcr(ra, rb, rc, rd, M, p, active);
     if (active) {
-      const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
-      const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
+      const cx<double> xs = rd[2 * p] * crecip(rb[2 * p]);
+      const cx<double> xe = rd[2 * p + 1] * crecip(rb[2 * p + 1]);
       cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
                       ? sol + J.soff + static_cast<size_t>(st) * N
                       : nullptr;
@@ -311,7 +305,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
           const int j = i - s;
           x = cd[j * P + p] - ca[j * P + p] * xs - cc[j * P + p] * xe;
         }
-        if (J.absorb_npix) x = scale(x, absorb_factor(i, N, J.absorb_npix, J.absorb_inv));
+        if (absorbing(i, N, J.absorb_npix)) x = scale(x, absorb_factor(i, N, J.absorb_npix, J.absorb_inv));
         const cx<R> xr = cvt<R>(x);
         next[i] = xr;
         if (sc) sc[i] = xr;
@@ -338,7 +332,9 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
   __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
   __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
-  const CnJob &J = jobs[blockIdx.x];
+  // a copy: the descriptor stays in registers instead of being re-read after
+  // every barrier (barriers order global memory, so a reference reloads it)
+  const CnJob J = jobs[blockIdx.x];
   const int p = threadIdx.x, N = J.nx, P = J.P;
   const bool active = p < P;
   cx<R> *fld = fields + J.foff;
@@ -391,14 +387,15 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
           cx<double> ai, bi, ci, di;
           cn_row_v(J, i, st, z, rho, r, mask, nw, u[j - 1], u[j], un, uin, ai, bi, ci, di);
           if (j == 1) {
-            ah[j] = cdiv(ai, bi);
-            ch[j] = cdiv(ci, bi);
-            dh[j] = cdiv(di, bi);
+            const cx<double> ib = crecip(bi);
+            ah[j] = ai * ib;
+            ch[j] = ci * ib;
+            dh[j] = di * ib;
           } else {
-            const cx<double> den = bi - ai * ch[j - 1];
-            ah[j] = cdiv(zero - ai * ah[j - 1], den);
-            ch[j] = cdiv(ci, den);
-            dh[j] = cdiv(di - ai * dh[j - 1], den);
+            const cx<double> iden = crecip(bi - ai * ch[j - 1]);
+            ah[j] = (zero - ai * ah[j - 1]) * iden;
+            ch[j] = ci * iden;
+            dh[j] = (di - ai * dh[j - 1]) * iden;
           }
         }
       }
@@ -433,8 +430,8 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
     __syncthreads();
     cn_pcr(ra, rb, rc, rd, M, p, active);
     if (active) {
-      const cx<double> xs = cdiv(rd[2 * p], rb[2 * p]);
-      const cx<double> xe = cdiv(rd[2 * p + 1], rb[2 * p + 1]);
+      const cx<double> xs = rd[2 * p] * crecip(rb[2 * p]);
+      const cx<double> xe = rd[2 * p + 1] * crecip(rb[2 * p + 1]);
       // back substitution from the chunk's end
       cx<double> xn = xe;
 #pragma unroll
@@ -453,7 +450,8 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
 #pragma unroll
       for (int j = 0; j < MM; ++j) {
         if (j < len) {
-          if (J.absorb_npix) u[j] = scale(u[j], absorb_factor(s + j, N, J.absorb_npix, J.absorb_inv));
+          if (absorbing(s + j, N, J.absorb_npix))
+            u[j] = scale(u[j], absorb_factor(s + j, N, J.absorb_npix, J.absorb_inv));
           if (sc) sc[s + j] = cvt<R>(u[j]);
         }
       }

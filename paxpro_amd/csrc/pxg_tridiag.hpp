@@ -31,6 +31,26 @@ struct TriArgs {
 
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
 
+// Transparent boundary factor of the reference, e = exp(i kd) with
+// kd = -i log(r), r = u_out / u_in, and Re(kd) = arg(r) clamped at 0
+// (src/crancNicholson.cpp:131-141, src/alternatingDirectionSolver.cpp:166-230).
+// exp(i kd) = |r| e^{i arg r} is r itself when arg r >= 0, and |r| when
+// arg r < 0 (the clamp removes the phase), so no logarithm or exponential is
+// needed: arg r < 0 exactly when atan2(Im r, Re r) < 0, i.e. Im r < 0, or
+// Im r = -0 with Re r < 0.  Returns false when |u_in| <= thresh (the reference
+// then leaves the row unchanged).
+__device__ __forceinline__ bool tbc_exp_ikd(cx<double> u_out, cx<double> u_in, double thresh, cx<double> &e)
+{
+  const double n2 = norm2(u_in);
+  if (!(sqrt(n2) > thresh)) return false;
+  const double in2 = 1.0 / n2;
+  const cx<double> r = {(u_out.x * u_in.x + u_out.y * u_in.y) * in2, (u_out.y * u_in.x - u_out.x * u_in.y) * in2};
+  const bool clamp = r.y < 0.0 || (r.y == 0.0 && signbit(r.y) && r.x < 0.0);
+  e = clamp ? cx<double>{sqrt(norm2(r)), 0.0} : r;
+  return true;
+}
+
+
 // One CN job: a field with its own grid, wavenumber and material, solved by
 // one work-group (k_cn_jobs).  Offsets index the plan's field, solution and
 // scratch buffers.
