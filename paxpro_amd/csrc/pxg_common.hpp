@@ -95,6 +95,18 @@ __host__ __device__ __forceinline__ double coord(double min, unsigned i, double 
   return min + static_cast<double>(i) * step;
 }
 
+// Work-group barrier that orders LDS accesses only: the fences are restricted
+// to the local address space, so the barrier waits for the wavefront's LDS
+// operations (lgkmcnt) but not for its outstanding global loads and stores,
+// which __syncthreads would drain (vmcnt).  For phases that exchange data
+// through LDS alone.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // LDS ordering between the phases of a computation.  When the LDS slots a
 // wavefront reads were written by that wavefront alone, completing the
 // wavefront's own LDS accesses (waitcnt, emitted by the work-group-scope

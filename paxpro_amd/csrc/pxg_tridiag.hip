@@ -94,16 +94,12 @@ __device__ __forceinline__ cx<double> ld(const cx<R> *f, int i)
 // transparent boundary rows at i = 0 and N-1 (src/crancNicholson.cpp:30-89,107-148).
 // ul, u, ur are the previous field at i-1, i, i+1 (zero outside); uin is the
 // inner neighbour of a boundary row (i = 1 or N-2), used only at i = 0, N-1.
-__device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, double z, double rho, double r,
-                                         const uint64_t *mask, int nw, cx<double> ul, cx<double> u,
-                                         cx<double> ur, cx<double> uin, cx<double> &ai, cx<double> &bi,
-                                         cx<double> &ci, cx<double> &di)
+// The row from the material of the step (d1, b1) and of the previous step (d0, b0).
+__device__ __forceinline__ void cn_row_m(int N, int bc, int i, double rho, double r, double d1, double b1,
+                                         double d0, double b0, cx<double> ul, cx<double> u, cx<double> ur,
+                                         cx<double> uin, cx<double> &ai, cx<double> &bi, cx<double> &ci,
+                                         cx<double> &di)
 {
-  const int N = J.nx;
-  const double x = coord(J.xmin, i, J.dx);
-  double d1, b1, d0, b0;
-  material_2d(J.mat, mask, nw, st, i, x, z, d1, b1);
-  material_2d(J.mat, mask, nw, st - 1, i, x, z - J.dz, d0, b0);
   const cx<double> off = {0.0, -0.25 * rho};
   ai = i > 0 ? off : cx<double>{0.0, 0.0};
   ci = i < N - 1 ? off : cx<double>{0.0, 0.0};
@@ -112,7 +108,7 @@ __device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, dou
   v = v - cx<double>{-0.5 * rho * u.y, 0.5 * rho * u.x};
   v = v + cx<double>{1.0 - 0.5 * (b0 * r), -0.5 * (d0 * r)} * u;
   di = v;
-  if (J.bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
+  if (bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
     cx<double> e;
     if (tbc_factor(u, uin, 1e-16, e)) {
       const cx<double> q = cx<double>{0.0, 0.25 * rho} * e;
@@ -120,6 +116,18 @@ __device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, dou
       di = di + q * u;
     }
   }
+}
+
+__device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, double z, double rho, double r,
+                                         const uint64_t *mask, int nw, cx<double> ul, cx<double> u,
+                                         cx<double> ur, cx<double> uin, cx<double> &ai, cx<double> &bi,
+                                         cx<double> &ci, cx<double> &di)
+{
+  const double x = coord(J.xmin, i, J.dx);
+  double d1, b1, d0, b0;
+  material_2d(J.mat, mask, nw, st, i, x, z, d1, b1);
+  material_2d(J.mat, mask, nw, st - 1, i, x, z - J.dz, d0, b0);
+  cn_row_m(J.nx, J.bc, i, rho, r, d1, b1, d0, b0, ul, u, ur, uin, ai, bi, ci, di);
 }
 
 template <class R>
@@ -151,53 +159,76 @@ __device__ __noinline__ double absorb_factor(int i, int N, unsigned npix, double
   return f;
 }
 
-// Parallel cyclic reduction of the M-row tridiagonal system (ra, rb, rc, rd)
-// in LDS; lane p owns rows 2p and 2p+1.  On return row i reads rb[i] x_i = rd[i].
-// Every lane of the work-group calls it (work-group barriers).
-__device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rb, cx<double> *rc, cx<double> *rd, int M,
-                                       int p, bool active)
+// One row a x_{i-1} + b x_i + c x_{i+1} = d of the reduced system.
+struct TriRow {
+  cx<double> a, b, c, d;
+};
+
+// Parallel cyclic reduction of the M-row reduced system; lane p owns rows 2p
+// and 2p+1 (r0, r1) and receives their solution (x0, x1).  Rows are kept
+// normalised to a unit diagonal, so a level needs one reciprocal per row and
+// only the neighbours' (a, c, d) from LDS (ra, rc, rd); a lane keeps its own
+// rows in registers.  Every lane of the work-group calls it; the barriers
+// order LDS only.
+__device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double> *rd, int M, int p, bool active,
+                                       const TriRow &r0, const TriRow &r1, cx<double> &x0, cx<double> &x1)
 {
+  const cx<double> zero{0.0, 0.0}, one{1.0, 0.0};
+  cx<double> a[2], c[2], d[2];
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const TriRow &r = q ? r1 : r0;
+      const cx<double> ib = crecip(r.b);
+      a[q] = r.a * ib;
+      c[q] = r.c * ib;
+      d[q] = r.d * ib;
+      ra[2 * p + q] = a[q];
+      rc[2 * p + q] = c[q];
+      rd[2 * p + q] = d[q];
+    }
+  }
+  lds_barrier();
   for (int h = 1; h < M; h *= 2) {
-    cx<double> na[2], nb[2], nc[2], nd[2];
+    cx<double> na[2], nc[2], nd[2];
     if (active) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int row = 2 * p + q;
-        cx<double> a_ = ra[row], b_ = rb[row], c_ = rc[row], d_ = rd[row];
-        cx<double> k1{0.0, 0.0}, k2{0.0, 0.0};
-        cx<double> am{0.0, 0.0}, cm{0.0, 0.0}, dm{0.0, 0.0};
-        cx<double> ap{0.0, 0.0}, cp{0.0, 0.0}, dp{0.0, 0.0};
+        cx<double> am = zero, cm = zero, dm = zero, ap = zero, cp = zero, dp = zero;
         if (row - h >= 0) {
-          k1 = a_ * crecip(rb[row - h]);
           am = ra[row - h];
           cm = rc[row - h];
           dm = rd[row - h];
         }
         if (row + h < M) {
-          k2 = c_ * crecip(rb[row + h]);
           ap = ra[row + h];
           cp = rc[row + h];
           dp = rd[row + h];
         }
-        na[q] = cx<double>{0.0, 0.0} - am * k1;
-        nc[q] = cx<double>{0.0, 0.0} - cp * k2;
-        nb[q] = b_ - cm * k1 - ap * k2;
-        nd[q] = d_ - dm * k1 - dp * k2;
+        // eliminate x_{row-h} and x_{row+h} with the (unit-diagonal) rows there
+        const cx<double> ib = crecip(one - a[q] * cm - c[q] * ap);
+        na[q] = (zero - a[q] * am) * ib;
+        nc[q] = (zero - c[q] * cp) * ib;
+        nd[q] = (d[q] - a[q] * dm - c[q] * dp) * ib;
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (active) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        const int row = 2 * p + q;
-        ra[row] = na[q];
-        rb[row] = nb[q];
-        rc[row] = nc[q];
-        rd[row] = nd[q];
+        a[q] = na[q];
+        c[q] = nc[q];
+        d[q] = nd[q];
+        ra[2 * p + q] = a[q];
+        rc[2 * p + q] = c[q];
+        rd[2 * p + q] = d[q];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
+  x0 = d[0];
+  x1 = d[1];
 }
 
 // ---------------------------------------------------------------- CN march
@@ -208,7 +239,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
                                                       char *scratch, cx<R> *sol, unsigned first,
                                                       unsigned nsteps)
 {
-  __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
   // a copy: the descriptor stays in registers instead of being re-read after
   // every barrier (barriers order global memory, so a reference reloads it)
@@ -240,6 +271,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
     if (J.mat.mode == MAT_CSG)
       csg_cull(J.mat, mask, nw, J.xmin, coord(J.xmin, N - 1, J.dx), 0.0, 0.0, z - J.dz, z);
 
+    TriRow r0{}, r1{};
     if (active) {
       // chunk elimination: rows s+1..e expressed through x_s (and x_{e+1})
       cx<double> as, bs, cs, ds;
@@ -266,10 +298,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
         cd[j * P + p] = dh;
       }
       // last row of the chunk is a reduced equation: ah x_s + x_e + ch x_{e+1} = dh
-      ra[2 * p + 1] = ah;
-      rb[2 * p + 1] = {1.0, 0.0};
-      rc[2 * p + 1] = ch;
-      rd[2 * p + 1] = dh;
+      r1 = {ah, {1.0, 0.0}, ch, dh};
       // backward: x_i = dt_i - at_i x_s - ct_i x_e, base at i = e: (0, 0, -1)
       cx<double> at{0.0, 0.0}, ct{-1.0, 0.0}, dt{0.0, 0.0};
       for (int i = e - 1; i >= s + 1; --i) {
@@ -283,17 +312,11 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
         cd[j * P + p] = dt;
       }
       // first row of the chunk: a_s x_{s-1} + (b_s - c_s at) x_s - c_s ct x_e = d_s - c_s dt
-      ra[2 * p] = as;
-      rb[2 * p] = bs - cs * at;
-      rc[2 * p] = cx<double>{0.0, 0.0} - cs * ct;
-      rd[2 * p] = ds - cs * dt;
+      r0 = {as, bs - cs * at, cx<double>{0.0, 0.0} - cs * ct, ds - cs * dt};
     }
-    __syncthreads();
-
-    cn_pcr(ra, rb, rc, rd, M, p, active);
+    cx<double> xs, xe;
+    cn_pcr(ra, rc, rd, M, p, active, r0, r1, xs, xe);
     if (active) {
-      const cx<double> xs = rd[2 * p] * crecip(rb[2 * p]);
-      const cx<double> xe = rd[2 * p + 1] * crecip(rb[2 * p + 1]);
       cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
                       ? sol + J.soff + static_cast<size_t>(st) * N
                       : nullptr;
@@ -329,18 +352,24 @@ template <class R, int MM>
 __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ jobs, cx<R> *fields, cx<R> *sol,
                                                      unsigned first, unsigned nsteps)
 {
-  __shared__ cx<double> ra[2 * CN_MAX_P], rb[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
   __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
-  // a copy: the descriptor stays in registers instead of being re-read after
-  // every barrier (barriers order global memory, so a reference reloads it)
-  const CnJob J = jobs[blockIdx.x];
-  const int p = threadIdx.x, N = J.nx, P = J.P;
+  // The scalars of the descriptor the step loop uses are copied: barriers
+  // order global memory, so fields read through the reference would be
+  // reloaded after each one.  The material stays behind the reference (a full
+  // copy spills scalar registers into vector ones).
+  const CnJob &J = jobs[blockIdx.x];
+  const int p = threadIdx.x, N = J.nx, P = J.P, bc = J.bc;
   const bool active = p < P;
   cx<R> *fld = fields + J.foff;
-  const double k = J.k;
-  const double rho = J.dz / (k * J.dx * J.dx);
-  const double r = k * J.dz;
+  const double k = J.k, dz = J.dz, dx = J.dx, xmin = J.xmin, zmin = J.zmin;
+  const double rho = dz / (k * dx * dx);
+  const double r = k * dz;
+  const unsigned npix = J.absorb_npix;
+  const double ainv = J.absorb_inv;
+  const size_t soff = J.soff;
+  const int nz = J.nz;
   const int len0 = N / P, rem = N % P;
   const int s = p * len0 + min(p, rem);
   const int len = active ? len0 + (p < rem ? 1 : 0) : 0;  // 2 <= len <= MM for active lanes
@@ -351,12 +380,35 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
   cx<double> u[MM];
 #pragma unroll
   for (int j = 0; j < MM; ++j) u[j] = j < len ? cvt<double>(fld[s + j]) : zero;
+  // z-invariant planes: the lane's material once, not a global load per row
+  // and step on the step's critical path
+  const bool fixed_mat = J.mat.mode == MAT_PLANES && J.mat.z_invariant;
+  double fd[MM], fb[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) {
+    fd[j] = fb[j] = 0.0;
+    if (fixed_mat && j < len) {
+      fd[j] = J.mat.planes[2 * (s + j)];
+      fb[j] = J.mat.planes[2 * (s + j) + 1];
+    }
+  }
+  // row i = s + j of the step
+  auto row = [&](int j, unsigned st, double z, cx<double> ul_, cx<double> u_, cx<double> ur_, cx<double> uin,
+                 cx<double> &ai, cx<double> &bi, cx<double> &ci, cx<double> &di) {
+    double d1 = fd[j], b1 = fb[j], d0 = fd[j], b0 = fb[j];
+    if (!fixed_mat) {
+      const double x = coord(xmin, s + j, dx);
+      material_2d(J.mat, mask, nw, st, s + j, x, z, d1, b1);
+      material_2d(J.mat, mask, nw, st - 1, s + j, x, z - dz, d0, b0);
+    }
+    cn_row_m(N, bc, s + j, rho, r, d1, b1, d0, b0, ul_, u_, ur_, uin, ai, bi, ci, di);
+  };
 
   for (unsigned it = 0; it < nsteps; ++it) {
     const unsigned st = first + it;
-    const double z = coord(J.zmin, st, J.dz);
+    const double z = coord(zmin, st, dz);
     if (J.mat.mode == MAT_CSG)
-      csg_cull(J.mat, mask, nw, J.xmin, coord(J.xmin, N - 1, J.dx), 0.0, 0.0, z - J.dz, z);
+      csg_cull(J.mat, mask, nw, xmin, coord(xmin, N - 1, dx), 0.0, 0.0, z - dz, z);
     // chunk-edge values for the neighbouring lanes
     cx<double> ulast = u[0], u2last = u[0];
 #pragma unroll
@@ -368,15 +420,16 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
       edge_lo[p] = u[0];
       edge_hi[p] = ulast;
     }
-    __syncthreads();
+    lds_barrier();
     const cx<double> ul = (active && p > 0) ? edge_hi[p - 1] : zero;
     const cx<double> ur = (active && p < P - 1) ? edge_lo[p + 1] : zero;
 
     cx<double> ah[MM], ch[MM], dh[MM];
+    TriRow r0{}, r1{};
     if (active) {
       // first row of the chunk
       cx<double> as, bs, cs, ds;
-      cn_row_v(J, s, st, z, rho, r, mask, nw, ul, u[0], u[1], u[1], as, bs, cs, ds);
+      row(0, st, z, ul, u[0], u[1], u[1], as, bs, cs, ds);
       // forward elimination of rows s+1 .. s+len-1
 #pragma unroll
       for (int j = 1; j < MM; ++j) {
@@ -385,7 +438,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
           const cx<double> un = (j + 1 < MM && j + 1 < len) ? u[j + 1 < MM ? j + 1 : 0] : ur;
           const cx<double> uin = (i == N - 1) ? u2last : u[j];
           cx<double> ai, bi, ci, di;
-          cn_row_v(J, i, st, z, rho, r, mask, nw, u[j - 1], u[j], un, uin, ai, bi, ci, di);
+          row(j, st, z, u[j - 1], u[j], un, uin, ai, bi, ci, di);
           if (j == 1) {
             const cx<double> ib = crecip(bi);
             ah[j] = ai * ib;
@@ -408,10 +461,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
           chl = ch[j];
           dhl = dh[j];
         }
-      ra[2 * p + 1] = ahl;
-      rb[2 * p + 1] = {1.0, 0.0};
-      rc[2 * p + 1] = chl;
-      rd[2 * p + 1] = dhl;
+      r1 = {ahl, {1.0, 0.0}, chl, dhl};
       // backward: row s+1 through x_s and x_e (running values, base (0, 0, -1) at e)
       cx<double> at = zero, ct{-1.0, 0.0}, dt = zero;
 #pragma unroll
@@ -422,16 +472,11 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
           ct = zero - ch[j] * ct;
         }
       }
-      ra[2 * p] = as;
-      rb[2 * p] = bs - cs * at;
-      rc[2 * p] = zero - cs * ct;
-      rd[2 * p] = ds - cs * dt;
+      r0 = {as, bs - cs * at, zero - cs * ct, ds - cs * dt};
     }
-    __syncthreads();
-    cn_pcr(ra, rb, rc, rd, M, p, active);
+    cx<double> xs, xe;
+    cn_pcr(ra, rc, rd, M, p, active, r0, r1, xs, xe);
     if (active) {
-      const cx<double> xs = rd[2 * p] * crecip(rb[2 * p]);
-      const cx<double> xe = rd[2 * p + 1] * crecip(rb[2 * p + 1]);
       // back substitution from the chunk's end
       cx<double> xn = xe;
 #pragma unroll
@@ -444,19 +489,18 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
         }
       }
       u[0] = xs;
-      cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
-                      ? sol + J.soff + static_cast<size_t>(st) * N
+      cx<R> *sc = (sol && soff != ~static_cast<size_t>(0) && static_cast<int>(st) < nz)
+                      ? sol + soff + static_cast<size_t>(st) * N
                       : nullptr;
 #pragma unroll
       for (int j = 0; j < MM; ++j) {
         if (j < len) {
-          if (absorbing(s + j, N, J.absorb_npix))
-            u[j] = scale(u[j], absorb_factor(s + j, N, J.absorb_npix, J.absorb_inv));
+          if (absorbing(s + j, N, npix)) u[j] = scale(u[j], absorb_factor(s + j, N, npix, ainv));
           if (sc) sc[s + j] = cvt<R>(u[j]);
         }
       }
     }
-    __syncthreads();  // edge arrays and the reduced system are reused next step
+    lds_barrier();  // edge arrays and the reduced system are reused next step
   }
 #pragma unroll
   for (int j = 0; j < MM; ++j)
