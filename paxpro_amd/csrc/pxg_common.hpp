@@ -110,7 +110,8 @@ __device__ __forceinline__ void lds_barrier()
 // LDS ordering between the phases of a computation.  When the LDS slots a
 // wavefront reads were written by that wavefront alone, completing the
 // wavefront's own LDS accesses (waitcnt, emitted by the work-group-scope
-// fences) orders them: no s_barrier.
+// fences) orders them: no s_barrier.  Otherwise a work-group barrier that
+// orders LDS only.
 template <bool WAVE_LOCAL>
 __device__ __forceinline__ void lds_sync()
 {
@@ -119,7 +120,7 @@ __device__ __forceinline__ void lds_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   } else {
-    __syncthreads();
+    lds_barrier();
   }
 }
 

@@ -231,7 +231,7 @@ __device__ __forceinline__ void fft_line_remap_in(cd (&v)[8], cd *la, cd *lb, in
   const TwBase<N, double> none{};
   stage_compute<N, F, 1, S>(v, ta, none);
   stage_store<N, F, 1>(v, la, ta);
-  __syncthreads();
+  lds_barrier();
   load_canon<N>(v, lb, tb);
   if constexpr (F * 8 < N) lds_sync<true>();
   StockhamStages<N, S, F, double, true>::run(v, lb, tb, twb_b);
@@ -255,7 +255,7 @@ struct RemapOutStages {
       lds_sync<true>();
       RemapOutStages<N, S, NS * R>::run(v, la, lb, ta, tb, twb_a, twb_b);
     } else {
-      __syncthreads();
+      lds_barrier();
       load_canon<N>(v, la, ta);
       stage_compute<N, 8, NS * R, S>(v, ta, twb_a);
     }

@@ -214,18 +214,19 @@ __device__ __forceinline__ bool box_hits(const DBox &bx, double xlo, double xhi,
 }
 
 // Work-group culling: parts whose boxes meet the region are marked in the LDS
-// mask (nw words, cleared here).  Must be called by every lane.
+// mask (nw words, cleared here).  Must be called by every lane; its barriers
+// order LDS only (outstanding global stores of the caller stay in flight).
 __device__ __forceinline__ void csg_cull(const MatDev &m, uint64_t *mask, int nw, double xlo,
                                          double xhi, double ylo, double yhi, double zlo,
                                          double zhi)
 {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int w = tid; w < nw; w += nt) mask[w] = 0;
-  __syncthreads();
+  lds_barrier();
   for (int e = tid; e < m.n_parts; e += nt)
     if (box_hits(m.boxes[e], xlo, xhi, ylo, yhi, zlo, zhi))
       atomicOr(reinterpret_cast<unsigned long long *>(&mask[e >> 6]), 1ull << (e & 63));
-  __syncthreads();
+  lds_barrier();
 }
 
 // Curved waveguide membership and material (src/curvedWaveGuide2D.cpp:49-52,
