@@ -167,13 +167,17 @@ struct TriRow {
 // Parallel cyclic reduction of the M-row reduced system; lane p owns rows 2p
 // and 2p+1 (r0, r1) and receives their solution (x0, x1).  Rows are kept
 // normalised to a unit diagonal, so a level needs one reciprocal per row and
-// only the neighbours' (a, c, d) from LDS (ra, rc, rd); a lane keeps its own
-// rows in registers.  Every lane of the work-group calls it; the barriers
+// only the neighbours' (a, c, d) from LDS (ra, rc, rd, cap entries each); a
+// lane keeps its own rows in registers.  When 2M <= cap the levels alternate
+// between two halves of the arrays (one barrier per level), otherwise they
+// update in place (two).  Every lane of the work-group calls it; the barriers
 // order LDS only.
-__device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double> *rd, int M, int p, bool active,
-                                       const TriRow &r0, const TriRow &r1, cx<double> &x0, cx<double> &x1)
+__device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double> *rd, int cap, int M, int p,
+                                       bool active, const TriRow &r0, const TriRow &r1, cx<double> &x0,
+                                       cx<double> &x1)
 {
   const cx<double> zero{0.0, 0.0}, one{1.0, 0.0};
+  const bool dbuf = 2 * M <= cap;
   cx<double> a[2], c[2], d[2];
   if (active) {
 #pragma unroll
@@ -189,6 +193,7 @@ __device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double
     }
   }
   lds_barrier();
+  int base = 0;  // offset of the half holding the current level
   for (int h = 1; h < M; h *= 2) {
     cx<double> na[2], nc[2], nd[2];
     if (active) {
@@ -197,14 +202,14 @@ __device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double
         const int row = 2 * p + q;
         cx<double> am = zero, cm = zero, dm = zero, ap = zero, cp = zero, dp = zero;
         if (row - h >= 0) {
-          am = ra[row - h];
-          cm = rc[row - h];
-          dm = rd[row - h];
+          am = ra[base + row - h];
+          cm = rc[base + row - h];
+          dm = rd[base + row - h];
         }
         if (row + h < M) {
-          ap = ra[row + h];
-          cp = rc[row + h];
-          dp = rd[row + h];
+          ap = ra[base + row + h];
+          cp = rc[base + row + h];
+          dp = rd[base + row + h];
         }
         // eliminate x_{row-h} and x_{row+h} with the (unit-diagonal) rows there
         const cx<double> ib = crecip(one - a[q] * cm - c[q] * ap);
@@ -213,16 +218,20 @@ __device__ __forceinline__ void cn_pcr(cx<double> *ra, cx<double> *rc, cx<double
         nd[q] = (d[q] - a[q] * dm - c[q] * dp) * ib;
       }
     }
-    lds_barrier();
+    if (dbuf) {
+      base = M - base;
+    } else {
+      lds_barrier();
+    }
     if (active) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         a[q] = na[q];
         c[q] = nc[q];
         d[q] = nd[q];
-        ra[2 * p + q] = a[q];
-        rc[2 * p + q] = c[q];
-        rd[2 * p + q] = d[q];
+        ra[base + 2 * p + q] = a[q];
+        rc[base + 2 * p + q] = c[q];
+        rd[base + 2 * p + q] = d[q];
       }
     }
     lds_barrier();
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
       r0 = {as, bs - cs * at, cx<double>{0.0, 0.0} - cs * ct, ds - cs * dt};
     }
     cx<double> xs, xe;
-    cn_pcr(ra, rc, rd, M, p, active, r0, r1, xs, xe);
+    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
     if (active) {
       cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
                       ? sol + J.soff + static_cast<size_t>(st) * N
@@ -383,6 +392,17 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
   // z-invariant planes: the lane's material once, not a global load per row
   // and step on the step's critical path
   const bool fixed_mat = J.mat.mode == MAT_PLANES && J.mat.z_invariant;
+  // absorber factors of the lane's rows (1 outside the layer), once per call
+  double af[MM];
+  bool lane_absorbs = false;
+#pragma unroll
+  for (int j = 0; j < MM; ++j) {
+    af[j] = 1.0;
+    if (j < len && absorbing(s + j, N, npix)) {
+      af[j] = absorb_factor(s + j, N, npix, ainv);
+      lane_absorbs = true;
+    }
+  }
   double fd[MM], fb[MM];
 #pragma unroll
   for (int j = 0; j < MM; ++j) {
@@ -475,7 +495,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
       r0 = {as, bs - cs * at, zero - cs * ct, ds - cs * dt};
     }
     cx<double> xs, xe;
-    cn_pcr(ra, rc, rd, M, p, active, r0, r1, xs, xe);
+    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
     if (active) {
       // back substitution from the chunk's end
       cx<double> xn = xe;
@@ -495,7 +515,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
 #pragma unroll
       for (int j = 0; j < MM; ++j) {
         if (j < len) {
-          if (absorbing(s + j, N, npix)) u[j] = scale(u[j], absorb_factor(s + j, N, npix, ainv));
+          if (lane_absorbs) u[j] = scale(u[j], af[j]);
           if (sc) sc[s + j] = cvt<R>(u[j]);
         }
       }
