@@ -527,6 +527,161 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
     if (j < len) fld[s + j] = cvt<R>(u[j]);
 }
 
+// CN march for chunks of 5..MM rows (N <= MM * CN_MAX_P).  The field stays in
+// registers as in k_cn_reg; the elimination coefficients of the chunk rows go
+// to the job's scratch (coalesced [row][lane]) and are read back once, by the
+// back substitution.  The reduced equation of the chunk's first row needs the
+// backward running values (at, ct, dt) of k_cn_reg; here they come from the
+// forward pass: the backward recurrence v_j = h_j - ch_j v_{j+1} is affine, so
+// its composition over rows 1..len-2 is carried forward as v_1 = P + Q v
+// (P <- P + Q h_j, Q <- -Q ch_j), with no second pass over the rows.
+constexpr int CN_BIG_ROWS = 16;
+
+template <class R, int MM>
+__global__ __launch_bounds__(CN_MAX_P) void k_cn_big(const CnJob *__restrict__ jobs, cx<R> *fields, char *scratch,
+                                                     cx<R> *sol, unsigned first, unsigned nsteps)
+{
+  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
+  __shared__ uint64_t mask[TRI_MASK_WORDS];
+  const CnJob &J = jobs[blockIdx.x];
+  const int p = threadIdx.x, N = J.nx, P = J.P, bc = J.bc, m = J.m;
+  const bool active = p < P;
+  cx<R> *fld = fields + J.foff;
+  const double k = J.k, dz = J.dz, dx = J.dx, xmin = J.xmin, zmin = J.zmin;
+  const double rho = dz / (k * dx * dx);
+  const double r = k * dz;
+  const unsigned npix = J.absorb_npix;
+  const double ainv = J.absorb_inv;
+  const size_t soff = J.soff;
+  const int nz = J.nz;
+  const int len0 = N / P, rem = N % P;
+  const int s = p * len0 + min(p, rem);
+  const int len = active ? len0 + (p < rem ? 1 : 0) : 0;  // 2 <= len <= m <= MM for active lanes
+  const int M = 2 * P;
+  const int nw = J.mat.mode == MAT_CSG ? (J.mat.n_parts + 63) >> 6 : 0;
+  const cx<double> zero{0.0, 0.0}, one{1.0, 0.0};
+  // scratch: [field copy][ah][ch][dh], each coefficient array m rows x P lanes
+  char *base = scratch + J.scoff;
+  const size_t cst = align256(static_cast<size_t>(m) * P * 16);
+  cx<double> *ca = reinterpret_cast<cx<double> *>(base + align256(static_cast<size_t>(N) * sizeof(cx<R>)));
+  cx<double> *cc = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(ca) + cst);
+  cx<double> *cd = reinterpret_cast<cx<double> *>(reinterpret_cast<char *>(cc) + cst);
+
+  cx<double> u[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) u[j] = j < len ? cvt<double>(fld[s + j]) : zero;
+
+  for (unsigned it = 0; it < nsteps; ++it) {
+    const unsigned st = first + it;
+    const double z = coord(zmin, st, dz);
+    if (J.mat.mode == MAT_CSG) csg_cull(J.mat, mask, nw, xmin, coord(xmin, N - 1, dx), 0.0, 0.0, z - dz, z);
+    cx<double> ulast = u[0], u2last = u[0];
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+      if (j == len - 1) ulast = u[j];
+      if (j == len - 2) u2last = u[j];
+    }
+    if (active) {
+      edge_lo[p] = u[0];
+      edge_hi[p] = ulast;
+    }
+    lds_barrier();
+    const cx<double> ul = (active && p > 0) ? edge_hi[p - 1] : zero;
+    const cx<double> ur = (active && p < P - 1) ? edge_lo[p + 1] : zero;
+
+    TriRow r0{}, r1{};
+    if (active) {
+      auto row = [&](int j, cx<double> ul_, cx<double> u_, cx<double> ur_, cx<double> uin, cx<double> &ai,
+                     cx<double> &bi, cx<double> &ci, cx<double> &di) {
+        cn_row_v(J, s + j, st, z, rho, r, mask, nw, ul_, u_, ur_, uin, ai, bi, ci, di);
+      };
+      cx<double> as, bs, cs, ds;
+      row(0, ul, u[0], u[1], u[1], as, bs, cs, ds);
+      cx<double> ah = zero, ch = zero, dh = zero;
+      // forward composition of the backward recurrence over rows 1..len-2
+      cx<double> Pa = zero, Pd = zero, Q = one;
+#pragma unroll
+      for (int j = 1; j < MM; ++j) {
+        if (j < len) {
+          const int i = s + j;
+          const cx<double> un = (j + 1 < MM && j + 1 < len) ? u[j + 1 < MM ? j + 1 : 0] : ur;
+          const cx<double> uin = (i == N - 1) ? u2last : u[j];
+          cx<double> ai, bi, ci, di;
+          row(j, u[j - 1], u[j], un, uin, ai, bi, ci, di);
+          if (j == 1) {
+            const cx<double> ib = crecip(bi);
+            ah = ai * ib;
+            ch = ci * ib;
+            dh = di * ib;
+          } else {
+            const cx<double> iden = crecip(bi - ai * ch);
+            ah = (zero - ai * ah) * iden;
+            ch = ci * iden;
+            dh = (di - ai * dh) * iden;
+          }
+          if (j <= len - 2) {
+            ca[j * P + p] = ah;
+            cc[j * P + p] = ch;
+            cd[j * P + p] = dh;
+            Pa = Pa + Q * ah;
+            Pd = Pd + Q * dh;
+            Q = zero - Q * ch;
+          }
+        }
+      }
+      // last row of the chunk: ah x_s + x_e + ch x_{e+1} = dh; first row through
+      // at = Pa, dt = Pd, ct = -Q (base values 0, 0, -1 at the chunk's end)
+      r1 = {ah, one, ch, dh};
+      r0 = {as, bs - cs * Pa, cs * Q, ds - cs * Pd};
+    }
+    cx<double> xs, xe;
+    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
+    if (active) {
+      // back substitution from the chunk's end; rows below m - 1 exist in the
+      // scratch for every lane, so their loads need no per-lane guard
+      cx<double> xn = xe;
+#pragma unroll
+      for (int j = MM - 1; j >= 1; --j) {
+        if (j < m - 1) {
+          const cx<double> a0 = ca[j * P + p], c0 = cc[j * P + p], d0 = cd[j * P + p];
+          if (j < len - 1) {
+            u[j] = d0 - a0 * xs - c0 * xn;
+            xn = u[j];
+          }
+        }
+        if (j == len - 1) u[j] = xe;
+      }
+      u[0] = xs;
+      cx<R> *sc = (sol && soff != ~static_cast<size_t>(0) && static_cast<int>(st) < nz)
+                      ? sol + soff + static_cast<size_t>(st) * N
+                      : nullptr;
+#pragma unroll
+      for (int j = 0; j < MM; ++j) {
+        if (j < len) {
+          if (absorbing(s + j, N, npix)) u[j] = scale(u[j], absorb_factor(s + j, N, npix, ainv));
+          if (sc) sc[s + j] = cvt<R>(u[j]);
+        }
+      }
+    }
+    lds_barrier();  // edge slots are rewritten next step
+  }
+#pragma unroll
+  for (int j = 0; j < MM; ++j)
+    if (j < len) fld[s + j] = cvt<R>(u[j]);
+}
+
+int cn_big_rows() { return CN_BIG_ROWS; }
+
+template <class R>
+hipError_t launch_cn_big(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch, cx<R> *sol,
+                         unsigned first, unsigned nsteps, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_cn_big<R, CN_BIG_ROWS>), dim3(njobs), dim3(max_P), 0, s, jobs_dev, fields,
+                     static_cast<char *>(scratch), sol, first, nsteps);
+  return hipGetLastError();
+}
+
 size_t cn_job_scratch(int nx, int esz)
 {
   return cn_field_scratch(nx, esz, cn_partition(nx));
@@ -760,6 +915,10 @@ int cn_reg_rows() { return CN_REG_ROWS; }
 template hipError_t launch_cn_reg<float>(const CnJob *, int, int, cx<float> *, cx<float> *, unsigned, unsigned,
                                          hipStream_t);
 template hipError_t launch_cn_reg<double>(const CnJob *, int, int, cx<double> *, cx<double> *, unsigned,
+                                          unsigned, hipStream_t);
+template hipError_t launch_cn_big<float>(const CnJob *, int, int, cx<float> *, void *, cx<float> *, unsigned,
+                                         unsigned, hipStream_t);
+template hipError_t launch_cn_big<double>(const CnJob *, int, int, cx<double> *, void *, cx<double> *, unsigned,
                                           unsigned, hipStream_t);
 template hipError_t launch_cn_jobs<float>(const CnJob *, int, int, cx<float> *, void *, cx<float> *,
                                           unsigned, unsigned, hipStream_t);

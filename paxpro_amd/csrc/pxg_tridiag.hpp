@@ -76,6 +76,12 @@ template <class R>
 hipError_t launch_cn_reg(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
                          unsigned first, unsigned nsteps, hipStream_t s);
 int cn_reg_rows();
+// Register field, scratch coefficients: jobs whose chunks have more than
+// cn_reg_rows() and at most cn_big_rows() rows.
+template <class R>
+hipError_t launch_cn_big(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch, cx<R> *sol,
+                         unsigned first, unsigned nsteps, hipStream_t s);
+int cn_big_rows();
 template <class R>
 hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
 // Tile-Jacobi ADI half-step for diagonally dominant systems (pxg_adi.hip).
