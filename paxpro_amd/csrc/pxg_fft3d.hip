@@ -38,13 +38,15 @@ __device__ __noinline__ cd screen_from_values(double k, double dz, double delta,
 // (src/fftSolver3D.cpp:253-273): f(z0) + f(z1) + 2 sum_{i=1}^{n-2} f(z0 + i dz/n),
 // divided by 2n; the reference skips sample n-1 (PXG_FIX_REFR_SAMPLES restores it).
 // The points (z0, z1, then the interior samples, summed in that order) are
-// classified CK at a time in one walk over the culled parts each.
+// classified CK at a time in one walk over the culled parts each (CK = 4 in
+// pass X; 1 in the standalone map, where four points in flight doubled the
+// kernel's registers and halved its occupancy).
+template <int CK>
 __device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_t *mask, int nw,
                                                   unsigned n_refr, unsigned quirks, double x,
                                                   double y, double za, double zb, double &delta,
                                                   double &beta)
 {
-  constexpr int CK = 4;
   const double h = (zb - za) / n_refr;
   const unsigned last = (quirks & 2u) ? n_refr : n_refr - 1;
   const unsigned npts = last + 1;  // z0, z1 and samples 1 .. last-1
@@ -194,13 +196,23 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 // against it); lane cell m is cell(m) = (column, row), m < CPL.  The field-0
 // work-groups of pass X build it for their rows; k_material_map does it in a
 // launch of its own for plans with few fields.
-template <int CPL, class R, class CellOf>
+template <int CPL, int CK, class R, class CellOf>
 __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t *mask, int n, int xlo, int xhi,
                                                    int ylo, int yhi, CellOf cell)
 {
   constexpr int K = CPL < 2 ? CPL : 2;
   const double z1 = coord(a.zmin, a.step, a.dz);
   const double z0 = coord(a.zmin, a.step - 1, a.dz);
+  // the previous step's codes (the z0 plane) are loaded before the culling so
+  // that their latency overlaps it
+  uint16_t prev[CPL];
+  if (a.map_prev_valid) {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int2 cr = cell(q);
+      prev[q] = a.mcode[static_cast<size_t>(cr.x) * n + cr.y];
+    }
+  }
   int nw = (a.mat.n_parts + 63) >> 6;
   csg_cull(a.mat, mask, nw, coord(a.xmin, xlo, a.dx), coord(a.xmin, xhi, a.dx), coord(a.ymin, ylo, a.dy),
            coord(a.ymin, yhi, a.dy), fmin(z0, z1), fmax(z0, z1));
@@ -222,10 +234,7 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
       // the z0 plane is the previous step's z1 plane: its parts are in the map
       csg_eval_x2<K, false>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
-        const int2 cr = cell(m0 + q);
-        e0[q] = a.mcode[static_cast<size_t>(cr.x) * n + cr.y] & (MAP_CROSSING - 1u);
-      }
+      for (int q = 0; q < K; ++q) e0[q] = prev[m0 + q] & (MAP_CROSSING - 1u);
     } else {
       csg_eval_x2<K>(a.mat, mask, nw, xv, yv, z1, z0, e1, e0);
     }
@@ -251,8 +260,8 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
   while (__ballot(pending != 0) != 0) {
     const int2 cr = cell(pending ? __builtin_ctz(pending) : 0);
     double d, b;
-    crossing_integral(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, cr.x, a.dx), coord(a.ymin, cr.y, a.dy),
-                      z0, z1, d, b);
+    crossing_integral<CK>(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, cr.x, a.dx),
+                          coord(a.ymin, cr.y, a.dy), z0, z1, d, b);
     if (pending) {
       a.mxdb[static_cast<size_t>(cr.x) * n + cr.y] = cd{d, b};
       pending &= pending - 1;
@@ -269,8 +278,8 @@ __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   const int c = blockIdx.y, r0 = blockIdx.x * blockDim.x;
   const int r = r0 + static_cast<int>(threadIdx.x);
-  material_map_cells<1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
-                        [&](int) { return make_int2(c, r); });
+  material_map_cells<1, 1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
+                           [&](int) { return make_int2(c, r); });
 }
 
 // ---------------------------------------------------------------------------
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   if (a.map_in_pass_x && a.mat.mode == MAT_CSG && b == 0) {
     __shared__ uint64_t mask[MAX_MASK_WORDS];
     const int r0 = g * L;
-    material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
+    material_map_cells<8, 4>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
                           [&](int m) { return make_int2(ta + m * T, ra); });
   }
 }
