@@ -38,9 +38,9 @@ __device__ __noinline__ cd screen_from_values(double k, double dz, double delta,
 // (src/fftSolver3D.cpp:253-273): f(z0) + f(z1) + 2 sum_{i=1}^{n-2} f(z0 + i dz/n),
 // divided by 2n; the reference skips sample n-1 (PXG_FIX_REFR_SAMPLES restores it).
 // The points (z0, z1, then the interior samples, summed in that order) are
-// classified CK at a time in one walk over the culled parts each (CK = 4 in
-// pass X; 1 in the standalone map, where four points in flight doubled the
-// kernel's registers and halved its occupancy).
+// classified CK at a time in one walk over the culled parts each.  The map
+// uses it only when a cell has more samples than a wavefront has lanes
+// (map_border_cells spreads the samples over the lanes otherwise).
 template <int CK>
 __device__ __forceinline__ void crossing_integral(const MatDev &m, const uint64_t *mask, int nw,
                                                   unsigned n_refr, unsigned quirks, double x,
@@ -196,9 +196,81 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
 // against it); lane cell m is cell(m) = (column, row), m < CPL.  The field-0
 // work-groups of pass X build it for their rows; k_material_map does it in a
 // launch of its own for plans with few fields.
-template <int CPL, int CK, class R, class CellOf>
+// LDS scratch a wavefront of `lanes` lanes needs for its border cells.
+template <int CPL>
+constexpr int map_scratch_bytes(int lanes)
+{
+  return lanes * CPL * 8 + lanes * 16;
+}
+
+// Border cells of a wavefront (listed in `list`, nb of them): the interior
+// sample points of their crossing integrals (S per cell) are spread over the
+// lanes, 64 points per CSG walk, instead of S walks per cell by its own lane;
+// then one lane per cell sums its values in the order of crossing_integral
+// (z0, z1, then the samples), so each result is the per-cell integral.
+template <class R>
+__device__ __forceinline__ void map_border_cells(const FftArgs<R> &a, const uint64_t *mask, int nw, int n,
+                                                 const ushort4 *list, int nb, cd *vals, int lanes, double z0,
+                                                 double z1)
+{
+  const int lane = static_cast<int>(__lane_id());
+  const int last = (a.quirks & 2u) ? static_cast<int>(a.n_refr) : static_cast<int>(a.n_refr) - 1;
+  const int S = last > 1 ? last - 1 : 0;  // interior samples per cell
+  const double h = (z1 - z0) / a.n_refr;
+  const int G = S > 0 ? (lanes / S > 0 ? lanes / S : 0) : lanes;  // cells per walk
+  if (S > 0 && G == 0) {
+    // more samples than lanes: each lane walks its cells' samples in turn
+    for (int c = lane; c < nb; c += lanes) {
+      const ushort4 en = list[c];
+      double d, b;
+      crossing_integral<1>(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, en.x, a.dx),
+                           coord(a.ymin, en.y, a.dy), z0, z1, d, b);
+      a.mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{d, b};
+    }
+    return;
+  }
+  for (int g0 = 0; g0 < nb; g0 += G) {
+    if (S > 0 && lane < G * S) {
+      const int c = g0 + lane / S;
+      if (c < nb) {
+        const ushort4 en = list[c];
+        const double z = z0 + static_cast<double>(lane % S + 1) * h;
+        double d, b;
+        part_values(a.mat, csg_eval(a.mat, mask, nw, coord(a.xmin, en.x, a.dx), coord(a.ymin, en.y, a.dy), z), d,
+                    b);
+        vals[lane] = cd{d, b};
+      }
+    }
+    lds_sync<true>();
+    if (lane < G && g0 + lane < nb) {
+      const ushort4 en = list[g0 + lane];
+      double d0, b0, d1, b1;
+      part_values(a.mat, en.w, d0, b0);
+      part_values(a.mat, en.z, d1, b1);
+      double delta = 0.0, beta = 0.0;
+      delta += 1.0 * d0;
+      beta += 1.0 * b0;
+      delta += 1.0 * d1;
+      beta += 1.0 * b1;
+      for (int q = 0; q < S; ++q) {
+        const cd v = vals[lane * S + q];
+        delta += 2.0 * v.x;
+        beta += 2.0 * v.y;
+      }
+      delta /= (2.0 * a.n_refr);
+      beta /= (2.0 * a.n_refr);
+      a.mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{delta, beta};
+    }
+    lds_sync<true>();
+  }
+}
+
+// wscr: this wavefront's LDS scratch, map_scratch_bytes<CPL>(lanes) bytes,
+// free for the map's use (every lane of the work-group has passed a barrier
+// since any other use of it).
+template <int CPL, class R, class CellOf>
 __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t *mask, int n, int xlo, int xhi,
-                                                   int ylo, int yhi, CellOf cell)
+                                                   int ylo, int yhi, CellOf cell, char *wscr, int lanes)
 {
   constexpr int K = CPL < 2 ? CPL : 2;
   const double z1 = coord(a.zmin, a.step, a.dz);
@@ -219,7 +291,9 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
   bool any = false;
   for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
   if (!any || a.mat.is_ref) nw = 0;  // no part near this block: every cell is vacuum
-  unsigned crossing = 0;
+  ushort4 *list = reinterpret_cast<ushort4 *>(wscr);
+  cd *vals = reinterpret_cast<cd *>(wscr + lanes * CPL * 8);
+  int nb = 0;  // border cells listed so far (uniform)
 #pragma unroll 1
   for (int m0 = 0; m0 < CPL; m0 += K) {
     double xv[K], yv[K];
@@ -241,31 +315,29 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       unsigned code = static_cast<unsigned>(e1[q]);
+      bool border = false;
       if (e1[q] != e0[q]) {
         double d1, b1, d0, b0;
         part_values(a.mat, e1[q], d1, b1);
         part_values(a.mat, e0[q], d0, b0);
-        if (fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10) {
-          code |= MAP_CROSSING;
-          crossing |= 1u << (m0 + q);
-        }
+        border = fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10;
       }
       const int2 cr = cell(m0 + q);
+      // list the border cells of the wavefront (slots in lane order)
+      const uint64_t bal = __ballot(border);
+      if (border) {
+        code |= MAP_CROSSING;
+        const int slot = nb + __popcll(bal & ((1ull << __lane_id()) - 1ull));
+        list[slot] = make_ushort4(static_cast<unsigned short>(cr.x), static_cast<unsigned short>(cr.y),
+                                  static_cast<unsigned short>(e1[q]), static_cast<unsigned short>(e0[q]));
+      }
+      nb += __popcll(bal);
       a.mcode[static_cast<size_t>(cr.x) * n + cr.y] = static_cast<uint16_t>(code);
     }
   }
-  // border cells: the whole wavefront runs the loop while any lane has one
-  // left, so that the CSG walk stays in uniform control flow
-  unsigned pending = crossing;
-  while (__ballot(pending != 0) != 0) {
-    const int2 cr = cell(pending ? __builtin_ctz(pending) : 0);
-    double d, b;
-    crossing_integral<CK>(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, cr.x, a.dx),
-                          coord(a.ymin, cr.y, a.dy), z0, z1, d, b);
-    if (pending) {
-      a.mxdb[static_cast<size_t>(cr.x) * n + cr.y] = cd{d, b};
-      pending &= pending - 1;
-    }
+  if (nb > 0) {
+    lds_sync<true>();
+    map_border_cells(a, mask, nw, n, list, nb, vals, lanes, z0, z1);
   }
 }
 
@@ -276,10 +348,11 @@ template <class R>
 __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
 {
   __shared__ uint64_t mask[MAX_MASK_WORDS];
+  __shared__ __align__(16) char scr[map_scratch_bytes<1>(64)];
   const int c = blockIdx.y, r0 = blockIdx.x * blockDim.x;
   const int r = r0 + static_cast<int>(threadIdx.x);
-  material_map_cells<1, 1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
-                           [&](int) { return make_int2(c, r); });
+  material_map_cells<1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
+                           [&](int) { return make_int2(c, r); }, scr, static_cast<int>(blockDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -332,8 +405,14 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
   if (a.map_in_pass_x && a.mat.mode == MAT_CSG && b == 0) {
     __shared__ uint64_t mask[MAX_MASK_WORDS];
     const int r0 = g * L;
-    material_map_cells<8, 4>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
-                          [&](int m) { return make_int2(ta + m * T, ra); });
+    // the FFT's LDS is free (the culling's barriers follow every lane's last
+    // use of it); each wavefront takes a slice for its border cells
+    constexpr int WL = L * T < 64 ? L * T : 64;
+    static_assert((L * T / WL) * map_scratch_bytes<8>(WL) <= L * LdsLine<N>::STRIDE * static_cast<int>(sizeof(cd)),
+                  "map scratch must fit the FFT's LDS");
+    char *wscr = reinterpret_cast<char *>(lds) + (tid / WL) * map_scratch_bytes<8>(WL);
+    material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
+                             [&](int m) { return make_int2(ta + m * T, ra); }, wscr, WL);
   }
 }
 
