@@ -25,13 +25,17 @@ constexpr int CN_MAX_P = 512;
 constexpr int CN_REG_ROWS = 4;  // chunk rows held in registers by k_cn_reg
 constexpr int TRI_MASK_WORDS = 16;
 
-// P lanes (a power of two, at most CN_MAX_P and n/4) own balanced chunks: the
-// first n mod P chunks have n/P + 1 rows, the others n/P (at least 4), so
-// every chunk has distinct first and last rows.  m is the largest chunk.
+// P lanes own balanced chunks: the first n mod P chunks have n/P + 1 rows,
+// the others n/P.  P is the smallest power of two that brings the chunks to
+// at most 4 rows (the register-resident march), capped at CN_MAX_P, and at
+// most n/2 so that every chunk has distinct first and last rows.  m is the
+// largest chunk.  (Rounding P down instead left most sizes with chunks of 5-8
+// rows, outside the register march: 12.4 us/step at 584 cells against 8.1 at
+// 1024.)
 CnPartition cn_partition(int n)
 {
   int P = 1;
-  while (P * 2 <= CN_MAX_P && P * 2 <= n / 4) P *= 2;
+  while (P < CN_MAX_P && P * 2 <= n / 2 && (n + P - 1) / P > CN_REG_ROWS) P *= 2;
   return {P, (n + P - 1) / P};
 }
 
@@ -248,7 +252,8 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
                                                       char *scratch, cx<R> *sol, unsigned first,
                                                       unsigned nsteps)
 {
-  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  // two halves of 2 * CN_MAX_P rows each: PCR levels alternate between them
+  __shared__ cx<double> ra[4 * CN_MAX_P], rc[4 * CN_MAX_P], rd[4 * CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
   // a copy: the descriptor stays in registers instead of being re-read after
   // every barrier (barriers order global memory, so a reference reloads it)
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_jobs(const CnJob *__restrict__ 
       r0 = {as, bs - cs * at, cx<double>{0.0, 0.0} - cs * ct, ds - cs * dt};
     }
     cx<double> xs, xe;
-    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
+    cn_pcr(ra, rc, rd, 4 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
     if (active) {
       cx<R> *sc = (sol && J.soff != ~static_cast<size_t>(0) && static_cast<int>(st) < J.nz)
                       ? sol + J.soff + static_cast<size_t>(st) * N
@@ -361,7 +366,8 @@ template <class R, int MM>
 __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ jobs, cx<R> *fields, cx<R> *sol,
                                                      unsigned first, unsigned nsteps)
 {
-  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  // two halves of 2 * CN_MAX_P rows each: PCR levels alternate between them
+  __shared__ cx<double> ra[4 * CN_MAX_P], rc[4 * CN_MAX_P], rd[4 * CN_MAX_P];
   __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
   // The scalars of the descriptor the step loop uses are copied: barriers
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
       r0 = {as, bs - cs * at, zero - cs * ct, ds - cs * dt};
     }
     cx<double> xs, xe;
-    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
+    cn_pcr(ra, rc, rd, 4 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
     if (active) {
       // back substitution from the chunk's end
       cx<double> xn = xe;
@@ -541,7 +547,8 @@ template <class R, int MM>
 __global__ __launch_bounds__(CN_MAX_P) void k_cn_big(const CnJob *__restrict__ jobs, cx<R> *fields, char *scratch,
                                                      cx<R> *sol, unsigned first, unsigned nsteps)
 {
-  __shared__ cx<double> ra[2 * CN_MAX_P], rc[2 * CN_MAX_P], rd[2 * CN_MAX_P];
+  // two halves of 2 * CN_MAX_P rows each: PCR levels alternate between them
+  __shared__ cx<double> ra[4 * CN_MAX_P], rc[4 * CN_MAX_P], rd[4 * CN_MAX_P];
   __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
   const CnJob &J = jobs[blockIdx.x];
@@ -636,7 +643,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_big(const CnJob *__restrict__ j
       r0 = {as, bs - cs * Pa, cs * Q, ds - cs * Pd};
     }
     cx<double> xs, xe;
-    cn_pcr(ra, rc, rd, 2 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
+    cn_pcr(ra, rc, rd, 4 * CN_MAX_P, M, p, active, r0, r1, xs, xe);
     if (active) {
       // back substitution from the chunk's end; rows below m - 1 exist in the
       // scratch for every lane, so their loads need no per-lane guard

@@ -32,13 +32,13 @@ def slab_planes(x, lo, hi, core=(0.0, 0.0), clad=(4.5e-5, 3.5e-6)):
 
 
 @pytest.mark.parametrize("nx,nsteps,absorb", [(1024, 200, 0), (300, 100, 50), (5, 20, 0), (2, 10, 0),
-                                               (4700, 20, 50), (9000, 10, 0)])
+                                               (3000, 20, 50), (4700, 20, 50), (9000, 10, 0)])
 def test_cn_slab_vs_oracle(ctx, nx, nsteps, absorb):
     """Config-0 style slab waveguide (host-sampled material), plane wave, TBC.
     The sizes reach every march kernel: chunks of at most 4 rows held in
-    registers (1024, 5, 2), chunks in scratch (300: 64 chunks of 4-5 rows;
-    9000: 512 chunks of 17-18 rows) and 9-16 rows with the field in registers
-    (4700: 512 chunks of 9-10 rows)."""
+    registers (1024; 300: 128 chunks of 2-3 rows; 5, 2), chunks in scratch
+    (3000: 512 chunks of 5-6 rows; 9000: 17-18 rows) and 9-16 rows with the
+    field in registers (4700: 512 chunks of 9-10 rows)."""
     announce("cn slab", nx, nsteps, absorb)
     k, dz = K_01, 500.0
     xmin, dx = -500.0, 1000.0 / (nx - 1)
