@@ -342,17 +342,18 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
 }
 
 // The same map in a launch of its own, for plans with few fields (inside pass
-// X the field-0 work-groups would carry all of it): one cell per lane, a
-// work-group covers up to 64 rows of one column.
-template <class R>
+// X the field-0 work-groups would carry all of it): a work-group is one
+// wavefront over 64 * CPL rows of one column, CPL cells per lane.
+template <class R, int CPL>
 __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
 {
   __shared__ uint64_t mask[MAX_MASK_WORDS];
-  __shared__ __align__(16) char scr[map_scratch_bytes<1>(64)];
-  const int c = blockIdx.y, r0 = blockIdx.x * blockDim.x;
+  __shared__ __align__(16) char scr[map_scratch_bytes<CPL>(64)];
+  const int lanes = static_cast<int>(blockDim.x);
+  const int c = blockIdx.y, r0 = blockIdx.x * lanes * CPL;
   const int r = r0 + static_cast<int>(threadIdx.x);
-  material_map_cells<1>(a, mask, n, c, c, r0, r0 + static_cast<int>(blockDim.x) - 1,
-                           [&](int) { return make_int2(c, r); }, scr, static_cast<int>(blockDim.x));
+  material_map_cells<CPL>(a, mask, n, c, c, r0, r0 + lanes * CPL - 1,
+                          [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,10 +567,16 @@ hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 {
-  // one wavefront per work-group: short column strips keep the culled part
-  // lists (and the CSG walks of border cells) short
-  const int rows = n < 64 ? n : 64;  // n is a power of two >= 16
-  hipLaunchKernelGGL(k_material_map<R>, dim3(n / rows, n), dim3(rows), 0, s, a, n);
+  // One wavefront per work-group: short column strips keep the culled part
+  // lists short.  From n = 128 on a strip is 128 rows, two cells per lane:
+  // one culling per 128 cells instead of 64.
+  // (four cells per lane took 98 registers, 4 waves per SIMD, and was slower)
+  if (n >= 128) {
+    hipLaunchKernelGGL((k_material_map<R, 2>), dim3(n / 128, n), dim3(64), 0, s, a, n);
+  } else {
+    const int rows = n < 64 ? n : 64;  // n is a power of two >= 16
+    hipLaunchKernelGGL((k_material_map<R, 1>), dim3(n / rows, n), dim3(rows), 0, s, a, n);
+  }
   return hipGetLastError();
 }
 
