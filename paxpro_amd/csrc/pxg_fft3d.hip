@@ -14,6 +14,7 @@
 // pass leaving the field in real space.  Each pass reads and writes the field
 // once: 4 x sizeof(complex) bytes per cell and step.
 #include "pxg_fft.hpp"
+#include "pxg_fp64math.hpp"
 #include "pxg_internal.hpp"
 
 namespace pxg {
@@ -21,16 +22,14 @@ namespace pxg {
 constexpr int MAX_MASK_WORDS = 16;  // CSG parts per material: up to 1024
 
 // Phase-screen factor exp(-k (beta + i delta) dz) / N^2 from FP64 material
-// values, in FP64 (src/fftSolver3D.cpp:154).  Out of line on purpose: its
-// callers are rare per-cell loops, and inlined into them the exp/sincos
-// polynomial constants are hoisted out of the loop into some 50 VGPRs that
-// then count against the whole kernel (occupancy 3 instead of 4 waves/SIMD).
-__device__ __noinline__ cd screen_from_values(double k, double dz, double delta, double beta,
-                                                 double inv_norm)
+// values, in FP64 (src/fftSolver3D.cpp:154).  Its callers are rare per-cell
+// loops; exp_fp64/sincos_fp64 keep it inline without many registers.
+__device__ __forceinline__ cd screen_from_values(double k, double dz, double delta, double beta,
+                                                   double inv_norm)
 {
-  const double amp = exp((-k * beta) * dz);
+  const double amp = exp_fp64((-k * beta) * dz);
   double s, c;
-  sincos((-k * delta) * dz, &s, &c);
+  sincos_fp64((-k * delta) * dz, s, c);
   return {c * amp * inv_norm, s * amp * inv_norm};
 }
 
