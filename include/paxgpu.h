@@ -213,6 +213,23 @@ int pxg_step_profiled(pxg_plan plan, unsigned first_step, unsigned nsteps, pxg_k
 /* Device pointer of field b (advanced use: zero-copy serving, benchmarks). */
 int pxg_field_device_ptr(pxg_plan plan, int b, void **dptr);
 
+/* ---- far field (3D plans: FFT3D, ADI) ----------------------------------------- */
+/* Replaces post::FarField::result(const Solver&, arma::mat&) (src/farFieldPost.cpp:141-230)
+ * as GenericScattering::getFarField calls it (src/genericScattering.cpp:235-238): the
+ * scattered intensity of field b minus field ref_b of plan ref (the vacuum run; ref may
+ * be NULL), zero padding, signal length M = max(pad, nx), angular window
+ * [phi_min, phi_max] degrees mapped by farFieldAngleToIndx (src/farFieldPost.cpp:80-103)
+ * with dy for rows and dx for columns and the wavenumber of field b, |FFT|^2 / M, then
+ * PostProcessingModule::resizeMatrix (src/postProcessing.cpp:29-49) when export_rows and
+ * export_cols are both nonzero (GenericScattering sets 512 x 512).  M must be a power of
+ * two (the reference accepts any length; other lengths return PXG_ERR_UNSUPPORTED).
+ * pxg_far_field_dims gives the result size (rows, cols); pxg_far_field writes it to out
+ * (host, column-major doubles) after the plans' pending work. */
+int pxg_far_field_dims(pxg_plan plan, int b, unsigned pad, double phi_min, double phi_max,
+                       unsigned export_rows, unsigned export_cols, int dims[2]);
+int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, double phi_min,
+                  double phi_max, unsigned export_rows, unsigned export_cols, double *out);
+
 /* ---- batched tridiagonal solve ---------------------------------------------- */
 /* Replaces ThomasAlgorithm::solve<cdouble> (include/thomasAlgorithm.tpp:2-32) for
  * many independent systems: diag/sub/rhs are host complex128 arrays of nsys

@@ -291,3 +291,77 @@ def rel_l2_intensity(a, ref):
     ia = np.abs(np.asarray(a)) ** 2
     ir = np.abs(np.asarray(ref)) ** 2
     return float(np.linalg.norm(ia - ir) / np.linalg.norm(ir))
+
+
+# ---------------------------------------------------------------------- far field
+def far_field_angle_index(angle_deg, size, k, step):
+    """post::FarField::farFieldAngleToIndx (src/farFieldPost.cpp:80-103): index of a
+    scattering angle in the fftshifted spectrum of `size` points, spacing `step`."""
+    pi = np.arccos(-1.0)
+    v = size * k * step * np.sin(angle_deg * pi / 180.0) / (2.0 * pi)
+    if abs(v) >= size // 2:  # covers values beyond the int range as well
+        return size - 1 if angle_deg > 0.0 else 0
+    n = int(v)  # C++ conversion: truncation toward zero
+    if abs(n) >= size // 2:
+        return size - 1 if angle_deg > 0.0 else 0
+    return n + size // 2
+
+
+def far_field_window(n, pad, k, dx, dy, phi_min, phi_max):
+    """Signal length and the (row, column) index windows of the far field
+    (src/farFieldPost.cpp:141-156,193-196): rows follow y (dy), columns x (dx)."""
+    M = max(pad, n)
+    ylo, yhi = far_field_angle_index(phi_min, M, k, dy), far_field_angle_index(phi_max, M, k, dy)
+    xlo, xhi = far_field_angle_index(phi_min, M, k, dx), far_field_angle_index(phi_max, M, k, dx)
+    return M, (ylo, yhi), (xlo, xhi)
+
+
+def resize_matrix(orig, rows, cols):
+    """PostProcessingModule::resizeMatrix (src/postProcessing.cpp:29-49): block means
+    over inclusive, truncated index ranges (neighbouring blocks share a row or
+    column); unchanged when either export size reaches the matrix size."""
+    if rows >= orig.shape[0] or cols >= orig.shape[1]:
+        return orig.copy()
+    dX = orig.shape[1] / (cols + 1.0)
+    dY = orig.shape[0] / (rows + 1.0)
+    out = np.empty((rows, cols), order="F")
+    for i in range(cols):
+        for j in range(rows):
+            blk = orig[int(j * dY):int((j + 1) * dY) + 1, int(i * dX):int((i + 1) * dX) + 1]
+            out[j, i] = blk.mean(axis=0).mean()  # mean of the column means
+    return out
+
+
+def _ref_fftshift(a, axis):
+    """post::FarField::fftshift (src/farFieldPost.cpp:60-71): swaps element i with
+    i + N/2 for i < N/2 (for odd N the last element stays)."""
+    N = a.shape[axis]
+    h = N // 2
+    idx = np.arange(N)
+    idx[:h], idx[h:2 * h] = np.arange(h, 2 * h), np.arange(h)
+    return np.take(a, idx, axis=axis)
+
+
+def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_cols=0):
+    """post::FarField::result for a 3D field (src/farFieldPost.cpp:141-230) with the
+    default zero padding: subtract the reference field, FFT every column zero-padded
+    at its end to the signal length M, fftshift, keep the rows of the y window; then
+    the same along the rows of that, keeping the x window; |.|^2 / M; then the
+    export resize when export sizes are set (GenericScattering sets 512 x 512).
+    The signal length is max(pad, n): the result matrix is taken to start empty.
+    sol, ref: (Ny rows, Nx cols) complex arrays; ref may be None."""
+    s = np.array(sol, dtype=np.complex128)
+    if ref is not None:
+        s = s - np.asarray(ref, dtype=np.complex128)
+    n = max(s.shape)
+    M, (ylo, yhi), (xlo, xhi) = far_field_window(n, pad, k, dx, dy, phi_min, phi_max)
+    if yhi < ylo or xhi < xlo:
+        raise ValueError("angle window is empty")
+    ft = _ref_fftshift(np.fft.fft(s, n=M, axis=0), 0)[ylo:yhi + 1, :]
+    # the row pass copies temporary.row(i).t(): Armadillo's t() conjugates complex
+    # values (src/farFieldPost.cpp:213), which mirrors the x axis of the pattern
+    ft = _ref_fftshift(np.fft.fft(np.conj(ft), n=M, axis=1), 1)[:, xlo:xhi + 1]
+    res = np.asfortranarray(np.abs(ft) ** 2 / M)
+    if export_rows and export_cols:
+        res = resize_matrix(res, export_rows, export_cols)
+    return res

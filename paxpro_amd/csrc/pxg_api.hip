@@ -1538,6 +1538,146 @@ int pxg_field_device_ptr(pxg_plan P, int b, void **dptr)
   });
 }
 
+// ---- far field ----------------------------------------------------------------
+namespace {
+
+// post::FarField::farFieldAngleToIndx (src/farFieldPost.cpp:80-103), same
+// operation order; |v| >= size/2 is tested before the conversion to int, which
+// gives the reference's result without converting values beyond the int range.
+unsigned ff_angle_index(double angle, unsigned size, double k, double step)
+{
+  const double v = size * k * step * std::sin(angle * PI / 180.0) / (2.0 * PI);
+  if (!(std::fabs(v) < size / 2)) return angle > 0.0 ? size - 1 : 0;
+  const int n = static_cast<int>(v);
+  if (static_cast<unsigned>(std::abs(n)) >= size / 2) return angle > 0.0 ? size - 1 : 0;
+  return static_cast<unsigned>(n + static_cast<int>(size / 2));
+}
+
+struct FfWindow {
+  unsigned M = 0, ylo = 0, yhi = 0, xlo = 0, xhi = 0;
+  int rows = 0, cols = 0;     // before the export resize
+  bool resize = false;
+  int out_rows = 0, out_cols = 0;
+};
+
+FfWindow ff_window(pxg_plan P, int b, unsigned pad, double phi_min, double phi_max, unsigned er,
+                   unsigned ec)
+{
+  check_b(P, b);
+  if (P->is_jobs || (P->p.kind != PXG_FFT3D && P->p.kind != PXG_ADI))
+    fail(PXG_ERR_INVALID, "the far field needs a 3D plan (FFT3D or ADI)");
+  FfWindow W;
+  const unsigned n = static_cast<unsigned>(P->p.nx);
+  // signalLength = max(pad, rows, cols) (src/farFieldPost.cpp:146-150)
+  W.M = std::max(pad, n);
+  if (W.M < 8 || W.M > (1u << 24) || (W.M & (W.M - 1)))
+    fail(PXG_ERR_UNSUPPORTED, "far-field signal length %u: powers of two from 8 to 2^24 only", W.M);
+  const double k = P->k_host[b];
+  W.ylo = ff_angle_index(phi_min, W.M, k, P->p.dy);
+  W.yhi = ff_angle_index(phi_max, W.M, k, P->p.dy);
+  W.xlo = ff_angle_index(phi_min, W.M, k, P->p.dx);
+  W.xhi = ff_angle_index(phi_max, W.M, k, P->p.dx);
+  if (W.yhi < W.ylo || W.xhi < W.xlo)
+    fail(PXG_ERR_INVALID, "empty angular window [%g, %g] degrees", phi_min, phi_max);
+  W.rows = static_cast<int>(W.yhi - W.ylo + 1);
+  W.cols = static_cast<int>(W.xhi - W.xlo + 1);
+  // resizeMatrix keeps the matrix when either export size reaches it
+  W.resize = er > 0 && ec > 0 && !(er >= static_cast<unsigned>(W.rows) ||
+                                   ec >= static_cast<unsigned>(W.cols));
+  W.out_rows = W.resize ? static_cast<int>(er) : W.rows;
+  W.out_cols = W.resize ? static_cast<int>(ec) : W.cols;
+  return W;
+}
+
+std::vector<cx<double>> twiddle_table(unsigned n)
+{
+  std::vector<cx<double>> tw(n);
+  for (unsigned m = 0; m < n; ++m) {
+    const double a = 2.0 * PI * (static_cast<double>(m) / n);
+    tw[m] = {std::cos(a), -std::sin(a)};
+  }
+  return tw;
+}
+
+}  // namespace
+
+int pxg_far_field_dims(pxg_plan P, int b, unsigned pad, double phi_min, double phi_max,
+                       unsigned export_rows, unsigned export_cols, int dims[2])
+{
+  return guarded([&] {
+    if (!dims) fail(PXG_ERR_INVALID, "null output");
+    const FfWindow W = ff_window(P, b, pad, phi_min, phi_max, export_rows, export_cols);
+    dims[0] = W.out_rows;
+    dims[1] = W.out_cols;
+  });
+}
+
+int pxg_far_field(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, double phi_min,
+                  double phi_max, unsigned export_rows, unsigned export_cols, double *out)
+{
+  return guarded([&] {
+    const FfWindow W = ff_window(P, b, pad, phi_min, phi_max, export_rows, export_cols);
+    if (!out) fail(PXG_ERR_INVALID, "null output");
+    if (!P->field_set) fail(PXG_ERR_STATE, "field not set (pxg_set_field)");
+    const int n = P->p.nx;
+    if (R) {
+      check_b(R, rb);
+      if (R->is_jobs || (R->p.kind != PXG_FFT3D && R->p.kind != PXG_ADI) || R->p.nx != n)
+        fail(PXG_ERR_INVALID, "the reference must be a 3D field of the same size");
+      if (R->ctx->device != P->ctx->device) fail(PXG_ERR_INVALID, "reference plan on another device");
+      if (!R->field_set) fail(PXG_ERR_STATE, "reference field not set");
+    }
+    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+    if (R && R != P) hip_check(hipStreamSynchronize(R->stream), "reference sync");
+    int np = 8;
+    while (np < n) np *= 2;
+    if (np > 4096) fail(PXG_ERR_UNSUPPORTED, "far field of fields above 4096 points per side");
+    const size_t need = static_cast<size_t>(W.M) * 16 + static_cast<size_t>(np) * 16 +
+                        static_cast<size_t>(W.rows) * n * 16 + static_cast<size_t>(W.rows) * W.cols * 8 +
+                        (W.resize ? static_cast<size_t>(W.out_rows) * W.out_cols * 8 : 0);
+    if (P->ctx->reserved.load() + need > P->ctx->budget)
+      fail(PXG_ERR_BUDGET, "far field needs %zu bytes of scratch", need);
+    hipStream_t s = P->stream;
+    DBuf twm, twn, tmp, res, rsz;
+    upload_cx<double>(twm, twiddle_table(W.M), s);
+    upload_cx<double>(twn, twiddle_table(static_cast<unsigned>(np)), s);
+    tmp.alloc(static_cast<size_t>(W.rows) * n * sizeof(cd));
+    res.alloc(static_cast<size_t>(W.rows) * W.cols * sizeof(double));
+    FfArgs a{};
+    a.field = P->field.as<char>() + P->foff[b] * P->esz;
+    a.field_c64 = P->esz == 8;
+    if (R) {
+      a.ref = R->field.as<char>() + R->foff[rb] * R->esz;
+      a.ref_c64 = R->esz == 8;
+    }
+    a.n = n;
+    a.M = static_cast<int>(W.M);
+    a.D = static_cast<int>(W.M) / np;
+    a.tw_m = twm.as<cd>();
+    a.tw_np = twn.as<cd>();
+    a.ylo = static_cast<int>(W.ylo);
+    a.ny = W.rows;
+    a.xlo = static_cast<int>(W.xlo);
+    a.nx = W.cols;
+    a.tmp = tmp.as<cd>();
+    a.out = res.as<double>();
+    a.inv_len = 1.0 / W.M;
+    hip_check(launch_far_field(a, np, s), "far field");
+    const double *src = res.as<double>();
+    if (W.resize) {
+      rsz.alloc(static_cast<size_t>(W.out_rows) * W.out_cols * sizeof(double));
+      hip_check(launch_ff_resize(res.as<double>(), W.rows, W.cols, rsz.as<double>(), W.out_rows,
+                                 W.out_cols, s),
+                "far field resize");
+      src = rsz.as<double>();
+    }
+    hip_check(hipMemcpyAsync(out, src, static_cast<size_t>(W.out_rows) * W.out_cols * sizeof(double),
+                             hipMemcpyDeviceToHost, s),
+              "far field download");
+    hip_check(hipStreamSynchronize(s), "far field sync");
+  });
+}
+
 int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
                        double *rhs)
 {

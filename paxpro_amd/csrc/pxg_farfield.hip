@@ -1,0 +1,153 @@
+// Far-field intensity of a 3D exit field: post::FarField::result
+// (src/farFieldPost.cpp:141-230) as GenericScattering uses it
+// (src/genericScattering.cpp:148-213,235-238).
+//
+//   D = field - reference (the vacuum run), n x n, column-major (rows = y)
+//   column pass: every column zero-padded at its end to the signal length M,
+//                forward FFT (unnormalised), fftshift, rows of the y window kept
+//   row pass:    every row of that, conjugated (Armadillo's t() of a complex
+//                row, src/farFieldPost.cpp:213), zero-padded to M, forward FFT,
+//                fftshift, columns of the x window kept, |.|^2 / M
+//   resize:      block means down to the export size
+//                (PostProcessingModule::resizeMatrix, src/postProcessing.cpp:29-49)
+//
+// A line holds at most n non-zero values, so its M-point DFT is computed as
+// D = M/NP transforms of NP = nextpow2(n) points: with f = D g + h,
+//   X[f] = sum_j x_j w_M^{h j} w_NP^{g j},
+// i.e. residue h pre-multiplies the line by w_M^{h j} and one NP-point FFT gives
+// every f = h (mod D).  Each (line, h) is one work-group running the LDS Stockham
+// FFT of the split-step passes; it writes only the outputs inside the window.
+// Pad lengths far beyond the field (the reference's examples use 1024 and 32768
+// for 512- and 1024-point fields) then need no transform longer than the field's.
+// Everything after the field load is FP64.
+#include "pxg_fft.hpp"
+#include "pxg_internal.hpp"
+
+namespace pxg {
+
+__device__ __forceinline__ cd ff_load(const void *p, int c64, size_t i)
+{
+  return c64 ? cvt<double>(static_cast<const cf *>(p)[i]) : static_cast<const cd *>(p)[i];
+}
+
+template <int NP>
+__global__ __launch_bounds__(NP / 8) void k_ff_cols(FfArgs a)
+{
+  constexpr int T = NP / 8;
+  constexpr bool WL = T <= 64;
+  __shared__ cd lds[LdsLine<NP>::SIZE];
+  const int t = threadIdx.x, c = blockIdx.x, h = blockIdx.y;
+  const TwBase<NP, double> twb = tw_base<NP>(a.tw_np, t);
+  cd v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * T;
+    if (j < a.n) {
+      const size_t i = static_cast<size_t>(c) * a.n + j;
+      cd u = ff_load(a.field, a.field_c64, i);
+      if (a.ref) u = u - ff_load(a.ref, a.ref_c64, i);
+      if (a.D > 1) u = u * a.tw_m[(static_cast<size_t>(h) * j) & (a.M - 1)];
+      v[m] = u;
+    } else {
+      v[m] = cd{0.0, 0.0};
+    }
+  }
+  fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const long f = static_cast<long>(a.D) * (t + m * T) + h;
+    const int r = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.ylo;  // fftshift, window
+    if (static_cast<unsigned>(r) < static_cast<unsigned>(a.ny))
+      a.tmp[static_cast<size_t>(c) * a.ny + r] = v[m];
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(NP / 8) void k_ff_rows(FfArgs a)
+{
+  constexpr int T = NP / 8;
+  constexpr bool WL = T <= 64;
+  __shared__ cd lds[LdsLine<NP>::SIZE];
+  const int t = threadIdx.x, r = blockIdx.x, h = blockIdx.y;
+  const TwBase<NP, double> twb = tw_base<NP>(a.tw_np, t);
+  cd v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * T;
+    if (j < a.n) {
+      cd u = conj(a.tmp[static_cast<size_t>(j) * a.ny + r]);
+      if (a.D > 1) u = u * a.tw_m[(static_cast<size_t>(h) * j) & (a.M - 1)];
+      v[m] = u;
+    } else {
+      v[m] = cd{0.0, 0.0};
+    }
+  }
+  fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const long f = static_cast<long>(a.D) * (t + m * T) + h;
+    const int col = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.xlo;
+    if (static_cast<unsigned>(col) < static_cast<unsigned>(a.nx)) {
+      const double mag = hypot(v[m].x, v[m].y);
+      a.out[static_cast<size_t>(col) * a.ny + r] = mag * mag * a.inv_len;
+    }
+  }
+}
+
+// resizeMatrix: output (j, i) is the mean of the column means of the block of
+// rows [j dY] .. [(j+1) dY] and columns [i dX] .. [(i+1) dX] (inclusive, indices
+// truncated), dY = rows/(er+1), dX = cols/(ec+1).
+__global__ __launch_bounds__(256) void k_ff_resize(const double *in, int rows, int cols,
+                                                   double *out, int er, int ec)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= er * ec) return;
+  const int j = idx % er, i = idx / er;
+  const double dX = cols / (ec + 1.0), dY = rows / (er + 1.0);
+  const int r0 = static_cast<int>(j * dY), r1 = static_cast<int>((j + 1) * dY);
+  const int c0 = static_cast<int>(i * dX), c1 = static_cast<int>((i + 1) * dX);
+  double s = 0.0;
+  for (int c = c0; c <= c1; ++c) {
+    double cs = 0.0;
+    for (int r = r0; r <= r1; ++r) cs += in[static_cast<size_t>(c) * rows + r];
+    s += cs / (r1 - r0 + 1);
+  }
+  out[idx] = s / (c1 - c0 + 1);
+}
+
+#define PXG_FF_CASE(NN)                                                                   \
+  case NN:                                                                                \
+    hipLaunchKernelGGL(k_ff_cols<NN>, dim3(a.n, a.D), dim3(NN / 8), 0, s, a);             \
+    if ((err = hipGetLastError()) != hipSuccess) return err;                              \
+    hipLaunchKernelGGL(k_ff_rows<NN>, dim3(a.ny, a.D), dim3(NN / 8), 0, s, a);            \
+    return hipGetLastError();
+
+hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s)
+{
+  hipError_t err = hipSuccess;
+  switch (np) {
+    PXG_FF_CASE(8)
+    PXG_FF_CASE(16)
+    PXG_FF_CASE(32)
+    PXG_FF_CASE(64)
+    PXG_FF_CASE(128)
+    PXG_FF_CASE(256)
+    PXG_FF_CASE(512)
+    PXG_FF_CASE(1024)
+    PXG_FF_CASE(2048)
+    PXG_FF_CASE(4096)
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,
+                            hipStream_t s)
+{
+  const int cells = er * ec;
+  hipLaunchKernelGGL(k_ff_resize, dim3((cells + 255) / 256), dim3(256), 0, s, in, rows, cols, out,
+                     er, ec);
+  return hipGetLastError();
+}
+
+}  // namespace pxg

@@ -56,6 +56,26 @@ template <class R>
 hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
                              cx<double> *cube, size_t cube_fstride, int slice, int batch,
                              hipStream_t s);
+// Far field of a 3D field (pxg_farfield.hip).  Windows are index ranges of the
+// fftshifted M-point spectrum; M = D * NP with NP the transform length.
+struct FfArgs {
+  const void *field;       // n x n, column-major
+  int field_c64;           // field stored as complex64 (else complex128)
+  const void *ref;         // subtracted first (vacuum run), or null
+  int ref_c64;
+  int n;                   // rows = columns of the field
+  int M, D;                // signal length, M / NP
+  const cd *tw_m;          // exp(-2 pi i m / M), m < M
+  const cd *tw_np;         // exp(-2 pi i m / NP), m < NP
+  int ylo, ny;             // row window (y): shifted indices ylo .. ylo + ny - 1
+  int xlo, nx;             // column window (x)
+  cd *tmp;                 // column-pass output, ny x n, column-major
+  double *out;             // ny x nx intensities, column-major
+  double inv_len;          // 1 / M
+};
+hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
+hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,
+                            hipStream_t s);
 // Lines per work-group for an LDS budget of lds_cap bytes (FP64 lines).
 int max_lines(int n, int lds_cap);
 // Stream of a plan and device of a context (server internals).

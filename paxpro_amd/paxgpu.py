@@ -39,6 +39,7 @@ EXPORTS = [
     "pxg_plan_set_lines", "pxg_plan_get_lines", "pxg_plan_autotune",
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
+    "pxg_far_field_dims", "pxg_far_field",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
@@ -134,6 +135,8 @@ def lib():
         L.pxg_server_destroy.argtypes = [P]
         L.pxg_set_job_material_curved_wg.argtypes = [P, I, P]
         L.pxg_job_solution_dims.argtypes = [P, I, P]
+        L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
+        L.pxg_far_field.argtypes = [P, I, P, I, U, D, D, U, U, P]
         _LIB = L
     return _LIB
 
@@ -250,6 +253,18 @@ class Plan:
     def set_wavenumbers(self, ks):
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         _check(lib().pxg_set_wavenumbers(self.h, _p(ks), ks.size))
+
+    def far_field(self, b=0, ref=None, ref_b=0, pad=1024, phi_min=-90.0, phi_max=90.0,
+                  export_rows=0, export_cols=0):
+        """Scattered far-field intensity of field b minus field ref_b of plan `ref`
+        (post::FarField::result, src/farFieldPost.cpp:141-230): rows follow y, columns x."""
+        dims = np.zeros(2, dtype=np.int32)
+        _check(lib().pxg_far_field_dims(self.h, b, pad, phi_min, phi_max, export_rows,
+                                        export_cols, _p(dims)))
+        out = np.empty((int(dims[0]), int(dims[1])), dtype=np.float64, order="F")
+        _check(lib().pxg_far_field(self.h, b, None if ref is None else ref.h, ref_b, pad, phi_min,
+                                   phi_max, export_rows, export_cols, _p(out)))
+        return out
 
     def material_none(self):
         _check(lib().pxg_set_material_none(self.h))

@@ -1,0 +1,131 @@
+"""CPU checks pinning the far-field restatement in oracle/pyoracle.py
+(post::FarField::result, src/farFieldPost.cpp:141-230; resizeMatrix,
+src/postProcessing.cpp:29-49) with closed forms: a point scatterer, plane waves
+along each axis (fftshift direction, row/column orientation, FFT sign, the
+mirrored x axis that Armadillo's conjugating t() gives the row pass), a tilted
+wave against the angle-to-index map, block means of simple matrices, and the
+form factor of a sphere (Fourier slice of a uniform ball)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+
+def full_window_k(dx):
+    # k dx = pi puts +-90 degrees exactly at |v| = M/2: the whole spectrum
+    return np.pi / dx
+
+
+def test_point_scatterer_is_flat_one_over_M():
+    n, M, dx = 8, 16, 0.5
+    sol = np.zeros((n, n), complex)
+    sol[3, 5] = 1.0
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0)
+    assert ff.shape == (M, M)
+    assert np.allclose(ff, 1.0 / M, rtol=1e-14, atol=0)
+
+
+def test_reference_is_subtracted():
+    rng = np.random.default_rng(0)
+    n, dx = 16, 0.5
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    ref = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    f1 = O.far_field(a + ref, ref, 32, full_window_k(dx), dx, dx, -90.0, 90.0)
+    f2 = O.far_field(a, None, 32, full_window_k(dx), dx, dx, -90.0, 90.0)
+    assert np.allclose(f1, f2, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("p", [1, 3, -2])
+def test_plane_wave_along_y_lands_on_its_row(p):
+    M = n = 16
+    dx = 0.5
+    j = np.arange(n)[:, None]
+    sol = np.exp(2j * np.pi * p * j / M) * np.ones((1, n))
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0)
+    r, c = np.unravel_index(np.argmax(ff), ff.shape)
+    # forward FFT e^{-2 pi i f j / M} peaks at f = p; fftshift puts f at f + M/2
+    assert (r, c) == ((p + M // 2) % M, M // 2)
+    assert ff[r, c] == pytest.approx(M ** 3, rel=1e-12)  # |M * M|^2 / M
+    ff[r, c] = 0
+    assert np.abs(ff).max() < 1e-18 * M ** 3
+
+
+@pytest.mark.parametrize("q", [1, 4])
+def test_plane_wave_along_x_is_mirrored(q):
+    M = n = 16
+    dx = 0.5
+    i = np.arange(n)[None, :]
+    sol = np.ones((n, 1)) * np.exp(2j * np.pi * q * i / M)
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0)
+    r, c = np.unravel_index(np.argmax(ff), ff.shape)
+    # the row pass transforms conj(row): the peak sits at -q, i.e. column M/2 - q
+    assert (r, c) == (M // 2, M // 2 - q)
+
+
+def test_tilted_wave_peaks_at_its_angle_index():
+    # a wave exp(i k sin(theta) y) along y scatters at theta: its spectral peak is
+    # the index farFieldAngleToIndx gives theta (a whole number of periods here)
+    M = n = 64
+    dy = 2.0
+    k = 2 * np.pi / 0.7
+    f0 = 5  # periods across the grid
+    sin_t = 2 * np.pi * f0 / (M * dy * k)
+    theta = np.degrees(np.arcsin(sin_t))
+    y = dy * np.arange(n)[:, None]
+    sol = np.exp(1j * k * sin_t * y) * np.ones((1, n))
+    ff = O.far_field(sol, None, M, k, dy, dy, -90.0, 90.0)
+    r = np.unravel_index(np.argmax(ff), ff.shape)[0]
+    ylo = O.far_field_angle_index(-90.0, M, k, dy)
+    assert r + ylo == O.far_field_angle_index(theta * (1 + 1e-9), M, k, dy) == f0 + M // 2
+
+
+def test_angle_index_edges():
+    # beyond the resolvable range the index clamps to the ends of the spectrum
+    assert O.far_field_angle_index(89.0, 64, 10.0, 1.0) == 63
+    assert O.far_field_angle_index(-89.0, 64, 10.0, 1.0) == 0
+    assert O.far_field_angle_index(0.0, 64, 10.0, 1.0) == 32
+    # truncation toward zero on both sides
+    k, dx, M = 1.0, 1.0, 64
+    a = np.degrees(np.arcsin(2 * np.pi * 2.5 / (M * k * dx)))
+    assert O.far_field_angle_index(a, M, k, dx) == 34
+    assert O.far_field_angle_index(-a, M, k, dx) == 30
+
+
+def test_resize_block_means():
+    A = np.full((40, 30), 2.5, order="F")
+    assert np.allclose(O.resize_matrix(A, 7, 5), 2.5)
+    r = np.arange(40)[:, None] + 1000.0 * np.arange(30)[None, :]
+    out = O.resize_matrix(r, 7, 5)
+    dY, dX = 40 / 8.0, 30 / 6.0
+    for j in range(7):
+        for i in range(5):
+            r0, r1 = int(j * dY), int((j + 1) * dY)
+            c0, c1 = int(i * dX), int((i + 1) * dX)
+            # the mean of a ramp over an inclusive block is its midpoint
+            assert out[j, i] == pytest.approx((r0 + r1) / 2 + 1000.0 * (c0 + c1) / 2, rel=1e-13)
+    # an export size at or above the matrix size keeps it
+    assert O.resize_matrix(r, 40, 5).shape == (40, 30)
+    assert O.resize_matrix(r, 7, 30).shape == (40, 30)
+
+
+def test_sphere_form_factor_first_minimum():
+    # weak phase object: the difference field is i*k*delta*t(x, y) with t the
+    # projected thickness of a ball of radius R; by the Fourier slice theorem its
+    # far field is the ball's 3D form factor on the plane qz = 0, whose first zero
+    # is at qR = 4.4934.  Along x the pattern is mirrored, a symmetric profile.
+    n, dx, R = 256, 1.0, 20.0
+    x = dx * (np.arange(n) - n / 2)
+    X, Y = np.meshgrid(x, x)
+    t = 2 * np.sqrt(np.clip(R ** 2 - X ** 2 - Y ** 2, 0, None))
+    sol = 1j * 1e-4 * t
+    M = 4096
+    k = np.pi / dx
+    ff = O.far_field(sol, None, M, k, dx, dx, -90.0, 90.0)
+    c = M // 2
+    for prof in (ff[c, c:], ff[c:, c], ff[c, c::-1]):
+        q = 2 * np.pi * np.arange(prof.size) / (M * dx)
+        i_min = np.argmin(prof[: int(6.0 / R / (2 * np.pi / (M * dx)))])
+        assert abs(q[i_min] * R - 4.4934) < 0.05
+        qr = q[1:40] * R
+        fac = (3 * (np.sin(qr) - qr * np.cos(qr)) / qr ** 3) ** 2
+        assert np.allclose(prof[1:40] / prof[0], fac, rtol=2e-2, atol=2e-4)
