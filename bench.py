@@ -426,7 +426,97 @@ def bench_served(args, ctx):
                 utilisation=round(st["busy_s"] / wall, 3), wall_s=round(wall, 3))
 
 
-WORKLOADS = {"cn2d": bench_cn2d, "adi3d": bench_adi3d, "cn_jobs": bench_cn_jobs, "served": bench_served}
+def bench_scatter(args, ctx):
+    """GenericScattering as Examples/sphere.cpp runs it (src/genericScattering.cpp:148-213):
+    a 500 nm sphere (delta 8.9e-6, beta 7e-7) on 512^2 over 3x the radius, 256 z-steps
+    across 2.1 radii, lambda 0.1569 nm, beam waist 400 radii; the reference (vacuum) run
+    over 3 steps, the sphere run, then the far field within 0.05 degrees with pad
+    length 1024 and the 512 x 512 export.  One job = both runs and the far field."""
+    n, r = 512, 500.0
+    steps = args.steps or 256
+    xmin, dx = -1.5 * r, 3.0 * r / n
+    zmin, zmax = -1.05 * r, 1.05 * r
+    dz, dz_ref = (zmax - zmin) / steps, (zmax - zmin) / 3.0
+    nz_ref = int(np.floor((zmax - zmin) / dz_ref + 1.0))
+    k = 2 * np.pi / 0.1569
+    x = xmin + dx * np.arange(n)
+    gx = np.exp(-(x / (400.0 * r)) ** 2)
+    u0 = np.asfortranarray(np.outer(gx, gx).astype(np.complex128))
+    shapes = np.zeros(1, P.SHAPE_DTYPE)
+    shapes[0]["type"], shapes[0]["op"], shapes[0]["T"], shapes[0]["p"] = 0, 0, np.eye(4).ravel(), (r, 0, 0)
+    parts = np.zeros(1, P.PART_DTYPE)
+    parts[0]["first_shape"], parts[0]["n_shapes"], parts[0]["op"] = 0, 1, 0
+    parts[0]["delta"], parts[0]["beta"] = 8.9e-6, 7e-7
+    modules = np.zeros(1, P.MODULE_DTYPE)
+    modules[0]["first_part"], modules[0]["n_parts"] = 0, 1
+    csg = (shapes, parts, modules)
+    pad, phi = 1024, 0.05
+    mk = lambda nz, d: P.problem(P.FFT3D, n, nz, dx, d, k, xmin=xmin, zmin=zmin, precision=P.PREC_FP64)
+    vac, run = P.Plan(ctx, mk(nz_ref, dz_ref)), P.Plan(ctx, mk(steps + 1, dz))
+    vac.material_csg(*csg, reference_run=True)
+    run.material_csg(*csg)
+
+    def job():
+        vac.set_field(u0)
+        run.set_field(u0)
+        vac.sync()
+        run.sync()
+        t0 = time.perf_counter()
+        vac.step(1, nz_ref - 1)
+        run.step(1, steps)
+        ff = run.far_field(0, vac, 0, pad, -phi, phi, 512, 512)
+        return time.perf_counter() - t0, ff
+
+    for _ in range(max(1, min(args.warmup, 2))):
+        job()
+    times = []
+    for _ in range(5):
+        t, ff = job()
+        times.append(t)
+    t_job = float(np.median(times))
+    ft = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        run.far_field(0, vac, 0, pad, -phi, phi, 512, 512)
+        ft.append(time.perf_counter() - t0)
+    t_ff = float(np.median(ft))
+    run.set_field(u0)
+    _, stats = run.step_profiled(1, steps)
+    log(f"scatter: job {t_job * 1e3:.2f} ms, far field {t_ff * 1e3:.2f} ms (call, incl. tables and copy-out)")
+    for name, st in stats.items():
+        log(f"  {name}: {st['launches']} launches, mean {st['mean_ms'] * 1e3:.1f} us")
+    vac.close()
+    run.close()
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        g = O.grid(xmin, xmin + (n - 1) * dx, dx, zmin, zmax, dz, k)
+        g_ref = O.grid(xmin, xmin + (n - 1) * dx, dx, zmin, zmax, dz_ref, k)
+        t0 = time.perf_counter()
+        O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, 4)
+        one = (time.perf_counter() - t0) / 4
+        ks = max(4, min(steps, int(args.cpu_seconds / max(one, 1e-4))))
+        t0 = time.perf_counter()
+        ref = O.fft3d_run(g_ref, O.CSGMaterial(*csg, reference_run=True), u0, 1, nz_ref - 1)
+        sol = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, ks)
+        O.far_field(sol, ref, pad, k, dx, dx, -phi, phi, 512, 512)
+        dt = time.perf_counter() - t0
+        job_s = dt + one * (steps - ks)  # the remaining steps at the measured rate
+        cpu = {"value": round(1.0 / job_s, 5), "unit": "jobs/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"oracle: reference run, first {ks} of {steps} sphere steps, far field "
+                         f"({dt:.1f} s); the other {steps - ks} steps at the measured per-step rate"}
+    return line("GenericScattering jobs/s (Examples/sphere.cpp: reference run + sphere run + far field)",
+                1.0 / t_job, "jobs/s", steps, args.warmup, t_job * 1e3 / steps, "fp64",
+                "synthetic: Gaussian beam + CSG sphere",
+                {"workload": "GenericScattering sphere r=500 nm, 512x512 over 1500 nm (512 nodes: the "
+                             "example's 513 trimmed to a power of two), 256 z-steps, lambda 0.1569 nm, "
+                             "waist 200 um, far field +-0.05 deg, pad 1024, export 512x512"},
+                roofline_of(stats), cpu, job_ms=round(t_job * 1e3, 3), far_field_ms=round(t_ff * 1e3, 3),
+                far_field_shape=list(ff.shape))
+
+
+WORKLOADS = {"cn2d": bench_cn2d, "adi3d": bench_adi3d, "cn_jobs": bench_cn_jobs, "served": bench_served,
+             "scatter": bench_scatter}
 
 
 def main():

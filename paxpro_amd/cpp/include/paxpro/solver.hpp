@@ -41,6 +41,8 @@ class Solver {
   /** GPU extension: field precision (PXG_PREC_FP64 or PXG_PREC_MIXED). */
   void setPrecision(int p);
   int getPrecision() const { return precision; }
+  /** GPU extension: the device plan holding the current field (null before a run). */
+  pxg_plan devicePlan() const { return plan; }
 
  protected:
   Dimension_t dim;

@@ -38,7 +38,8 @@ using paxpro::check;
 namespace {
 
 // Device-evaluated material for the simulation, if the device can evaluate it
-// exactly: a plain ParaxialSimulation with no material or a CSGMaterial, or an
+// exactly: a plain ParaxialSimulation with no material or a CSGMaterial, a
+// GenericScattering run with a CSGMaterial or its reference (vacuum) run, or an
 // unmodified CurvedWaveGuideFD (2D).  Anything else is host-sampled.
 bool set_device_material(pxg_plan plan, const ParaxialSimulation &sim, bool is2d)
 {
@@ -57,6 +58,26 @@ bool set_device_material(pxg_plan plan, const ParaxialSimulation &sim, bool is2d
     wg.use_smoothed = w.usesSmoothedWG();
     check(pxg_set_material_curved_wg(plan, &wg));
     return true;
+  }
+  if (!is2d && typeid(sim) == typeid(GenericScattering)) {
+    // GenericScattering::getXrayMatProp: zero in the reference run, else the material
+    const auto &gs = static_cast<const GenericScattering &>(sim);
+    if (sim.material != nullptr && typeid(*sim.material) == typeid(CSGMaterial)) {
+      const auto &csg = static_cast<const CSGMaterial &>(*sim.material);
+      std::vector<pxg_shape> s;
+      std::vector<pxg_part> p;
+      std::vector<pxg_module> m;
+      csg.flatten(s, p, m);
+      check(pxg_set_material_csg(plan, s.data(), static_cast<int>(s.size()), p.data(),
+                                 static_cast<int>(p.size()), m.data(), static_cast<int>(m.size()),
+                                 gs.isReference() || csg.isReferenceRun));
+      return true;
+    }
+    if (gs.isReference()) {
+      check(pxg_set_material_none(plan));
+      return true;
+    }
+    return false;
   }
   if (typeid(sim) != typeid(ParaxialSimulation)) return false;
   if (sim.material == nullptr) {

@@ -175,6 +175,71 @@ static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
   return 0;
 }
 
+// Examples/sphere.cpp with n points per side and nsteps z-steps: GenericScattering
+// of a 500 nm sphere (a CSG part, or the example's own MaterialFunction subclass in
+// "scatter_user"), far field within 0.05 degrees, written as raw doubles
+// (column-major) with its shape printed.
+static int scatter(const std::string &mode, int n, int nsteps, const char *out, int prec)
+{
+  paxpro::setDefaultPrecision(prec);  // before the simulation constructs its solvers
+  const double r = 500.0;
+  const double xmin = -1.5 * r, ext = 3.0 * r;
+  const double dx = ext / n;
+  const double zmin = -1.05 * r, zmax = 1.05 * r;
+  geom::Sphere sphere(r);
+  geom::Part part("sphere");
+  part.add(sphere);
+  part.delta = 8.9e-6;
+  part.beta = 7e-7;
+  geom::Module mod("phantom");
+  mod.add(part);
+  CSGMaterial csg;
+  csg.addModule(mod);
+  struct Ball : MaterialFunction {
+    double rad;
+    explicit Ball(double r) : rad(r) {}
+    void getXrayMatProp(double x, double y, double z, double &delta, double &beta) const override
+    {
+      if (x * x + y * y + z * z < rad * rad) {
+        delta = 8.9e-6;
+        beta = 7e-7;
+        return;
+      }
+      delta = 0.0;
+      beta = 0.0;
+    }
+  } ball(r);
+  GenericScattering simulation("sphere");
+  simulation.setBeamWaist(400.0 * r);
+  simulation.setMaxScatteringAngle(0.05);
+  simulation.xmin = xmin;
+  simulation.xmax = xmin + (n - 1) * dx;
+  simulation.ymin = xmin;
+  simulation.ymax = xmin + (n - 1) * dx;
+  simulation.zmin = zmin;
+  simulation.zmax = zmax;
+  simulation.dx = dx;
+  simulation.dy = dx;
+  simulation.dz = (zmax - zmin) / nsteps;
+  simulation.downSampleX = n;
+  simulation.downSampleY = n;
+  simulation.downSampleZ = nsteps;
+  simulation.wavelength = 0.1569;
+  simulation.supressMessages = true;
+  if (mode == "scatter_user")
+    simulation.setMaterial(ball);
+  else
+    simulation.setMaterial(csg);
+  simulation.solve();
+  arma::mat ff;
+  simulation.getFarField(ff);
+  std::ofstream f(out, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(ff.memptr()), ff.n_elem * sizeof(double));
+  std::printf("%s n=%d steps=%d farfield=%zux%zu\n", mode.c_str(), n, nsteps, (size_t)ff.n_rows,
+              (size_t)ff.n_cols);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   try {
@@ -184,6 +249,7 @@ int main(int argc, char **argv)
     if (argc < 5) return 2;
     const int n = std::atoi(argv[2]), nsteps = std::atoi(argv[3]);
     const int prec = argc > 5 ? std::atoi(argv[5]) : PXG_PREC_FP64;
+    if (mode.rfind("scatter", 0) == 0) return scatter(mode, n, nsteps, argv[4], prec);
     if (mode.rfind("fft3d", 0) == 0 || mode.rfind("adi", 0) == 0) return run3d(mode, n, nsteps, argv[4], prec);
     return run2d(mode, n, nsteps, argv[4]);
   } catch (const std::exception &e) {
