@@ -370,6 +370,7 @@ def bench_cn_jobs(args, ctx):
                          f"{steps} steps each, one after another, {dt:.1f} s"}
     roof = roofline_of(stats)
     roof["note"] = "one work-group per job marching all steps: latency of the dependent sweeps"
+    trans = transmittivity_timing(ctx, max(jobs, key=lambda j: j["nx"]), steps)
     return line("Mcell*steps/s mixed-prec vs FP64 at <=1e-5 rel L2 (batched Crank-Nicolson jobs)",
                 cells * steps / tm / 1e6, "Mcell*steps/s", steps, args.warmup, tm * 1e3 / steps,
                 "c64 storage, fp64 sweeps", "synthetic: plane waves into curved waveguides",
@@ -377,7 +378,41 @@ def bench_cn_jobs(args, ctx):
                              f"{steps} steps of 100 nm, lambda U[0.05, 0.2] nm, curved guides R U[1e6, 5e7] nm, "
                              "w U[50, 200] nm (seed 0x5EED0003), transparent BC"},
                 roof, cpu, rel_l2_field_vs_fp64_max=ef, rel_l2_intensity_vs_fp64_max=ei,
-                fp64_batch={"value": round(cells * steps / t64 / 1e6, 2), "ms_per_step": round(t64 * 1e3 / steps, 5)})
+                fp64_batch={"value": round(cells * steps / t64 / 1e6, 2), "ms_per_step": round(t64 * 1e3 / steps, 5)},
+                transmittivity=trans)
+
+
+def transmittivity_timing(ctx, job, steps):
+    """CurvedWaveGuideFD's per-step transmittivity (src/transmittivity.cpp:44-83) for the
+    largest job: its FP64 march with every column stored, then the trapezoid sums of all
+    steps in one pxg_intensity_trapezoid call; the guide edges and the truncating
+    closestIndex are host arithmetic, as in the drop-in class."""
+    prob = P.problem(P.CN, job["nx"], steps + 1, job["dx"], job["dz"], 2 * np.pi / job["lam"],
+                     xmin=job["xmin"], bc=P.BC_TRANSPARENT, store_dsx=1, store_dsz=1)
+    plan = P.Plan(ctx, prob)
+    plan.material_curved(**job["wg"])
+    plan.set_field(np.ones(job["nx"], complex))
+    plan.step(1, steps)
+    plan.sync()
+    idx = lambda x: int((x - job["xmin"]) / job["dx"])
+    st, a, e = [1], [idx(0.0)], [idx(job["width"])]
+    for n in range(1, steps + 1):
+        xs = -0.5 * (n * job["dz"]) ** 2 / job["R"]
+        st.append(n)
+        a.append(idx(xs))
+        e.append(idx(xs + job["width"]))
+    ts = []
+    for _ in range(7):
+        t0 = time.perf_counter()
+        plan.intensity_trapezoid(st, a, e)
+        ts.append(time.perf_counter() - t0)
+    plan.close()
+    t = float(np.median(ts))
+    cells = sum(c - b + 1 for b, c in zip(a, e))
+    return {"nx": job["nx"], "steps": steps, "call_ms": round(t * 1e3, 3),
+            "us_per_step": round(t * 1e6 / len(st), 3), "cells_summed": cells,
+            "note": "one call for all steps, incl. index upload and result copy; the march itself "
+                    "is not in this time"}
 
 
 def bench_served(args, ctx):

@@ -230,6 +230,17 @@ int pxg_far_field_dims(pxg_plan plan, int b, unsigned pad, double phi_min, doubl
 int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, double phi_min,
                   double phi_max, unsigned export_rows, unsigned export_cols, double *out);
 
+/* ---- per-step reductions (2D plans storing the full solution) ----------------- */
+/* Replaces post::Transmittivity::trapezoidalIntegrateIntensityZ (src/transmittivity.cpp:30-42),
+ * which post::Transmittivity::compute (src/transmittivity.cpp:44-83) applies after every step
+ * of CurvedWaveGuideFD::solve (src/curvedWaveGuide2D.cpp:207-219), for many steps at once:
+ * sums[i] = |u(s)|^2 + |u(e)|^2 + 2 sum_{s<j<e} |u(j)|^2 over column steps[i] of the stored
+ * solution of field/job b, s = ix_start[i], e = ix_end[i].  The caller applies the
+ * reference's 0.5 (x_e - x_s)/(e - s) and the normalisation.  Needs store_dsz = 1 and
+ * store_dsx = 1 (every step at full resolution), 0 <= s, e < nx and 0 <= steps[i] < nz. */
+int pxg_intensity_trapezoid(pxg_plan plan, int b, int n, const int *steps, const int *ix_start,
+                            const int *ix_end, double *sums);
+
 /* ---- batched tridiagonal solve ---------------------------------------------- */
 /* Replaces ThomasAlgorithm::solve<cdouble> (include/thomasAlgorithm.tpp:2-32) for
  * many independent systems: diag/sub/rhs are host complex128 arrays of nsys

@@ -365,3 +365,45 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     if export_rows and export_cols:
         res = resize_matrix(res, export_rows, export_cols)
     return res
+
+
+# ---------------------------------------------------------------------- transmittivity
+def closest_index(x, vmin, step):
+    """ParaxialSimulation::closestIndex (src/paraxialSimulation.cpp:215-219): the
+    truncating conversion of (x - min)/step to unsigned (undefined below 0 in the
+    reference; rejected here)."""
+    v = (x - vmin) / step
+    if not (v >= 0.0 and np.isfinite(v)):
+        raise ValueError(f"position {x} below the grid")
+    return int(v)
+
+
+def trapezoid_intensity(col, s, e, xmin, dx):
+    """post::Transmittivity::trapezoidalIntegrateIntensityZ (src/transmittivity.cpp:30-42),
+    in its order: the two ends, then 2|u|^2 for s < ix < e, times 0.5 (x_e - x_s)/(e - s)
+    with the unsigned difference e - s."""
+    integral = abs(col[s]) ** 2 + abs(col[e]) ** 2
+    for ix in range(s + 1, e):
+        integral += 2.0 * abs(col[ix]) ** 2
+    span = (e - s) % (1 << 32)
+    dxe = ((xmin + e * dx) - (xmin + s * dx)) / float(span) if span else float("nan")
+    return integral * dxe * 0.5
+
+
+def transmittivity_curved(solution, xmin, dx, zmin, dz, R, width):
+    """post::Transmittivity::compute after every step n = 1 .. nz-1 of
+    CurvedWaveGuideFD::solve (src/transmittivity.cpp:44-83, src/curvedWaveGuide2D.cpp:61-69,
+    207-219), no border tracker.  The normalisation is computed on the first call, i.e.
+    from the field after step 1, over the guide's z = 0 extent [0, width].
+    solution: (nx, nz) complex, column n = field after step n.  Returns (T, I0)."""
+    nz = solution.shape[1]
+    i0 = trapezoid_intensity(solution[:, 1], closest_index(0.0, xmin, dx),
+                             closest_index(width, xmin, dx), xmin, dx)
+    out = np.empty(nz - 1)
+    for n in range(1, nz):
+        z = zmin + n * dz
+        xs = -0.5 * z * z / R
+        xe = xs + width
+        s, e = closest_index(xs, xmin, dx), closest_index(xe, xmin, dx)
+        out[n - 1] = trapezoid_intensity(solution[:, n], s, e, xmin, dx) / i0
+    return out, i0

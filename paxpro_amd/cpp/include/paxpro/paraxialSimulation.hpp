@@ -46,6 +46,8 @@ class ParaxialSimulation {
   void setSolver(Solver &solv);
   std::string getName() const { return name; }
   void step();
+  /** Grid indices of (x, z), truncated (src/paraxialSimulation.cpp:215-219) */
+  void closestIndex(double x, double z, unsigned int &ix, unsigned int &iz) const;
   double getZ(unsigned int iz) const;
   double getX(int ix) const;
   double getY(int iy) const;

@@ -1,11 +1,12 @@
 // 2D waveguide simulations (include/waveGuideFDSimulation.hpp,
 // include/curvedWaveGuide2D.hpp of the reference).  The curved guide's material
-// is evaluated on the device; border tracking, eigenmode projection and HDF5
-// output are not part of this build.
+// is evaluated on the device, and so are the per-step transmittivity sums; border
+// tracking, eigenmode projection and HDF5 output are not part of this build.
 #pragma once
 
 #include "cladding.hpp"
 #include "paraxialSimulation.hpp"
+#include "transmittivity.hpp"
 
 class WaveGuideFDSimulation : public ParaxialSimulation {
  public:
@@ -29,7 +30,11 @@ class WaveGuideFDSimulation : public ParaxialSimulation {
 
 class CurvedWaveGuideFD : public WaveGuideFDSimulation {
  public:
-  CurvedWaveGuideFD() : WaveGuideFDSimulation("CurvedWaveGuide2D") {}
+  CurvedWaveGuideFD() : WaveGuideFDSimulation("CurvedWaveGuide2D") { transmittivity.linkWaveguide(*this); }
+  /** Returns the transmittivity module */
+  const post::Transmittivity &getTransmittivity() { return transmittivity; }
+  /** The z-march, then the transmittivity of every step (src/curvedWaveGuide2D.cpp:207-219) */
+  virtual void solve() override;
   void setRadiusOfCurvature(double newR) { R = newR; }
   void setWidth(double newWidth) { width = newWidth; }
   double getWidth() const { return width; }
@@ -46,4 +51,5 @@ class CurvedWaveGuideFD : public WaveGuideFDSimulation {
   double R{1.0};
   double width{1.0};
   bool useSmoothed{false};
+  post::Transmittivity transmittivity;
 };

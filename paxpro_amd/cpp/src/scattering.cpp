@@ -177,3 +177,102 @@ void GenericScattering::printInfo() const
 }
 
 void GenericScattering::getFarField(arma::mat &farF) { ff.result(*solver, farF); }
+
+// ----------------------------------------------------------- closestIndex
+// src/paraxialSimulation.cpp:215-219.  The reference converts a negative value to
+// unsigned (undefined); here that throws.
+void ParaxialSimulation::closestIndex(double x, double z, unsigned int &ix, unsigned int &iz) const
+{
+  const double vx = (x - xDisc.min) / xDisc.step, vz = (z - zDisc.min) / zDisc.step;
+  if (!(vx >= 0.0) || !(vz >= 0.0) || vx >= 4294967296.0 || vz >= 4294967296.0)
+    throw std::runtime_error("closestIndex: position outside the grid");
+  ix = static_cast<unsigned int>(vx);
+  iz = static_cast<unsigned int>(vz);
+}
+
+// ------------------------------------------------------- post::Transmittivity
+// 0.5 (x_e - x_s)/(e - s) of trapezoidalIntegrateIntensityZ (src/transmittivity.cpp:30-42),
+// with the reference's unsigned difference.
+double post::Transmittivity::trapezoidScale(unsigned int ixStart, unsigned int ixEnd) const
+{
+  const double dx = (guide->getX(static_cast<int>(ixEnd)) - guide->getX(static_cast<int>(ixStart))) /
+                    static_cast<double>(ixEnd - ixStart);
+  return dx * 0.5;
+}
+
+double post::Transmittivity::trapezoidalIntegrateIntensityZ(unsigned int ixStart, unsigned int ixEnd) const
+{
+  const arma::cx_vec &u = guide->getSolver().getLastSolution();
+  if (ixStart >= u.n_elem || ixEnd >= u.n_elem) throw std::runtime_error("Transmittivity: guide outside the grid");
+  double integral = std::pow(std::abs(u(ixStart)), 2) + std::pow(std::abs(u(ixEnd)), 2);
+  for (unsigned int ix = ixStart + 1; ix + 1 <= ixEnd; ++ix) integral += 2.0 * std::pow(std::abs(u(ix)), 2);
+  return integral * trapezoidScale(ixStart, ixEnd);
+}
+
+// src/transmittivity.cpp:44-83 (no border tracker in this build)
+void post::Transmittivity::compute(double z)
+{
+  if (guide == nullptr) throw std::runtime_error("Transmittivity: no waveguide linked");
+  unsigned int wgStart = 0, wgEnd = 0, zIndx = 0;
+  if (computeIntensityAtZero) {
+    guide->closestIndex(0.0, 0.0, wgStart, zIndx);
+    guide->closestIndex(guide->getWidth(), 0.0, wgEnd, zIndx);
+    intensityAtZero = trapezoidalIntegrateIntensityZ(wgStart, wgEnd);
+    computeIntensityAtZero = false;
+  }
+  guide->closestIndex(guide->waveGuideStartX(z), z, wgStart, zIndx);
+  guide->closestIndex(guide->waveGuideEndX(z), z, wgEnd, zIndx);
+  transmission.push_back(trapezoidalIntegrateIntensityZ(wgStart, wgEnd) / intensityAtZero);
+}
+
+void post::Transmittivity::computeSteps(unsigned int first, unsigned int n)
+{
+  if (guide == nullptr) throw std::runtime_error("Transmittivity: no waveguide linked");
+  if (n == 0) return;
+  const pxg_plan plan = guide->getSolver().devicePlan();
+  if (plan == nullptr) throw std::runtime_error("Transmittivity: the solver holds no solution on the device");
+  const bool zero = computeIntensityAtZero;
+  const unsigned m = n + (zero ? 1 : 0);
+  std::vector<int> steps(m), s(m), e(m);
+  std::vector<unsigned> us(m), ue(m);
+  unsigned int zIndx = 0;
+  size_t k = 0;
+  if (zero) {
+    // the first compute() runs after step `first`, on that step's field
+    guide->closestIndex(0.0, 0.0, us[0], zIndx);
+    guide->closestIndex(guide->getWidth(), 0.0, ue[0], zIndx);
+    steps[0] = static_cast<int>(first);
+    k = 1;
+  }
+  for (unsigned i = 0; i < n; ++i, ++k) {
+    const double z = guide->getZ(first + i);
+    guide->closestIndex(guide->waveGuideStartX(z), z, us[k], zIndx);
+    guide->closestIndex(guide->waveGuideEndX(z), z, ue[k], zIndx);
+    steps[k] = static_cast<int>(first + i);
+  }
+  for (unsigned i = 0; i < m; ++i) {
+    if (us[i] > 2147483647u || ue[i] > 2147483647u) throw std::runtime_error("Transmittivity: guide outside the grid");
+    s[i] = static_cast<int>(us[i]);
+    e[i] = static_cast<int>(ue[i]);
+  }
+  std::vector<double> sums(m);
+  check(pxg_intensity_trapezoid(plan, 0, static_cast<int>(m), steps.data(), s.data(), e.data(), sums.data()));
+  k = 0;
+  if (zero) {
+    intensityAtZero = sums[0] * trapezoidScale(us[0], ue[0]);
+    computeIntensityAtZero = false;
+    k = 1;
+  }
+  for (; k < m; ++k) transmission.push_back(sums[k] * trapezoidScale(us[k], ue[k]) / intensityAtZero);
+}
+
+// src/curvedWaveGuide2D.cpp:207-219: the march (one device call, then the
+// longitudinal filter and downsampling as Solver2D::solve applies them), then the
+// transmittivity after every step n = 1 .. Nz-1 from the stored columns.
+void CurvedWaveGuideFD::solve()
+{
+  if (solver == nullptr) throw std::runtime_error("No solver specified!");
+  const unsigned nz = nodeNumberLongitudinal();
+  solver->solve();
+  if (nz > 1) transmittivity.computeSteps(1, nz - 1);
+}

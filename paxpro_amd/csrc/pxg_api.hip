@@ -1678,6 +1678,41 @@ int pxg_far_field(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, double ph
   });
 }
 
+int pxg_intensity_trapezoid(pxg_plan P, int b, int n, const int *steps, const int *ix_start,
+                            const int *ix_end, double *sums)
+{
+  return guarded([&] {
+    check_b(P, b);
+    const pxg_problem &q = P->is_jobs ? P->jp[b] : P->p;
+    if (q.kind != PXG_CN && q.kind != PXG_FFT2D) fail(PXG_ERR_INVALID, "needs a 2D plan (CN or FFT2D)");
+    if (q.store_dsz != 1 || q.store_dsx != 1)
+      fail(PXG_ERR_STATE, "needs the full stored solution (store_dsz = store_dsx = 1)");
+    if (n < 0 || (n > 0 && (!steps || !ix_start || !ix_end || !sums))) fail(PXG_ERR_INVALID, "null array");
+    for (int i = 0; i < n; ++i) {
+      if (steps[i] < 0 || steps[i] >= q.nz) fail(PXG_ERR_INVALID, "step %d out of [0,%d)", steps[i], q.nz);
+      if (ix_start[i] < 0 || ix_start[i] >= q.nx || ix_end[i] < 0 || ix_end[i] >= q.nx)
+        fail(PXG_ERR_INVALID, "index range [%d,%d] outside [0,%d)", ix_start[i], ix_end[i], q.nx);
+    }
+    if (n == 0) return;
+    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+    hipStream_t s = P->stream;
+    std::vector<int> idx(3 * static_cast<size_t>(n));
+    std::copy(steps, steps + n, idx.begin());
+    std::copy(ix_start, ix_start + n, idx.begin() + n);
+    std::copy(ix_end, ix_end + n, idx.begin() + 2 * n);
+    DBuf di, ds;
+    di.alloc(idx.size() * sizeof(int));
+    ds.alloc(static_cast<size_t>(n) * sizeof(double));
+    hip_check(hipMemcpyAsync(di.p, idx.data(), di.bytes, hipMemcpyHostToDevice, s), "upload");
+    const char *sol = P->sol.as<char>() + P->soff[b] * P->esz;
+    const int *d = di.as<int>();
+    hip_check(launch_trapz_intensity(sol, P->esz == 8, q.nx, n, d, d + n, d + 2 * n, ds.as<double>(), s),
+              "trapezoid");
+    hip_check(hipMemcpyAsync(sums, ds.p, ds.bytes, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "sync");
+  });
+}
+
 int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
                        double *rhs)
 {

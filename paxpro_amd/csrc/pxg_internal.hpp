@@ -76,6 +76,10 @@ struct FfArgs {
 hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
 hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,
                             hipStream_t s);
+// Trapezoid sums of |u|^2 over per-step index ranges of a stored 2D solution
+// (pxg_post.hip): one work-group per entry.
+hipError_t launch_trapz_intensity(const void *sol, int c64, int nx, int n, const int *steps,
+                                  const int *ixs, const int *ixe, double *sums, hipStream_t s);
 // Lines per work-group for an LDS budget of lds_cap bytes (FP64 lines).
 int max_lines(int n, int lds_cap);
 // Stream of a plan and device of a context (server internals).

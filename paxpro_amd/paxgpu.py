@@ -39,7 +39,7 @@ EXPORTS = [
     "pxg_plan_set_lines", "pxg_plan_get_lines", "pxg_plan_autotune",
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
-    "pxg_far_field_dims", "pxg_far_field",
+    "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
@@ -137,6 +137,7 @@ def lib():
         L.pxg_job_solution_dims.argtypes = [P, I, P]
         L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
         L.pxg_far_field.argtypes = [P, I, P, I, U, D, D, U, U, P]
+        L.pxg_intensity_trapezoid.argtypes = [P, I, I, P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -253,6 +254,16 @@ class Plan:
     def set_wavenumbers(self, ks):
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         _check(lib().pxg_set_wavenumbers(self.h, _p(ks), ks.size))
+
+    def intensity_trapezoid(self, steps, ix_start, ix_end, b=0):
+        """Trapezoid sums of |u|^2 over [ix_start[i], ix_end[i]] of stored solution column
+        steps[i] (post::Transmittivity::trapezoidalIntegrateIntensityZ without its 0.5 dx)."""
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        a = np.ascontiguousarray(ix_start, dtype=np.int32)
+        e = np.ascontiguousarray(ix_end, dtype=np.int32)
+        out = np.empty(st.size, dtype=np.float64)
+        _check(lib().pxg_intensity_trapezoid(self.h, b, st.size, _p(st), _p(a), _p(e), _p(out)))
+        return out
 
     def far_field(self, b=0, ref=None, ref_b=0, pad=1024, phi_min=-90.0, phi_max=90.0,
                   export_rows=0, export_cols=0):

@@ -10,6 +10,7 @@
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "paxpro/paxpro.hpp"
 
@@ -171,6 +172,14 @@ static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
   write(out, last.memptr(), last.n_elem);
   const arma::cx_mat &sol = solver->getSolution();
   write((std::string(out) + ".sol").c_str(), sol.memptr(), sol.n_elem);
+  if (mode == "cn_curved") {
+    // CurvedWaveGuideFD::solve also computed the transmittivity after every step
+    std::vector<double> tr(1, wg.getTransmittivity().getIntensityAtZero());
+    const std::vector<double> &t = wg.getTransmittivity().get();
+    tr.insert(tr.end(), t.begin(), t.end());
+    std::ofstream f((std::string(out) + ".trans").c_str(), std::ios::binary);
+    f.write(reinterpret_cast<const char *>(tr.data()), tr.size() * sizeof(double));
+  }
   std::printf("%s nx=%d steps=%d sol=%zux%zu\n", mode.c_str(), nx, nsteps, (size_t)sol.n_rows, (size_t)sol.n_cols);
   return 0;
 }

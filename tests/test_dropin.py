@@ -100,6 +100,13 @@ def test_dropin_2d(tmp_path, mode, nx):
         ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol)
     assert rel_l2(load(out, (nx,)), ref) < 1e-10
     assert rel_l2(load(out + ".sol", sol.shape), sol) < 1e-10
+    if mode == "cn_curved":
+        # post::Transmittivity after every step, from the device-stored columns
+        tr = np.fromfile(out + ".trans", dtype=np.float64)
+        T, i0 = O.transmittivity_curved(sol, lo, dx, 0.0, dz, R, w)
+        assert tr.size == nsteps + 1
+        assert tr[0] == pytest.approx(i0, rel=1e-10)
+        assert np.allclose(tr[1:], T, rtol=1e-10, atol=0)
 
 
 @pytest.mark.gpu
