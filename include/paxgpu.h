@@ -150,6 +150,13 @@ int pxg_plan_get_lines(pxg_plan plan, int *lines_x, int *lines_y);
  * after the material and before pxg_set_field.  reps <= 0 means 5 timed
  * launches per shape.  A no-op for the other kinds. */
 int pxg_plan_autotune(pxg_plan plan, int reps);
+/* FFT3D z-step loops as HIP graphs: with enable != 0, pxg_step captures the
+ * kernels of a call into a graph on first use and replays it when a later call has
+ * the same steps, work-group shapes, material state and kernel arguments (up to 8
+ * shapes per plan are kept).  Results are identical to the stream enqueue.  No
+ * reference counterpart: FFTW executes its plans from the host
+ * (src/fftSolver3D.cpp:95,119). */
+int pxg_plan_set_graphs(pxg_plan plan, int enable);
 /* Job plan: n independent problems of one kind in one plan, each with its own
  * grid, wavenumber and material (uneven Nx, photon energies, waveguides), all
  * advanced by one launch per pxg_step.  Field/solution index b = job b.

@@ -135,6 +135,15 @@ struct pxg_plan_s {
   // solution storage
   DBuf sol;
   int sol_dims[3] = {0, 0, 0};
+  // FFT3D z-step loops captured as HIP graphs (pxg_plan_set_graphs): replayed when
+  // the call's shape and every kernel argument match the capture
+  bool use_graphs = false;
+  struct Graph {
+    std::vector<char> key;
+    long map_step_out;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
   size_t sol_fstride = 0;  // elements per field
   // tridiagonal scratch (CN, ADI)
   DBuf scratch;
@@ -738,12 +747,73 @@ void run_adi(pxg_plan P, unsigned first, unsigned nsteps)
   }
 }
 
+template <class T>
+void key_put(std::vector<char> &k, const T &v)
+{
+  const char *b = reinterpret_cast<const char *>(&v);
+  k.insert(k.end(), b, b + sizeof(T));
+}
+
+// An FFT3D march as one HIP graph: everything run_fft3d enqueues is kernel
+// launches on the plan stream (no memset, copy or host synchronisation), so the
+// loop is captured once and replayed.  The key holds the call's steps, the
+// work-group shapes, the map state the first step reads, the problem and the full
+// argument block of the passes (every pointer and scalar the kernels receive): a
+// replay therefore launches exactly what a fresh enqueue would.
+template <class R>
+void run_fft3d_graph(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  std::vector<char> key;
+  const FftArgs<R> a = fft_args<R>(P);
+  key_put(key, first);
+  key_put(key, nsteps);
+  key_put(key, P->lines_x);
+  key_put(key, P->lines_y);
+  key_put(key, P->map_step);
+  key_put(key, P->mat_mode);
+  key_put(key, P->p);
+  key_put(key, a);
+  for (auto &g : P->graphs)
+    if (g.key == key) {
+      hip_check(hipGraphLaunch(g.exec, P->stream), "hipGraphLaunch");
+      P->map_step = g.map_step_out;
+      return;
+    }
+  hip_check(hipStreamBeginCapture(P->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+  hipGraph_t graph = nullptr;
+  try {
+    run_fft3d<R>(P, first, nsteps);
+  } catch (...) {
+    (void)hipStreamEndCapture(P->stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(P->stream, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  hip_check(e, "hipGraphInstantiate");
+  if (P->graphs.size() >= 8) {  // keep the 8 most recent shapes
+    (void)hipGraphExecDestroy(P->graphs.front().exec);
+    P->graphs.erase(P->graphs.begin());
+  }
+  P->graphs.push_back({std::move(key), P->map_step, exec});
+  hip_check(hipGraphLaunch(exec, P->stream), "hipGraphLaunch");
+}
+
 void run(pxg_plan P, unsigned first, unsigned nsteps)
 {
   if (!P->field_set) fail(PXG_ERR_STATE, "initial conditions not set (pxg_set_field)");
   if (first < 1) fail(PXG_ERR_INVALID, "steps start at 1 (step 0 is the initial condition)");
   if (nsteps == 0) return;
   const int k = P->p.kind;
+  if (k == PXG_FFT3D && P->use_graphs && !P->profiling) {
+    if (P->mixed)
+      run_fft3d_graph<float>(P, first, nsteps);
+    else
+      run_fft3d_graph<double>(P, first, nsteps);
+    return;
+  }
   if (P->mixed) {
     if (k == PXG_FFT3D) run_fft3d<float>(P, first, nsteps);
     else if (k == PXG_FFT2D) run_fft2d<float>(P, first, nsteps);
@@ -761,6 +831,7 @@ void check_plan(pxg_plan P)
 {
   if (!P) fail(PXG_ERR_INVALID, "null plan");
 }
+
 
 void check_b(pxg_plan P, int b)
 {
@@ -1165,6 +1236,7 @@ int pxg_plan_destroy(pxg_plan P)
       }
     for (hipEvent_t e : {P->fork_ev, P->join_ev, P->join2_ev})
       if (e) (void)hipEventDestroy(e);
+    for (auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     pxg_ctx c = P->ctx;
     bool last = false;
@@ -1391,6 +1463,19 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
     P->mat_mode = MAT_PLANES;
     P->map_step = -1;
     P->jobs_dirty = true;
+  });
+}
+
+int pxg_plan_set_graphs(pxg_plan P, int enable)
+{
+  return guarded([&] {
+    check_plan(P);
+    P->use_graphs = enable != 0;
+    if (!P->use_graphs) {
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+      for (auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
+      P->graphs.clear();
+    }
   });
 }
 

@@ -39,7 +39,7 @@ EXPORTS = [
     "pxg_plan_set_lines", "pxg_plan_get_lines", "pxg_plan_autotune",
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
-    "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid",
+    "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
@@ -138,6 +138,7 @@ def lib():
         L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
         L.pxg_far_field.argtypes = [P, I, P, I, U, D, D, U, U, P]
         L.pxg_intensity_trapezoid.argtypes = [P, I, I, P, P, P, P]
+        L.pxg_plan_set_graphs.argtypes = [P, I]
         _LIB = L
     return _LIB
 
@@ -254,6 +255,10 @@ class Plan:
     def set_wavenumbers(self, ks):
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         _check(lib().pxg_set_wavenumbers(self.h, _p(ks), ks.size))
+
+    def set_graphs(self, enable=True):
+        """Capture FFT3D z-step loops as HIP graphs and replay them (pxg_plan_set_graphs)."""
+        _check(lib().pxg_plan_set_graphs(self.h, int(enable)))
 
     def intensity_trapezoid(self, steps, ix_start, ix_end, b=0):
         """Trapezoid sums of |u|^2 over [ix_start[i], ix_end[i]] of stored solution column

@@ -283,3 +283,47 @@ def test_fft3d_smallest_and_largest_grid(ctx, n, nsteps):
     assert err < 1e-11
     _, (mix,) = _run_gpu(ctx, fft3d_problem(g, n, nsteps + 1, precision=P.PREC_MIXED), u0, nsteps, csg)
     assert rel_l2(mix, out) < 1e-5
+
+
+@pytest.mark.parametrize("batch,prec", [(1, P.PREC_FP64), (8, P.PREC_MIXED)])
+def test_fft3d_graph_replays_equal_eager(ctx, batch, prec):
+    """pxg_plan_set_graphs: every replay of a captured march equals the eager
+    enqueue bit for bit, with eager work between replays; a material change forces
+    a new capture (the key holds the kernels' arguments), and cube storage and the
+    pass-X material map (batches of 8) are inside the graph."""
+    announce("fft3d graphs", batch, prec)
+    n, nsteps = 64, 12
+    g = fft3d_case(n, dz=10.0, nsteps=nsteps)
+    u0 = gaussian_field(g, n, 3000.0)
+    csg1 = spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)])
+    csg2 = spheres_scene([(500.0, 0.0, 0.0, 3000.0, 8e-6, 2e-7)])
+    prob = fft3d_problem(g, n, nsteps + 1, precision=prec, batch=batch, store=(4, 3))
+
+    def run(plan, first, count):
+        for b in range(batch):
+            plan.set_field(u0, b)
+        plan.step(first, count)
+        plan.sync()
+        return [plan.get_field(b) for b in range(batch)], plan.get_solution(0)
+
+    eager = P.Plan(ctx, prob)
+    graph = P.Plan(ctx, prob)
+    graph.set_graphs(True)
+    for csg in (csg1, csg2):
+        eager.material_csg(*csg)
+        graph.material_csg(*csg)
+        ref = run(eager, 1, nsteps)
+        for rep in range(3):
+            out = run(graph, 1, nsteps)
+            assert all(np.array_equal(a, r) for a, r in zip(out[0], ref[0])), rep
+            assert np.array_equal(out[1], ref[1]), rep
+            run(eager, 1, 5)  # eager work on the device between replays
+        # a call of another shape is captured on its own
+        a5 = run(graph, 1, 5)
+        e5 = run(eager, 1, 5)
+        assert all(np.array_equal(a, r) for a, r in zip(a5[0], e5[0]))
+    graph.set_graphs(False)
+    out = run(graph, 1, nsteps)
+    assert all(np.array_equal(a, r) for a, r in zip(out[0], ref[0]))
+    eager.close()
+    graph.close()
