@@ -552,7 +552,7 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_big(const CnJob *__restrict__ j
   __shared__ cx<double> edge_lo[CN_MAX_P], edge_hi[CN_MAX_P];
   __shared__ uint64_t mask[TRI_MASK_WORDS];
   const CnJob &J = jobs[blockIdx.x];
-  const int p = threadIdx.x, N = J.nx, P = J.P, bc = J.bc, m = J.m;
+  const int p = threadIdx.x, N = J.nx, P = J.P, m = J.m;
   const bool active = p < P;
   cx<R> *fld = fields + J.foff;
   const double k = J.k, dz = J.dz, dx = J.dx, xmin = J.xmin, zmin = J.zmin;

@@ -103,6 +103,29 @@ def test_generic_scattering_sphere_vs_oracle(ctx, prec, tol):
     run.close()
 
 
+@pytest.mark.parametrize("pad,phi,export", [
+    (0, (-0.05, 0.12), (0, 0)),       # M = n, asymmetric interior window: rows != cols
+    (512, (-0.3, 0.08), (20, 1000)),  # one export size above the matrix: no resize
+    (4096, (-0.02, 0.03), (9, 7)),    # D = 32, small export
+])
+def test_far_field_anisotropic_grid(ctx, pad, phi, export):
+    # dy != dx: the row window follows dy, the column window dx
+    n = 128
+    g = _grid(n)
+    dy = 0.7 * g.dx
+    prob = P.problem(P.FFT3D, n, 3, g.dx, g.dz, K, xmin=g.xmin, ymin=g.ymin, dy=dy, zmin=g.zmin)
+    plan, refp = P.Plan(ctx, prob), P.Plan(ctx, prob)
+    plan.set_field(_random_field(n, 7))
+    refp.set_field(_random_field(n, 8))
+    out = plan.far_field(0, refp, 0, pad, phi[0], phi[1], *export)
+    exp = O.far_field(plan.get_field(), refp.get_field(), pad, K, g.dx, dy, phi[0], phi[1], *export)
+    announce("anisotropic far field", pad, phi, export, out.shape)
+    assert out.shape == exp.shape
+    assert rel_l2(out, exp) < 1e-12
+    plan.close()
+    refp.close()
+
+
 def test_far_field_errors(ctx):
     g = _grid(64)
     plan = P.Plan(ctx, fft3d_problem(g, 64, 3))
