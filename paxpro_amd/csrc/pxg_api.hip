@@ -794,6 +794,8 @@ void run_fft3d_graph(pxg_plan P, unsigned first, unsigned nsteps)
   (void)hipGraphDestroy(graph);
   hip_check(e, "hipGraphInstantiate");
   if (P->graphs.size() >= 8) {  // keep the 8 most recent shapes
+    // the evicted graph may still be running on the stream
+    hip_check(hipStreamSynchronize(P->stream), "sync");
     (void)hipGraphExecDestroy(P->graphs.front().exec);
     P->graphs.erase(P->graphs.begin());
   }
