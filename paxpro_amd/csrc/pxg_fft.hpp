@@ -100,7 +100,22 @@ constexpr int tw_slot()
 
 template <int N, class Rt>
 struct TwBase {
+  static constexpr bool POW = false;
   cx<Rt> w[tw_stages<N>() > 0 ? tw_stages<N>() : 1];
+};
+
+// The seven powers of the last twiddle stage's base, formed once (the same
+// products as the per-stage computation), with the bases of the other stages:
+// for kernels that run several transforms per lane and have registers to spare.
+// Inverse transforms multiply by the conjugates (the per-stage path conjugates
+// the base first; the two agree to rounding, as contraction may pair the
+// operands differently).
+template <int N, class Rt>
+struct TwPow {
+  static constexpr bool POW = true;
+  static constexpr int KEEP = tw_stages<N>() - 1;  // slots from here hold powers
+  cx<Rt> b[tw_stages<N>() > 0 ? tw_stages<N>() : 1];     // bases
+  cx<Rt> w[tw_stages<N>() > 0 ? tw_stages<N>() : 1][7];  // w[s][r - 1] = base^r
 };
 
 template <int N, class Rt>
@@ -113,9 +128,30 @@ __device__ __forceinline__ TwBase<N, Rt> tw_base(const cx<Rt> *__restrict__ tw, 
   return b;
 }
 
+template <int N, class Rt>
+__device__ __forceinline__ TwPow<N, Rt> tw_pow(const TwBase<N, Rt> &b)
+{
+  TwPow<N, Rt> p;
+#pragma unroll
+  for (int s = 0; s < tw_stages<N>(); ++s) p.b[s] = b.w[s];
+#pragma unroll
+  for (int s = TwPow<N, Rt>::KEEP; s < tw_stages<N>(); ++s) {
+    const cx<Rt> w1 = b.w[s];
+    const cx<Rt> w2 = w1 * w1, w3 = w2 * w1, w4 = w2 * w2;
+    p.w[s][0] = w1;
+    p.w[s][1] = w2;
+    p.w[s][2] = w3;
+    p.w[s][3] = w4;
+    p.w[s][4] = w4 * w1;
+    p.w[s][5] = w3 * w3;
+    p.w[s][6] = w4 * w3;
+  }
+  return p;
+}
+
 // Radix-R butterflies of one stage on the canonical registers of lane t.
-template <int N, int R, int NS, int S, class Rt>
-__device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TwBase<N, Rt> &twb)
+template <int N, int R, int NS, int S, class Rt, class TW>
+__device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TW &twb)
 {
   constexpr int Q = 8 / R;
 #pragma unroll
@@ -123,7 +159,14 @@ __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TwBas
     cx<Rt> u[8];
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
-    if constexpr (NS > 1) {
+    if constexpr (NS > 1 && TW::POW && tw_slot<N, NS>() >= TwPow<N, Rt>::KEEP) {
+      static_assert(R == 8, "twiddle stages are radix 8");
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cx<Rt> w = twb.w[tw_slot<N, NS>()][r - 1];
+        u[r] = u[r] * (S > 0 ? conj(w) : w);
+      }
+    } else if constexpr (NS > 1) {
       // The base w = e^{-2 pi i (j mod NS)/(NS R)} of butterfly j = t + q N/8
       // (held in twb); its powers w^2..w^{R-1}
       // are products at most three deep (a few ulp in FP64).  Loading all R-1
@@ -131,7 +174,10 @@ __device__ __forceinline__ void stage_compute(cx<Rt> (&v)[8], int t, const TwBas
       // the pass time.
       static_assert(R == 8, "twiddle stages are radix 8");
       cx<Rt> w[8];
-      w[1] = twb.w[tw_slot<N, NS>()];
+      if constexpr (TW::POW)
+        w[1] = twb.b[tw_slot<N, NS>()];
+      else
+        w[1] = twb.w[tw_slot<N, NS>()];
       // Opaque copy: keeps the powers local to this stage.  Without it the
       // compiler forms them once per kernel and keeps up to 18 complex values
       // live across both transforms (spills at 1024 lanes).
@@ -197,7 +243,8 @@ __device__ __forceinline__ void load_canon(cx<Rt> (&v)[8], const cx<Rt> *lds, in
 // lds_sync<true> orders the stages.
 template <int N, int S, int NS, class Rt, bool WL>
 struct StockhamStages {
-  static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TwBase<N, Rt> &twb)
+  template <class TW>
+  static __device__ __forceinline__ void run(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TW &twb)
   {
     constexpr int R = (NS == 1) ? FftShape<N>::FIRST : 8;
     stage_compute<N, R, NS, S>(v, t, twb);
@@ -279,8 +326,8 @@ __device__ __forceinline__ void fft_line_remap_out(cd (&v)[8], cd *la, cd *lb, i
 // canonical slots (lds_pos(t + m N/8)).  The leading sync protects LDS still
 // being read by a preceding transform; LEAD = false skips it where the line's
 // LDS has not been used before in the kernel.
-template <int N, int S, class Rt, bool WL = false, bool LEAD = true>
-__device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TwBase<N, Rt> &twb)
+template <int N, int S, class Rt, bool WL = false, bool LEAD = true, class TW>
+__device__ __forceinline__ void fft_line(cx<Rt> (&v)[8], cx<Rt> *lds, int t, const TW &twb)
 {
   if constexpr (LEAD) lds_sync<WL>();
   StockhamStages<N, S, 1, Rt, WL>::run(v, lds, t, twb);

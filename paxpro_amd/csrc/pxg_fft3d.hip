@@ -13,6 +13,8 @@
 // A run of n steps is one forward FFT_y pass, then n (X, Y) pairs, the last Y
 // pass leaving the field in real space.  Each pass reads and writes the field
 // once: 4 x sizeof(complex) bytes per cell and step.
+#include <type_traits>
+
 #include "pxg_fft.hpp"
 #include "pxg_fp64math.hpp"
 #include "pxg_internal.hpp"
@@ -104,7 +106,16 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
   cd *ll = lds + line * LdsLine<N>::STRIDE;
   const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
-  const TwBase<N, double> twb = tw_base<N>(a.tw, t);
+  // Up to 512 points the powers of the last twiddle stage are formed once for both
+  // transforms (the registers are free: LDS bounds the occupancy); longer lines
+  // form them per stage and transform
+  using TwHold = std::conditional_t<(N <= 512), TwPow<N, double>, TwBase<N, double>>;
+  const TwHold twb = [&] {
+    if constexpr (N <= 512)
+      return tw_pow<N>(tw_base<N>(a.tw, t));
+    else
+      return tw_base<N>(a.tw, t);
+  }();
 
   // Field loads and, for a CSG material, the cells' material-map codes are
   // issued together before any arithmetic.
