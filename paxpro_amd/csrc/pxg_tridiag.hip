@@ -839,9 +839,9 @@ hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s)
   char *s0 = static_cast<char *>(a.scratch);
   // Dominant systems: converged tile-Jacobi; otherwise one sequential sweep per
   // line.  FP64 fields take the sweeps that reach FP64 rounding; complex64
-  // fields stop where the truncation is 1e-12 relative, four orders of
-  // magnitude below the complex64 rounding of the stored result.
-  const double tol = sizeof(R) == sizeof(double) ? 1e-17 : 1e-12;
+  // fields stop where the truncation is 1e-10 relative, nearly three orders of
+  // magnitude below the complex64 rounding (6e-8) of the stored result.
+  const double tol = sizeof(R) == sizeof(double) ? 1e-17 : 1e-10;
   const int iters = adi_jacobi_iters(a.dz, a.kmin, implicit_x ? a.dx : a.dy, tol);
   for (int b = 0; b < a.batch; ++b) {
     TriArgs<R> ab = a;
