@@ -8,9 +8,10 @@
 // clamped at 0).  The two off-diagonals are -i/(2 k d^2), so the Jacobi
 // iteration x <- D^-1 (b - O x) contracts by rho = 2|off|/|diag| <= h/(k d^2)
 // per sweep.  With rho <= 5e-3 (the 2048^2 / 20 nm grid: rho = 4.2e-4) K sweeps
-// give an error below rho^K relative, and K is chosen so that rho^K < 1e-17: the
-// result is the exact solution to FP64 rounding, like the reference's Thomas
-// sweep.  Less dominant systems use the sequential kernel.
+// give an error below rho^K relative.  For complex128 fields K is chosen so that
+// rho^K < 1e-17: the result is the exact solution to FP64 rounding, like the
+// reference's Thomas sweep.  Complex64 fields stop at rho^K < 1e-10, below the
+// rounding of their stored values.  Less dominant systems use the sequential kernel.
 //
 // Tiles.  A work-group holds TL lines x TP positions; each lane holds ADI_C
 // consecutive positions of one line in registers.  The K-sweep iterate at a
