@@ -436,7 +436,7 @@ def bench_served(args, ctx):
     t_sub = time.perf_counter() - t0
     lat = []
     failed = 0
-    for jid in ids:
+    for jid in ids:  # in arrival order
         rc, ms, _ = srv.wait(jid, timeout=600.0)
         if rc == P.OK:
             lat.append(ms)
@@ -447,6 +447,10 @@ def bench_served(args, ctx):
     srv.close()
     offered = sum(r["cells"] * r["nsteps"] for r in trace)
     lat = np.array(lat)
+    # queue growth: latency by arrival order.  Rising quarter medians mean the offered
+    # load exceeds what the device serves and the queue (and p99) grow with the trace.
+    quarters = [round(float(np.percentile(q, 50)), 1) for q in np.array_split(lat, 4) if q.size]
+    halves = [round(float(np.percentile(h, 99)), 1) for h in np.array_split(lat, 2) if h.size]
     log(f"served {len(trace)} requests in {wall:.2f} s (submitted over {t_sub:.2f} s), failed {failed}")
     return line("Mcell*steps/s served, with p50/p99 request latency (mixed FFT3D + CN stream)",
                 st["cell_steps"] / wall / 1e6, "Mcell*steps/s", len(trace), 0, wall * 1e3 / len(trace),
@@ -458,7 +462,8 @@ def bench_served(args, ctx):
                 p99_ms=round(float(np.percentile(lat, 99)), 2) if lat.size else None,
                 mean_ms=round(float(lat.mean()), 2) if lat.size else None, failed=failed,
                 offered_mcell_steps_per_s=round(offered / trace[-1]["t"] / 1e6, 1),
-                utilisation=round(st["busy_s"] / wall, 3), wall_s=round(wall, 3))
+                utilisation=round(st["busy_s"] / wall, 3), wall_s=round(wall, 3),
+                p50_ms_by_arrival_quarter=quarters, p99_ms_first_second_half=halves)
 
 
 def bench_scatter(args, ctx):
