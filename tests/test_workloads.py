@@ -56,3 +56,21 @@ def test_curved_guide_jobs_and_trace():
     assert {r["n"] for r in tr if r["kind"] == "fft3d"} == {256, 512, 1024, 2048}
     # mean arrival rate within 20 % of 200 / s
     assert 160 < len(tr) / t[-1] < 240
+
+
+def test_generated_workloads_match_fixtures():
+    # the committed inputs of the ADI and batched-CN benchmarks (floats round-trip
+    # exactly through JSON)
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ph = json.load(open(os.path.join(here, "adi_phantom_64.json")))
+    assert [list(s) for s in W.adi_phantom()] == ph["spheres"]
+    jobs = json.load(open(os.path.join(here, "cn_jobs_256.json")))["jobs"]
+    gen = W.curved_guide_jobs()
+    assert len(gen) == len(jobs) == 256
+    for g, j in zip(gen, jobs):
+        for k, v in j.items():
+            assert g[k] == v, k
+    nx = [j["nx"] for j in jobs]
+    assert min(nx) >= 256 and max(nx) <= 8192
