@@ -239,6 +239,7 @@ static int scatter(const std::string &mode, int n, int nsteps, const char *out, 
     simulation.setMaterial(ball);
   else
     simulation.setMaterial(csg);
+  if (mode == "scatter_adi") simulation.propagator = GenericScattering::SolverType_t::ADI;
   simulation.solve();
   arma::mat ff;
   simulation.getFarField(ff);

@@ -112,7 +112,7 @@ def test_dropin_2d(tmp_path, mode, nx):
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="drop-in driver not built")
 @pytest.mark.parametrize("mode,prec,tol", [("scatter", 0, 1e-9), ("scatter_user", 0, 1e-9),
-                                           ("scatter", 1, 3e-5)])
+                                           ("scatter", 1, 3e-5), ("scatter_adi", 0, 1e-9)])
 def test_dropin_generic_scattering_far_field(tmp_path, mode, prec, tol):
     # GenericScattering as Examples/sphere.cpp drives it (128 points per side, 32
     # steps): reference run, sphere run and far field, against the oracle flow
@@ -126,8 +126,9 @@ def test_dropin_generic_scattering_far_field(tmp_path, mode, prec, tol):
     g_ref = O.grid(g.xmin, g.xmax, g.dx, zmin, zmax, dz_ref, g.k)
     csg = spheres_scene([(0.0, 0.0, 0.0, r, 8.9e-6, 7e-7)])
     u0 = O.gauss3d_field(g, O.Gauss(g.k, 1.0, 0.0, 400.0 * r, 0.0, 0.0, 0.0, 0.0), n)
-    ref = O.fft3d_run(g_ref, O.CSGMaterial(*csg, reference_run=True), u0, 1, nz_ref - 1)
-    sol = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps)
+    prop = O.adi_run if mode == "scatter_adi" else O.fft3d_run  # the propagator the example selects
+    ref = prop(g_ref, O.CSGMaterial(*csg, reference_run=True), u0, 1, nz_ref - 1)
+    sol = prop(g, O.CSGMaterial(*csg), u0, 1, nsteps)
     exp = O.far_field(sol, ref, 1024, g.k, g.dx, g.dy, -0.05, 0.05, 512, 512)
     ff = np.fromfile(out, dtype=np.float64).reshape(exp.shape, order="F")
     err = rel_l2(ff, exp)
