@@ -144,6 +144,11 @@ def test_far_field_errors(ctx):
     with pytest.raises(P.PxgError) as e:
         p2.far_field(0, None, 0, 1024)
     assert e.value.code == 1
+    # the plan as its own reference: the difference, and the far field, vanish
+    assert not np.any(plan.far_field(0, plan, 0, 256, -3.0, 3.0))
+    with pytest.raises(P.PxgError) as e:
+        plan.far_field(0, plan, 1, 256)  # reference field index out of range
+    assert e.value.code == 1
     # no reference: the plain far field of the field
     f = plan.get_field()
     out = plan.far_field(0, None, 0, 256, -3.0, 3.0)
