@@ -159,6 +159,7 @@ struct pxg_plan_s {
   DBuf jobs_reg_dev;  // CN jobs marched by k_cn_reg
   DBuf jobs_big_dev;  // CN jobs marched by k_cn_big
   int njobs_glob = 0, njobs_reg = 0, njobs_big = 0, max_P_reg = 1, max_P_big = 1;
+  bool reg_fixed = false;  // register-class jobs march with the factored matrix (k_cn_fixed)
   // extra streams: the CN job sets march concurrently
   hipStream_t aux = nullptr, aux2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr, join2_ev = nullptr;
@@ -648,6 +649,10 @@ void build_cn_jobs(pxg_plan P)
     }
   }
   P->njobs_reg = static_cast<int>(reg.size());
+  // the fixed-matrix march when every register-class job has a z-invariant material
+  P->reg_fixed = !reg.empty() && P->max_P_reg <= cn_fixed_max_p();
+  for (const CnJob &J : reg)
+    if (!(J.mat.mode == MAT_NONE || (J.mat.mode == MAT_PLANES && J.mat.z_invariant))) P->reg_fixed = false;
   P->njobs_big = static_cast<int>(big.size());
   P->njobs_glob = static_cast<int>(glob.size());
   if (!big.empty()) {
@@ -717,10 +722,17 @@ void run_cn(pxg_plan P, unsigned first, unsigned nsteps)
   }
   if (P->njobs_reg) {
     const int ev0 = begin(P);
-    record(P, "cn_march_reg", bytes,
-           launch_cn_reg<R>(P->jobs_reg_dev.as<CnJob>(), P->njobs_reg, P->max_P_reg, P->field.as<cx<R>>(),
-                            P->sol.as<cx<R>>(), first, nsteps, P->stream),
-           ev0);
+    // the factorization is paid once per call: short calls keep the plain march
+    if (P->reg_fixed && nsteps >= 8)
+      record(P, "cn_march_fixed", bytes,
+             launch_cn_fixed<R>(P->jobs_reg_dev.as<CnJob>(), P->njobs_reg, P->max_P_reg, P->field.as<cx<R>>(),
+                                P->sol.as<cx<R>>(), first, nsteps, P->stream),
+             ev0);
+    else
+      record(P, "cn_march_reg", bytes,
+             launch_cn_reg<R>(P->jobs_reg_dev.as<CnJob>(), P->njobs_reg, P->max_P_reg, P->field.as<cx<R>>(),
+                              P->sol.as<cx<R>>(), first, nsteps, P->stream),
+             ev0);
   }
   if (gs != P->stream) {
     hip_check(hipEventRecord(P->join_ev, gs), "join");

@@ -533,6 +533,366 @@ __global__ __launch_bounds__(CN_MAX_P) void k_cn_reg(const CnJob *__restrict__ j
     if (j < len) fld[s + j] = cvt<R>(u[j]);
 }
 
+// CN march for a z-invariant material (z-invariant planes, or none): every
+// row of the system is the same at every step except the transparent-boundary
+// terms, a change of the diagonal at rows 0 and N-1 (cn_row_m).  So the system
+// without them, A0, is factored once per call, with k_cn_reg's elimination
+// (chunks, then PCR over the chunk ends): the local factors and, per PCR level
+// and reduced row, the row's (a, c) and the reciprocal of its new diagonal, all
+// kept in registers.  A step then carries only its right-hand side through the
+// factors, y = A0^-1 d, with no complex reciprocal on its path, and applies the
+// boundary terms by the Woodbury identity, A = A0 + U D U^T with U = [e0, eN],
+// D = diag(d0, dN):
+//   x = y - W (I + D G)^-1 D U^T y,  W = A0^-1 U (two solves, once), G = U^T W.
+// Lanes are limited to CN_FIXED_P so that the PCR factors of every level fit in
+// registers (N <= 1024 with k_cn_reg's partition).
+constexpr int CN_FIXED_P = 256;
+constexpr int CN_FIXED_LEVELS = 9;  // PCR levels for up to 2 * CN_FIXED_P reduced rows
+
+template <class R, int MM>
+__global__ __launch_bounds__(CN_FIXED_P) void k_cn_fixed(const CnJob *__restrict__ jobs, cx<R> *fields, cx<R> *sol,
+                                                         unsigned first, unsigned nsteps)
+{
+  __shared__ cx<double> ra[4 * CN_FIXED_P], rc[4 * CN_FIXED_P], rd[4 * CN_FIXED_P];
+  __shared__ cx<double> edge_lo[CN_FIXED_P], edge_hi[CN_FIXED_P];
+  __shared__ cx<double> wb[8];
+  const CnJob &J = jobs[blockIdx.x];
+  const int p = threadIdx.x, N = J.nx, P = J.P, bc = J.bc;
+  const bool active = p < P;
+  cx<R> *fld = fields + J.foff;
+  const double k = J.k, dz = J.dz, dx = J.dx;
+  const double rho = dz / (k * dx * dx);
+  const double r = k * dz;
+  const unsigned npix = J.absorb_npix;
+  const double ainv = J.absorb_inv;
+  const size_t soff = J.soff;
+  const int nz = J.nz;
+  const int len0 = N / P, rem = N % P;
+  const int s = p * len0 + min(p, rem);
+  const int len = active ? len0 + (p < rem ? 1 : 0) : 0;  // 2 <= len <= MM for active lanes
+  const int M = 2 * P;
+  const bool dbuf = 2 * M <= 4 * CN_FIXED_P;
+  const bool tbc = bc == PXG_BC_TRANSPARENT && N >= 2;
+  const cx<double> zero{0.0, 0.0}, one{1.0, 0.0};
+
+  cx<double> u[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) u[j] = j < len ? cvt<double>(fld[s + j]) : zero;
+  double af[MM], fd[MM], fb[MM];
+  bool lane_absorbs = false;
+#pragma unroll
+  for (int j = 0; j < MM; ++j) {
+    af[j] = 1.0;
+    fd[j] = fb[j] = 0.0;
+    if (j < len) {
+      if (absorbing(s + j, N, npix)) {
+        af[j] = absorb_factor(s + j, N, npix, ainv);
+        lane_absorbs = true;
+      }
+      if (J.mat.mode == MAT_PLANES) {
+        fd[j] = J.mat.planes[2 * (s + j)];
+        fb[j] = J.mat.planes[2 * (s + j) + 1];
+      }
+    }
+  }
+  // rows of A0 (cn_row_m without the boundary terms)
+  const cx<double> off = {0.0, -0.25 * rho};
+  auto a_of = [&](int j) { return s + j > 0 ? off : zero; };
+  auto c_of = [&](int j) { return s + j < N - 1 ? off : zero; };
+  auto b_of = [&](int j) { return cx<double>{1.0 + 0.5 * (fb[j] * r), 0.5 * rho + 0.5 * (fd[j] * r)}; };
+
+  // local factors of the chunk rows 1 .. len-1 and the reduced rows' coefficients
+  cx<double> iden[MM], ah[MM], ch[MM];
+  cx<double> ra0 = zero, rb0 = one, rc0 = zero, ra1 = zero, rc1 = zero;
+#pragma unroll
+  for (int j = 0; j < MM; ++j) iden[j] = ah[j] = ch[j] = zero;
+  if (active) {
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+      if (j < len) {
+        const cx<double> ai = a_of(j), bi = b_of(j), ci = c_of(j);
+        if (j == 1) {
+          iden[j] = crecip(bi);
+          ah[j] = ai * iden[j];
+          ch[j] = ci * iden[j];
+        } else {
+          iden[j] = crecip(bi - ai * ch[j - 1]);
+          ah[j] = (zero - ai * ah[j - 1]) * iden[j];
+          ch[j] = ci * iden[j];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 1; j < MM; ++j)
+      if (j == len - 1) {
+        ra1 = ah[j];
+        rc1 = ch[j];
+      }
+    cx<double> at = zero, ct{-1.0, 0.0};
+#pragma unroll
+    for (int j = MM - 2; j >= 1; --j) {
+      if (j <= len - 2) {
+        at = ah[j] - ch[j] * at;
+        ct = zero - ch[j] * ct;
+      }
+    }
+    ra0 = a_of(0);
+    rb0 = b_of(0) - c_of(0) * at;
+    rc0 = zero - c_of(0) * ct;
+  }
+  // PCR factors: the normalisation of the two reduced rows, then per level the
+  // rows' (a, c) before the level and the reciprocal of the level's diagonal
+  cx<double> nb[2];
+  cx<double> la[CN_FIXED_LEVELS][2], lc[CN_FIXED_LEVELS][2], lb[CN_FIXED_LEVELS][2];
+  {
+    cx<double> a[2] = {zero, zero}, c[2] = {zero, zero};
+    nb[0] = nb[1] = one;
+    if (active) {
+      nb[0] = crecip(rb0);
+      nb[1] = crecip(one);
+      a[0] = ra0 * nb[0];
+      c[0] = rc0 * nb[0];
+      a[1] = ra1 * nb[1];
+      c[1] = rc1 * nb[1];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        ra[2 * p + q] = a[q];
+        rc[2 * p + q] = c[q];
+      }
+    }
+    lds_barrier();
+    int base = 0;
+#pragma unroll
+    for (int lv = 0; lv < CN_FIXED_LEVELS; ++lv) {
+      const int h = 1 << lv;
+      if (h >= M) break;
+      cx<double> na[2], nc[2];
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int row = 2 * p + q;
+          cx<double> am = zero, cm = zero, ap = zero, cp = zero;
+          if (row - h >= 0) {
+            am = ra[base + row - h];
+            cm = rc[base + row - h];
+          }
+          if (row + h < M) {
+            ap = ra[base + row + h];
+            cp = rc[base + row + h];
+          }
+          const cx<double> ib = crecip(one - a[q] * cm - c[q] * ap);
+          la[lv][q] = a[q];
+          lc[lv][q] = c[q];
+          lb[lv][q] = ib;
+          na[q] = (zero - a[q] * am) * ib;
+          nc[q] = (zero - c[q] * cp) * ib;
+        }
+      }
+      if (dbuf) {
+        base = M - base;
+      } else {
+        lds_barrier();
+      }
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          a[q] = na[q];
+          c[q] = nc[q];
+          ra[base + 2 * p + q] = a[q];
+          rc[base + 2 * p + q] = c[q];
+        }
+      }
+      lds_barrier();
+    }
+  }
+
+  // y = A0^-1 d for the lane's rows (every lane calls it: it holds barriers)
+  auto solve = [&](const cx<double> (&d)[MM], cx<double> (&y)[MM]) {
+    cx<double> dh[MM];
+#pragma unroll
+    for (int j = 0; j < MM; ++j) dh[j] = zero;
+    cx<double> d0 = zero, d1 = zero;
+    if (active) {
+#pragma unroll
+      for (int j = 1; j < MM; ++j) {
+        if (j < len) dh[j] = (j == 1) ? d[j] * iden[j] : (d[j] - a_of(j) * dh[j - 1]) * iden[j];
+      }
+      cx<double> dt = zero;
+#pragma unroll
+      for (int j = MM - 2; j >= 1; --j)
+        if (j <= len - 2) dt = dh[j] - ch[j] * dt;
+      cx<double> dl = zero;
+#pragma unroll
+      for (int j = 1; j < MM; ++j)
+        if (j == len - 1) dl = dh[j];
+      d0 = (d[0] - c_of(0) * dt) * nb[0];
+      d1 = dl * nb[1];
+      rd[2 * p] = d0;
+      rd[2 * p + 1] = d1;
+    }
+    lds_barrier();
+    int base = 0;
+#pragma unroll
+    for (int lv = 0; lv < CN_FIXED_LEVELS; ++lv) {
+      const int h = 1 << lv;
+      if (h >= M) break;
+      cx<double> nd[2];
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int row = 2 * p + q;
+          const cx<double> dm = row - h >= 0 ? rd[base + row - h] : zero;
+          const cx<double> dp = row + h < M ? rd[base + row + h] : zero;
+          nd[q] = ((q ? d1 : d0) - la[lv][q] * dm - lc[lv][q] * dp) * lb[lv][q];
+        }
+      }
+      if (dbuf) {
+        base = M - base;
+      } else {
+        lds_barrier();
+      }
+      if (active) {
+        d0 = nd[0];
+        d1 = nd[1];
+        rd[base + 2 * p] = d0;
+        rd[base + 2 * p + 1] = d1;
+      }
+      lds_barrier();
+    }
+    if (active) {
+      cx<double> xn = d1;
+#pragma unroll
+      for (int j = MM - 1; j >= 1; --j) {
+        if (j == len - 1) {
+          y[j] = d1;
+        } else if (j < len - 1) {
+          y[j] = dh[j] - ah[j] * d0 - ch[j] * xn;
+          xn = y[j];
+        }
+      }
+      y[0] = d0;
+    }
+  };
+
+  // Woodbury columns W = A0^-1 [e0, eN] and G = U^T W
+  cx<double> w0[MM], wn[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) w0[j] = wn[j] = zero;
+  cx<double> g00 = zero, g01 = zero, g10 = zero, g11 = zero;
+  if (tbc) {
+    cx<double> e[MM];
+#pragma unroll
+    for (int j = 0; j < MM; ++j) e[j] = (active && j < len && s + j == 0) ? one : zero;
+    solve(e, w0);
+#pragma unroll
+    for (int j = 0; j < MM; ++j) e[j] = (active && j < len && s + j == N - 1) ? one : zero;
+    solve(e, wn);
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < MM; ++j) {
+        if (j < len && s + j == 0) {
+          wb[0] = w0[j];
+          wb[1] = wn[j];
+        }
+        if (j < len && s + j == N - 1) {
+          wb[2] = w0[j];
+          wb[3] = wn[j];
+        }
+      }
+    }
+    lds_barrier();
+    g00 = wb[0];
+    g01 = wb[1];
+    g10 = wb[2];
+    g11 = wb[3];
+    lds_barrier();
+  }
+
+  for (unsigned it = 0; it < nsteps; ++it) {
+    const unsigned st = first + it;
+    cx<double> ulast = u[0], u2last = u[0];
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+      if (j == len - 1) ulast = u[j];
+      if (j == len - 2) u2last = u[j];
+    }
+    if (active) {
+      edge_lo[p] = u[0];
+      edge_hi[p] = ulast;
+    }
+    lds_barrier();
+    const cx<double> ul = (active && p > 0) ? edge_hi[p - 1] : zero;
+    const cx<double> ur = (active && p < P - 1) ? edge_lo[p + 1] : zero;
+    // right-hand side (cn_row_m) and the boundary rows' diagonal changes
+    cx<double> d[MM], y[MM];
+    cx<double> dl0 = zero, dlN = zero;
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+      d[j] = y[j] = zero;
+      if (active && j < len) {
+        const cx<double> ulj = j == 0 ? ul : u[j > 0 ? j - 1 : 0];
+        const cx<double> urj = (j + 1 < len) ? u[j + 1 < MM ? j + 1 : 0] : ur;
+        cx<double> v = (ulj + urj) * cx<double>{0.0, 0.25 * rho};
+        v = v - cx<double>{-0.5 * rho * u[j].y, 0.5 * rho * u[j].x};
+        v = v + cx<double>{1.0 - 0.5 * (fb[j] * r), -0.5 * (fd[j] * r)} * u[j];
+        const int i = s + j;
+        if (tbc && (i == 0 || i == N - 1)) {
+          const cx<double> uin = i == 0 ? u[j + 1 < MM ? j + 1 : 0] : u2last;
+          cx<double> e;
+          if (tbc_factor(u[j], uin, 1e-16, e)) {
+            const cx<double> q = cx<double>{0.0, 0.25 * rho} * e;
+            v = v + q * u[j];
+            if (i == 0) dl0 = zero - q;
+            if (i == N - 1) dlN = zero - q;
+          }
+        }
+        d[j] = v;
+      }
+    }
+    solve(d, y);
+    if (tbc) {
+      // x = y - W (I + D G)^-1 D [y0; yN]
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < MM; ++j) {
+          if (j < len && s + j == 0) {
+            wb[4] = y[j];
+            wb[6] = dl0;
+          }
+          if (j < len && s + j == N - 1) {
+            wb[5] = y[j];
+            wb[7] = dlN;
+          }
+        }
+      }
+      lds_barrier();
+      const cx<double> y0 = wb[4], yn = wb[5], e0 = wb[6], en = wb[7];
+      const cx<double> m00 = one + e0 * g00, m01 = e0 * g01, m10 = en * g10, m11 = one + en * g11;
+      const cx<double> r0 = e0 * y0, r1 = en * yn;
+      const cx<double> idet = crecip(m00 * m11 - m01 * m10);
+      const cx<double> c0 = (m11 * r0 - m01 * r1) * idet, c1 = (m00 * r1 - m10 * r0) * idet;
+#pragma unroll
+      for (int j = 0; j < MM; ++j) y[j] = y[j] - w0[j] * c0 - wn[j] * c1;
+    }
+    if (active) {
+      cx<R> *sc = (sol && soff != ~static_cast<size_t>(0) && static_cast<int>(st) < nz)
+                      ? sol + soff + static_cast<size_t>(st) * N
+                      : nullptr;
+#pragma unroll
+      for (int j = 0; j < MM; ++j) {
+        if (j < len) {
+          u[j] = lane_absorbs ? scale(y[j], af[j]) : y[j];
+          if (sc) sc[s + j] = cvt<R>(u[j]);
+        }
+      }
+    }
+    lds_barrier();  // edge arrays, the reduced system and wb are reused next step
+  }
+#pragma unroll
+  for (int j = 0; j < MM; ++j)
+    if (j < len) fld[s + j] = cvt<R>(u[j]);
+}
+
 // CN march for chunks of 5..MM rows (N <= MM * CN_MAX_P).  The field stays in
 // registers as in k_cn_reg; the elimination coefficients of the chunk rows go
 // to the job's scratch (coalesced [row][lane]) and are read back once, by the
@@ -702,6 +1062,18 @@ hipError_t launch_cn_reg(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fie
                      nsteps);
   return hipGetLastError();
 }
+
+template <class R>
+hipError_t launch_cn_fixed(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
+                           unsigned first, unsigned nsteps, hipStream_t s)
+{
+  if (max_P > CN_FIXED_P) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_cn_fixed<R, CN_REG_ROWS>), dim3(njobs), dim3(max_P), 0, s, jobs_dev, fields, sol, first,
+                     nsteps);
+  return hipGetLastError();
+}
+
+int cn_fixed_max_p() { return CN_FIXED_P; }
 
 template <class R>
 hipError_t launch_cn_jobs(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, void *scratch,
@@ -919,6 +1291,10 @@ hipError_t launch_thomas_batched(int layout, int nsys, int n, cx<double> *diag,
 }
 
 int cn_reg_rows() { return CN_REG_ROWS; }
+template hipError_t launch_cn_fixed<float>(const CnJob *, int, int, cx<float> *, cx<float> *, unsigned,
+                                           unsigned, hipStream_t);
+template hipError_t launch_cn_fixed<double>(const CnJob *, int, int, cx<double> *, cx<double> *, unsigned,
+                                            unsigned, hipStream_t);
 template hipError_t launch_cn_reg<float>(const CnJob *, int, int, cx<float> *, cx<float> *, unsigned, unsigned,
                                          hipStream_t);
 template hipError_t launch_cn_reg<double>(const CnJob *, int, int, cx<double> *, cx<double> *, unsigned,

@@ -76,6 +76,13 @@ template <class R>
 hipError_t launch_cn_reg(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
                          unsigned first, unsigned nsteps, hipStream_t s);
 int cn_reg_rows();
+// Register-resident march with the matrix factored once (z-invariant material:
+// planes or none), boundary terms by the Woodbury identity: reg-class jobs of
+// at most cn_fixed_max_p() lanes.
+template <class R>
+hipError_t launch_cn_fixed(const CnJob *jobs_dev, int njobs, int max_P, cx<R> *fields, cx<R> *sol,
+                           unsigned first, unsigned nsteps, hipStream_t s);
+int cn_fixed_max_p();
 // Register field, scratch coefficients: jobs whose chunks have more than
 // cn_reg_rows() and at most cn_big_rows() rows.
 template <class R>

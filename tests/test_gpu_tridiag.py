@@ -63,6 +63,44 @@ def test_cn_slab_vs_oracle(ctx, nx, nsteps, absorb):
     assert rel_l2(plan.get_solution(), sol_ref) < 1e-10
 
 
+@pytest.mark.parametrize("nx,bc,absorb,material", [(1024, P.BC_TRANSPARENT, 0, "slab"),
+                                                    (777, P.BC_TRANSPARENT, 50, "slab"),
+                                                    (300, P.BC_DIRICHLET, 50, "slab"),
+                                                    (512, P.BC_TRANSPARENT, 0, "none"),
+                                                    (6, P.BC_TRANSPARENT, 0, "slab")])
+def test_cn_factored_march_equals_stepwise(ctx, nx, bc, absorb, material):
+    """A z-invariant CN call of many steps marches with the matrix factored once
+    and the transparent-boundary rows applied by the Woodbury identity
+    (k_cn_fixed); calls of one step take the plain elimination (k_cn_reg).  The
+    two agree to FP64 rounding, on the exit field and on every stored step."""
+    nsteps, dz = 120, 500.0
+    announce("cn factored vs stepwise", nx, bc, absorb, material)
+    xmin, dx = -500.0, 1000.0 / (nx - 1)
+    x = xmin + dx * np.arange(nx)
+    prob = P.problem(P.CN, nx, nsteps + 1, dx, dz, K_01, xmin=xmin, bc=bc,
+                     absorb=(absorb, 0, 0.1, 0.0), store_dsx=1, store_dsz=1)
+    u0 = np.exp(-((x + 60.0) / 150.0) ** 2) * np.exp(0.02j * x) + 0.1
+    out = []
+    for stepwise in (False, True):
+        plan = P.Plan(ctx, prob)
+        if material == "slab":
+            plan.material_planes(slab_planes(x, -50.0, 50.0), z_invariant=True)
+        else:
+            plan.material_none()
+        plan.set_field(u0)
+        if stepwise:
+            for st in range(1, nsteps + 1):
+                plan.step(st, 1)
+        else:
+            plan.step(1, nsteps)
+        out.append((plan.get_field(), plan.get_solution()))
+        plan.close()
+    err = rel_l2(out[0][0], out[1][0])
+    announce("rel_l2 field", err)
+    assert err < 1e-12
+    assert rel_l2(out[0][1], out[1][1]) < 1e-12
+
+
 def test_cn_curved_waveguide_and_dirichlet(ctx):
     nx, nsteps, dz = 777, 150, 100.0
     announce("cn curved", nx, nsteps)
