@@ -555,8 +555,46 @@ def bench_scatter(args, ctx):
                 far_field_shape=list(ff.shape))
 
 
+def bench_post(args, ctx):
+    """The exit-field post-processing modules (src/postProcessMod.cpp:7-151) on a 4096^2
+    FP64 field held by a device plan: ExitIntensity (|u|^2), ExitPhase (arg u) and
+    ExitField (Re u) as full matrices and resized to 512^2, and the 8-bit |u| image.
+    Each call includes the result's copy to the host, which dominates at this size;
+    the kernels' own times come from a kernel trace of this workload."""
+    n = 4096
+    rng = np.random.default_rng(11)
+    u = np.asfortranarray(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    plan = P.Plan(ctx, P.problem(P.ADI, n, 2, 1.0, 1.0, 60.0, precision=P.PREC_FP64))
+    plan.set_field(u)
+    plan.sync()
+    calls = {"exit_intensity": lambda: plan.field_quantity(P.Q_ABS2),
+             "exit_phase": lambda: plan.field_quantity(P.Q_ARG),
+             "exit_field": lambda: plan.field_quantity(P.Q_REAL),
+             "exit_intensity_512": lambda: plan.field_quantity(P.Q_ABS2, export_rows=512, export_cols=512),
+             "uint8_field": lambda: plan.field_uint8(source=P.SRC_FIELD)}
+    reps = max(3, args.steps or 10)
+    res = {}
+    for name, fn in calls.items():
+        for _ in range(max(1, min(args.warmup, 3))):
+            fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        res[name] = round(float(np.median(ts)) * 1e3, 3)
+        log(f"post {name}: {res[name]:.3f} ms per call")
+    plan.close()
+    t = res["exit_intensity"] * 1e-3
+    return line("Mcell/s of the ExitIntensity module on a 4096^2 FP64 field (result copied to the host)",
+                n * n / t / 1e6, "Mcell/s", reps, args.warmup, t * 1e3, "fp64",
+                "synthetic: complex Gaussian noise field",
+                {"workload": "post-processing modules on a 4096x4096 FP64 field"}, None, None,
+                call_ms=res, note="kernel durations: kernel trace of this workload")
+
+
 WORKLOADS = {"cn2d": bench_cn2d, "adi3d": bench_adi3d, "cn_jobs": bench_cn_jobs, "served": bench_served,
-             "scatter": bench_scatter}
+             "scatter": bench_scatter, "post": bench_post}
 
 
 def main():

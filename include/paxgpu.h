@@ -248,6 +248,39 @@ int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, d
 int pxg_intensity_trapezoid(pxg_plan plan, int b, int n, const int *steps, const int *ix_start,
                             const int *ix_end, double *sums);
 
+/* ---- field quantities of the post-processing modules ------------------------- */
+/* Replace the result() methods of post::Intensity, Phase, ExitField, ExitIntensity,
+ * ExitPhase, IntensityUint8 and LogIntensityUint8 (src/postProcessMod.cpp:7-151), computed
+ * on the device from what the solver holds there:
+ *   source PXG_SRC_FIELD     the current field b: Solver::getLastSolution (2D, a vector
+ *                            of nx) or getLastSolution3D (3D, an nx x nx matrix);
+ *   source PXG_SRC_SOLUTION  the stored solution of field/job b: Solver::getSolution (2D,
+ *                            nx x stored steps) or getSolution3D (3D, the block-averaged
+ *                            cube); needs store_dsz > 0.
+ * quantity: PXG_Q_ABS |u| (Intensity), PXG_Q_ABS2 |u|^2 (ExitIntensity), PXG_Q_ARG
+ * atan2(Im u, Re u) (Phase, ExitPhase), PXG_Q_REAL Re u (ExitField).  When the result is
+ * a matrix and export_rows, export_cols are both nonzero,
+ * PostProcessingModule::resizeMatrix (src/postProcessing.cpp:29-49) follows: block means
+ * down to export_rows x export_cols, unless either size reaches the matrix's (then it is
+ * kept).  Vectors and cubes are never resized (export sizes ignored), as in the modules.
+ * pxg_field_quantity_dims gives (rows, cols, slices); pxg_field_quantity writes that many
+ * doubles to out (host, column-major) after the plan's pending work.
+ * pxg_field_uint8 writes rows*cols*slices bytes: log_scale 0 is IntensityUint8,
+ * 255 |u| / max|u|; log_scale 1 is LogIntensityUint8, log|u| mapped from [bot, top] to
+ * [0, 255] and clamped, top = log max|u| lowered to log(*max_value) when max_value is
+ * given (setMaxValue), bot = log min|u| raised to log(*min_value) when min_value is given
+ * (setMinValue); NULL = not set.  Values truncate to unsigned char as the reference's
+ * assignment does; where the reference converts a NaN (a field of zeros, top = bot) the
+ * byte is 0.  No resize (the reference's 8-bit modules have none). */
+enum { PXG_SRC_FIELD = 0, PXG_SRC_SOLUTION = 1 };
+enum { PXG_Q_ABS = 0, PXG_Q_ABS2 = 1, PXG_Q_ARG = 2, PXG_Q_REAL = 3 };
+int pxg_field_quantity_dims(pxg_plan plan, int b, int source, unsigned export_rows,
+                            unsigned export_cols, int dims[3]);
+int pxg_field_quantity(pxg_plan plan, int b, int source, int quantity, unsigned export_rows,
+                       unsigned export_cols, double *out);
+int pxg_field_uint8(pxg_plan plan, int b, int source, int log_scale, const double *min_value,
+                    const double *max_value, unsigned char *out);
+
 /* ---- batched tridiagonal solve ---------------------------------------------- */
 /* Replaces ThomasAlgorithm::solve<cdouble> (include/thomasAlgorithm.tpp:2-32) for
  * many independent systems: diag/sub/rhs are host complex128 arrays of nsys

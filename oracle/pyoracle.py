@@ -332,6 +332,67 @@ def resize_matrix(orig, rows, cols):
     return out
 
 
+def field_quantity(u, quantity, export_rows=0, export_cols=0):
+    """The field quantities of the post-processing modules (src/postProcessMod.cpp:7-151):
+    quantity 0 |u| (Intensity, :7-15), 1 |u|^2 (ExitIntensity, :122-136), 2 atan2(Im, Re)
+    (Phase and ExitPhase, :90-104, :138-151), 3 Re u (ExitField, :106-120).  A matrix
+    (2D array) is then resized by resizeMatrix when both export sizes are set; vectors
+    and cubes are returned as they are."""
+    u = np.asarray(u)
+    if quantity == 0:
+        r = np.abs(u)
+    elif quantity == 1:
+        r = np.abs(u) ** 2
+    elif quantity == 2:
+        r = np.arctan2(u.imag, u.real)
+    elif quantity == 3:
+        r = u.real.copy()
+    else:
+        raise ValueError(quantity)
+    if r.ndim == 2 and export_rows and export_cols:
+        r = resize_matrix(np.asfortranarray(r), export_rows, export_cols)
+    return r
+
+
+def _to_uint8(v):
+    """Assignment of a double in [0, 256) to unsigned char: truncation.  Values the
+    reference would convert with undefined behaviour (NaN) become 0."""
+    v = np.asarray(v, dtype=np.float64)
+    out = np.zeros(v.shape, dtype=np.uint8)
+    ok = (v >= 0.0) & (v < 256.0)
+    out[ok] = np.trunc(v[ok]).astype(np.uint8)
+    return out
+
+
+def intensity_uint8(u):
+    """post::IntensityUint8::result (src/postProcessMod.cpp:17-30): 255 |u| / |max u|,
+    where Armadillo's max() of a complex cube is the element of largest modulus."""
+    a = np.abs(np.asarray(u))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return _to_uint8(255.0 * a / a.max())
+
+
+def log_intensity_uint8(u, min_value=None, max_value=None):
+    """post::LogIntensityUint8::result (src/postProcessMod.cpp:32-76): log|u| mapped from
+    [bot, top] to [0, 255], clamped outside.  top = log|max u|, lowered to
+    log(max_value) when setMaxValue was called and log|max u| exceeds it; bot =
+    log|min u|, raised to log(min_value) when setMinValue was called and log|min u| is
+    below it (setMinValue / setMaxValue store logarithms, :78-88)."""
+    a = np.abs(np.asarray(u))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lmax, lmin = np.log(a.max()), np.log(a.min())
+        top, bot = lmax, lmin
+        if max_value is not None:
+            hi = np.log(max_value)
+            top = hi if lmax > hi else lmax
+        if min_value is not None:
+            lo = np.log(min_value)
+            bot = lo if lmin < lo else lmin
+        l = np.log(a)
+        v = np.where(l > top, 255.0, np.where(l < bot, 0.0, 255.0 * (l - bot) / (top - bot)))
+    return _to_uint8(v)
+
+
 def _ref_fftshift(a, axis):
     """post::FarField::fftshift (src/farFieldPost.cpp:60-71): swaps element i with
     i + N/2 for i < N/2 (for odd N the last element stays)."""

@@ -17,13 +17,6 @@ void post::FarField::setAngleRange(double angMin, double angMax)
   phiMax = angMax;
 }
 
-void post::FarField::setExportDimensions(unsigned int rows, unsigned int cols)
-{
-  exportRows = rows;
-  exportCols = cols;
-  resizeMatrices = true;
-}
-
 void post::FarField::verifyConsistentAngles(double dx, double dy, double wavenumber) const
 {
   const double d = dx > dy ? dy : dx;

@@ -1812,6 +1812,134 @@ int pxg_intensity_trapezoid(pxg_plan P, int b, int n, const int *steps, const in
   });
 }
 
+namespace {
+
+// What a post-processing module reads: the current field or the stored solution
+// of field/job b, as (rows, cols, slices) of complex cells on the device.
+struct QuantitySource {
+  const void *p = nullptr;
+  int c64 = 0;
+  int dims[3] = {0, 0, 0};
+  bool matrix = false;  // a 2D result, which resizeMatrix applies to
+  size_t cells() const { return static_cast<size_t>(dims[0]) * dims[1] * dims[2]; }
+};
+
+QuantitySource quantity_source(pxg_plan P, int b, int source)
+{
+  check_b(P, b);
+  const pxg_problem &q = P->is_jobs ? P->jp[b] : P->p;
+  const bool is3d = q.kind == PXG_FFT3D || q.kind == PXG_ADI;
+  QuantitySource S;
+  if (source == PXG_SRC_FIELD) {
+    if (!P->field_set) fail(PXG_ERR_STATE, "field not set (pxg_set_field)");
+    S.p = P->field.as<char>() + P->foff[b] * P->esz;
+    S.c64 = P->esz == 8;
+    S.dims[0] = q.nx;
+    S.dims[1] = is3d ? q.nx : 1;
+    S.dims[2] = 1;
+    S.matrix = is3d;
+  } else if (source == PXG_SRC_SOLUTION) {
+    if (q.store_dsz <= 0) fail(PXG_ERR_STATE, "plan stores no solution (store_dsz = 0)");
+    const int esz = (is3d || !P->mixed) ? 16 : 8;
+    S.p = P->sol.as<char>() + P->soff[b] * esz;
+    S.c64 = esz == 8;
+    if (P->is_jobs) {
+      S.dims[0] = q.nx;
+      S.dims[1] = q.nz;
+      S.dims[2] = 1;
+    } else {
+      for (int i = 0; i < 3; ++i) S.dims[i] = P->sol_dims[i];
+    }
+    S.matrix = !is3d;
+  } else {
+    fail(PXG_ERR_INVALID, "unknown source %d", source);
+  }
+  if (S.cells() == 0) fail(PXG_ERR_STATE, "empty %s", source == PXG_SRC_FIELD ? "field" : "solution");
+  return S;
+}
+
+// resizeMatrix applies to matrices when both export sizes are set and both are
+// below the matrix's (src/postProcessing.cpp:33-37 keeps the matrix otherwise)
+bool quantity_resized(const QuantitySource &S, unsigned er, unsigned ec)
+{
+  return S.matrix && er > 0 && ec > 0 && er < static_cast<unsigned>(S.dims[0]) &&
+         ec < static_cast<unsigned>(S.dims[1]);
+}
+
+}  // namespace
+
+int pxg_field_quantity_dims(pxg_plan P, int b, int source, unsigned export_rows,
+                            unsigned export_cols, int dims[3])
+{
+  return guarded([&] {
+    if (!dims) fail(PXG_ERR_INVALID, "null output");
+    const QuantitySource S = quantity_source(P, b, source);
+    const bool rs = quantity_resized(S, export_rows, export_cols);
+    dims[0] = rs ? static_cast<int>(export_rows) : S.dims[0];
+    dims[1] = rs ? static_cast<int>(export_cols) : S.dims[1];
+    dims[2] = S.dims[2];
+  });
+}
+
+int pxg_field_quantity(pxg_plan P, int b, int source, int quantity, unsigned export_rows,
+                       unsigned export_cols, double *out)
+{
+  return guarded([&] {
+    const QuantitySource S = quantity_source(P, b, source);
+    if (quantity < PXG_Q_ABS || quantity > PXG_Q_REAL) fail(PXG_ERR_INVALID, "unknown quantity %d", quantity);
+    if (!out) fail(PXG_ERR_INVALID, "null output");
+    const bool rs = quantity_resized(S, export_rows, export_cols);
+    const size_t n = S.cells();
+    const size_t n_out = rs ? static_cast<size_t>(export_rows) * export_cols : n;
+    const size_t need = n * 8 + (rs ? n_out * 8 : 0);
+    if (P->ctx->reserved.load() + need > P->ctx->budget)
+      fail(PXG_ERR_BUDGET, "field quantity needs %zu bytes of scratch", need);
+    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+    hipStream_t s = P->stream;
+    DBuf res, rsz;
+    res.alloc(n * sizeof(double));
+    hip_check(launch_field_quantity(S.p, S.c64, n, quantity, res.as<double>(), s), "field quantity");
+    const double *src = res.as<double>();
+    if (rs) {
+      rsz.alloc(n_out * sizeof(double));
+      hip_check(launch_ff_resize(res.as<double>(), S.dims[0], S.dims[1], rsz.as<double>(),
+                                 static_cast<int>(export_rows), static_cast<int>(export_cols), s),
+                "field quantity resize");
+      src = rsz.as<double>();
+    }
+    hip_check(hipMemcpyAsync(out, src, n_out * sizeof(double), hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "sync");
+  });
+}
+
+int pxg_field_uint8(pxg_plan P, int b, int source, int log_scale, const double *min_value,
+                    const double *max_value, unsigned char *out)
+{
+  return guarded([&] {
+    const QuantitySource S = quantity_source(P, b, source);
+    if (log_scale != 0 && log_scale != 1) fail(PXG_ERR_INVALID, "log_scale must be 0 or 1");
+    if (!out) fail(PXG_ERR_INVALID, "null output");
+    const size_t n = S.cells();
+    const int G = extrema_groups(n);
+    const size_t need = n + static_cast<size_t>(G) * 16;
+    if (P->ctx->reserved.load() + need > P->ctx->budget)
+      fail(PXG_ERR_BUDGET, "8-bit image needs %zu bytes of scratch", need);
+    // setMinValue / setMaxValue store the logarithm (src/postProcessMod.cpp:78-88)
+    const double lo = min_value ? std::log(*min_value) : 0.0;
+    const double hi = max_value ? std::log(*max_value) : 0.0;
+    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+    hipStream_t s = P->stream;
+    DBuf part, img;
+    part.alloc(static_cast<size_t>(G) * 2 * sizeof(double));
+    img.alloc(n);
+    hip_check(launch_field_uint8(S.p, S.c64, n, part.as<double>(), log_scale, min_value != nullptr, lo,
+                                 max_value != nullptr, hi, img.as<unsigned char>(), s),
+              "8-bit image");
+    hip_check(hipMemcpyAsync(out, img.p, n, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "sync");
+  });
+}
+
 int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
                        double *rhs)
 {

@@ -80,6 +80,14 @@ hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, i
 // (pxg_post.hip): one work-group per entry.
 hipError_t launch_trapz_intensity(const void *sol, int c64, int nx, int n, const int *steps,
                                   const int *ixs, const int *ixe, double *sums, hipStream_t s);
+// Field quantities of n cells (pxg_post.hip): q = 0 |u|, 1 |u|^2, 2 arg u, 3 Re u.
+hipError_t launch_field_quantity(const void *src, int c64, size_t n, int q, double *out, hipStream_t s);
+// Work-groups of the extrema pass of launch_field_uint8; `part` holds 2 doubles per group.
+int extrema_groups(size_t n);
+// 8-bit |u| (linear) or log|u| images of n cells; lo / hi are logarithms.
+hipError_t launch_field_uint8(const void *src, int c64, size_t n, double *part, int log_scale,
+                              int lo_set, double lo, int hi_set, double hi, unsigned char *out,
+                              hipStream_t s);
 // Lines per work-group for an LDS budget of lds_cap bytes (FP64 lines).
 int max_lines(int n, int lds_cap);
 // Stream of a plan and device of a context (server internals).

@@ -40,9 +40,12 @@ EXPORTS = [
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
+    "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
+SRC_FIELD, SRC_SOLUTION = 0, 1
+Q_ABS, Q_ABS2, Q_ARG, Q_REAL = 0, 1, 2, 3
 
 
 class PxgError(RuntimeError):
@@ -138,6 +141,9 @@ def lib():
         L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
         L.pxg_far_field.argtypes = [P, I, P, I, U, D, D, U, U, P]
         L.pxg_intensity_trapezoid.argtypes = [P, I, I, P, P, P, P]
+        L.pxg_field_quantity_dims.argtypes = [P, I, I, U, U, P]
+        L.pxg_field_quantity.argtypes = [P, I, I, I, U, U, P]
+        L.pxg_field_uint8.argtypes = [P, I, I, I, P, P, P]
         L.pxg_plan_set_graphs.argtypes = [P, I]
         _LIB = L
     return _LIB
@@ -268,6 +274,29 @@ class Plan:
         e = np.ascontiguousarray(ix_end, dtype=np.int32)
         out = np.empty(st.size, dtype=np.float64)
         _check(lib().pxg_intensity_trapezoid(self.h, b, st.size, _p(st), _p(a), _p(e), _p(out)))
+        return out
+
+    def field_quantity(self, quantity, source=SRC_FIELD, b=0, export_rows=0, export_cols=0):
+        """|u|, |u|^2, arg u or Re u of the current field or the stored solution, with
+        resizeMatrix for matrices (post::Intensity, ExitIntensity, Phase, ExitPhase,
+        ExitField; src/postProcessMod.cpp:7-151).  Column-major (rows, cols, slices)."""
+        dims = np.zeros(3, dtype=np.int32)
+        _check(lib().pxg_field_quantity_dims(self.h, b, source, export_rows, export_cols, _p(dims)))
+        out = np.empty(tuple(int(d) for d in dims), dtype=np.float64, order="F")
+        _check(lib().pxg_field_quantity(self.h, b, source, quantity, export_rows, export_cols, _p(out)))
+        return out
+
+    def field_uint8(self, source=SRC_SOLUTION, b=0, log_scale=False, min_value=None, max_value=None):
+        """8-bit |u| image (post::IntensityUint8) or log|u| image (post::LogIntensityUint8,
+        min_value / max_value as given to setMinValue / setMaxValue)."""
+        dims = np.zeros(3, dtype=np.int32)
+        _check(lib().pxg_field_quantity_dims(self.h, b, source, 0, 0, _p(dims)))
+        out = np.empty(tuple(int(d) for d in dims), dtype=np.uint8, order="F")
+        lo = None if min_value is None else np.array([min_value], dtype=np.float64)
+        hi = None if max_value is None else np.array([max_value], dtype=np.float64)
+        _check(lib().pxg_field_uint8(self.h, b, source, int(bool(log_scale)),
+                                     None if lo is None else _p(lo), None if hi is None else _p(hi),
+                                     _p(out)))
         return out
 
     def far_field(self, b=0, ref=None, ref_b=0, pad=1024, phi_min=-90.0, phi_max=90.0,

@@ -20,6 +20,84 @@ static void write(const char *path, const cdouble *p, size_t n)
   f.write(reinterpret_cast<const char *>(p), n * sizeof(cdouble));
 }
 
+static void write_raw(const std::string &path, const void *p, size_t bytes)
+{
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(p), bytes);
+}
+
+// The post-processing modules of a 3D run: exit-field quantities (the phase resized
+// to 20 x 30), the cube's intensity and phase, and the 8-bit images.
+static void post3d(const Solver &solver, const std::string &out)
+{
+  post::ExitIntensity ei;
+  post::ExitPhase ep;
+  post::ExitField ef;
+  post::Intensity in;
+  post::Phase ph;
+  post::IntensityUint8 u8;
+  post::LogIntensityUint8 lu8;
+  if (ei.getReturnType(solver) != post::PostProcessingModule::ReturnType_t::matrix2D ||
+      in.getReturnType(solver) != post::PostProcessingModule::ReturnType_t::cube3D || !u8.isUint8 ||
+      lu8.getName() != "LogAmplitude" || ep.getName() != "exitPhase")
+    throw std::runtime_error("post module kinds");
+  arma::mat m;
+  ei.result(solver, m);
+  write_raw(out + ".exitint", m.memptr(), m.n_elem * 8);
+  ep.setExportDimensions(20, 30);
+  ep.result(solver, m);
+  write_raw(out + ".exitphase_rs", m.memptr(), m.n_elem * 8);
+  ef.result(solver, m);
+  write_raw(out + ".exitfield", m.memptr(), m.n_elem * 8);
+  arma::cube c;
+  in.result(solver, c);
+  write_raw(out + ".intcube", c.memptr(), c.n_elem * 8);
+  ph.result(solver, c);
+  write_raw(out + ".phasecube", c.memptr(), c.n_elem * 8);
+  arma::Cube<unsigned char> b;
+  u8.result(solver, b);
+  write_raw(out + ".u8", b.memptr(), b.n_elem);
+  lu8.setMinValue(1e-3);
+  lu8.setMaxValue(0.9);
+  lu8.result(solver, b);
+  write_raw(out + ".logu8", b.memptr(), b.n_elem);
+  bool threw = false;
+  try {
+    arma::vec v;
+    ei.result(solver, v);  // the 2D version on a 3D solver
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  if (!threw) throw std::runtime_error("2D result on a 3D solver did not throw");
+}
+
+// The modules of a 2D run: the solution's intensity and phase (resized to 40 x 7),
+// and the exit quantities as vectors.
+static void post2d(const Solver &solver, const std::string &out)
+{
+  post::Intensity in;
+  post::Phase ph;
+  post::ExitIntensity ei;
+  post::ExitPhase ep;
+  post::ExitField ef;
+  if (in.getReturnType(solver) != post::PostProcessingModule::ReturnType_t::matrix2D ||
+      ei.getReturnType(solver) != post::PostProcessingModule::ReturnType_t::vector1D)
+    throw std::runtime_error("post module kinds");
+  arma::mat m;
+  in.result(solver, m);
+  write_raw(out + ".int2d", m.memptr(), m.n_elem * 8);
+  ph.setExportDimensions(40, 7);
+  ph.result(solver, m);
+  write_raw(out + ".phase2d_rs", m.memptr(), m.n_elem * 8);
+  arma::vec v;
+  ei.result(solver, v);
+  write_raw(out + ".exitint", v.memptr(), v.n_elem * 8);
+  ep.result(solver, v);
+  write_raw(out + ".exitphase", v.memptr(), v.n_elem * 8);
+  ef.result(solver, v);
+  write_raw(out + ".exitfield", v.memptr(), v.n_elem * 8);
+}
+
 static int kats()
 {
   struct K { int op; double a, b, c; double p[3]; double e[3]; } ks[] = {
@@ -128,6 +206,7 @@ static int run3d(const std::string &mode, int n, int nsteps, const char *out, in
   write(out, last.memptr(), last.n_elem);
   const arma::cx_cube &cube = solver->getSolution3D();
   write((std::string(out) + ".cube").c_str(), cube.memptr(), cube.n_elem);
+  if (mode == "fft3d") post3d(*solver, out);
   std::printf("%s n=%d steps=%d cube=%zux%zux%zu\n", mode.c_str(), n, nsteps, (size_t)cube.n_rows,
               (size_t)cube.n_cols, (size_t)cube.n_slices);
   return 0;
@@ -172,6 +251,7 @@ static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
   write(out, last.memptr(), last.n_elem);
   const arma::cx_mat &sol = solver->getSolution();
   write((std::string(out) + ".sol").c_str(), sol.memptr(), sol.n_elem);
+  if (mode == "cn") post2d(*solver, out);
   if (mode == "cn_curved") {
     // CurvedWaveGuideFD::solve also computed the transmittivity after every step
     std::vector<double> tr(1, wg.getTransmittivity().getIntensityAtZero());

@@ -61,6 +61,31 @@ def test_dropin_fft3d(tmp_path, mode):
     ref = O.fft3d_run(g, csg, u0, 1, nsteps, absorber=ab, cube=cube, dsz=dsz)
     assert rel_l2(load(out, (n, n)), ref) < 1e-11
     assert rel_l2(load(out + ".cube", cube.shape), cube) < 1e-11
+    if mode == "fft3d":
+        _check_post3d(out, load(out, (n, n)), load(out + ".cube", cube.shape))
+
+
+def _dbl(path, shape):
+    return np.fromfile(path, dtype=np.float64).reshape(shape, order="F")
+
+
+def _check_post3d(out, last, cube):
+    # post::ExitIntensity / ExitPhase (resized 20 x 30) / ExitField / Intensity / Phase /
+    # IntensityUint8 / LogIntensityUint8 as the driver's solver returned them, against the
+    # oracle's restatement on the same field and cube (last-ulp differences only)
+    n = last.shape[0]
+    for ext, q, shape, er, ec in ((".exitint", 1, (n, n), 0, 0), (".exitphase_rs", 2, (20, 30), 20, 30),
+                                  (".exitfield", 3, (n, n), 0, 0)):
+        exp = O.field_quantity(last, q, er, ec)
+        assert np.max(np.abs(_dbl(out + ext, shape) - exp)) <= 4e-15 * np.max(np.abs(exp)), ext
+    for ext, q in ((".intcube", 0), (".phasecube", 2)):
+        exp = O.field_quantity(cube, q)
+        assert np.max(np.abs(_dbl(out + ext, cube.shape) - exp)) <= 2e-15 * np.max(np.abs(exp)), ext
+    for ext, exp in ((".u8", O.intensity_uint8(cube)),
+                     (".logu8", O.log_intensity_uint8(cube, min_value=1e-3, max_value=0.9))):
+        got = np.fromfile(out + ext, dtype=np.uint8).reshape(cube.shape, order="F")
+        d = np.abs(got.astype(int) - exp.astype(int))
+        assert d.max() <= 1 and np.count_nonzero(d) <= 1, ext
 
 
 @pytest.mark.gpu
@@ -100,6 +125,18 @@ def test_dropin_2d(tmp_path, mode, nx):
         ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol)
     assert rel_l2(load(out, (nx,)), ref) < 1e-10
     assert rel_l2(load(out + ".sol", sol.shape), sol) < 1e-10
+    if mode == "cn":
+        # post::Intensity, Phase (resized 40 x 7) of the solution and the exit quantities
+        # as vectors, against the oracle's restatement on the driver's own field
+        last, dsol = load(out, (nx,)), load(out + ".sol", sol.shape)
+        for ext, src, q, shape, er, ec in ((".int2d", dsol, 0, sol.shape, 0, 0),
+                                           (".phase2d_rs", dsol, 2, (40, 7), 40, 7),
+                                           (".exitint", last, 1, (nx,), 0, 0),
+                                           (".exitphase", last, 2, (nx,), 0, 0),
+                                           (".exitfield", last, 3, (nx,), 0, 0)):
+            exp = O.field_quantity(src, q, er, ec)
+            got = np.fromfile(out + ext, dtype=np.float64).reshape(shape, order="F")
+            assert np.max(np.abs(got - exp)) <= 2e-15 * np.max(np.abs(exp)), ext
     if mode == "cn_curved":
         # post::Transmittivity after every step, from the device-stored columns
         tr = np.fromfile(out + ".trans", dtype=np.float64)
