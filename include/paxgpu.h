@@ -236,6 +236,15 @@ int pxg_far_field_dims(pxg_plan plan, int b, unsigned pad, double phi_min, doubl
                        unsigned export_rows, unsigned export_cols, int dims[2]);
 int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, double phi_min,
                   double phi_max, unsigned export_rows, unsigned export_cols, double *out);
+/* The same with post::FarField::setPadding (src/farFieldPost.cpp:249-321): PXG_PAD_ZERO (the
+ * default, = pxg_far_field) or PXG_PAD_CONSTANT (entries n .. M-1 of every column, and of
+ * every conjugated row of the column pass's window, repeat that line's entry 0).
+ * PXG_PAD_LINEAR_FIT (a GSL least-squares fit the reference marks untested) returns
+ * PXG_ERR_UNSUPPORTED. */
+enum { PXG_PAD_ZERO = 0, PXG_PAD_LINEAR_FIT = 1, PXG_PAD_CONSTANT = 2 };
+int pxg_far_field_padded(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, int padding,
+                         double phi_min, double phi_max, unsigned export_rows, unsigned export_cols,
+                         double *out);
 
 /* ---- per-step reductions (2D plans storing the full solution) ----------------- */
 /* Replaces post::Transmittivity::trapezoidalIntegrateIntensityZ (src/transmittivity.cpp:30-42),

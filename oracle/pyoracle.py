@@ -403,9 +403,26 @@ def _ref_fftshift(a, axis):
     return np.take(a, idx, axis=axis)
 
 
-def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_cols=0):
-    """post::FarField::result for a 3D field (src/farFieldPost.cpp:141-230) with the
-    default zero padding: subtract the reference field, FFT every column zero-padded
+def _constant_pad(lines, M, axis):
+    """post::FarField::constantPadding (src/farFieldPost.cpp:314-321): entries n .. M-1
+    of every line (along `axis`) repeat the line's entry 0."""
+    n = lines.shape[axis]
+    shape = list(lines.shape)
+    shape[axis] = M
+    out = np.empty(shape, dtype=np.complex128)
+    if axis == 0:
+        out[:n] = lines
+        out[n:] = lines[0:1, :]
+    else:
+        out[:, :n] = lines
+        out[:, n:] = lines[:, 0:1]
+    return out
+
+
+def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_cols=0,
+              padding="zero"):
+    """post::FarField::result for a 3D field (src/farFieldPost.cpp:141-230) with zero
+    padding (the default) or constant padding: subtract the reference field, FFT every column zero-padded
     at its end to the signal length M, fftshift, keep the rows of the y window; then
     the same along the rows of that, keeping the x window; |.|^2 / M; then the
     export resize when export sizes are set (GenericScattering sets 512 x 512).
@@ -418,10 +435,17 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     M, (ylo, yhi), (xlo, xhi) = far_field_window(n, pad, k, dx, dy, phi_min, phi_max)
     if yhi < ylo or xhi < xlo:
         raise ValueError("angle window is empty")
-    ft = _ref_fftshift(np.fft.fft(s, n=M, axis=0), 0)[ylo:yhi + 1, :]
+    if padding not in ("zero", "constant"):
+        raise ValueError(padding)
+    cols = _constant_pad(s, M, 0) if padding == "constant" else s
+    ft = _ref_fftshift(np.fft.fft(cols, n=M, axis=0), 0)[ylo:yhi + 1, :]
     # the row pass copies temporary.row(i).t(): Armadillo's t() conjugates complex
-    # values (src/farFieldPost.cpp:213), which mirrors the x axis of the pattern
-    ft = _ref_fftshift(np.fft.fft(np.conj(ft), n=M, axis=1), 1)[:, xlo:xhi + 1]
+    # values (src/farFieldPost.cpp:227), which mirrors the x axis of the pattern;
+    # padSignal then pads that conjugated row (:228)
+    rows = np.conj(ft)
+    if padding == "constant":
+        rows = _constant_pad(rows, M, 1)
+    ft = _ref_fftshift(np.fft.fft(rows, n=M, axis=1), 1)[:, xlo:xhi + 1]
     res = np.asfortranarray(np.abs(ft) ** 2 / M)
     if export_rows and export_cols:
         res = resize_matrix(res, export_rows, export_cols)

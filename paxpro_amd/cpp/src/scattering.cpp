@@ -30,8 +30,9 @@ void post::FarField::verifyConsistentAngles(double dx, double dy, double wavenum
 void post::FarField::result(const Solver &solver, arma::mat &res)
 {
   if (sim == nullptr) throw std::runtime_error("FarField: no ParaxialSimulation linked");
-  if (padding != Pad_t::ZERO)
-    throw std::runtime_error("FarField: only zero padding is implemented on the device");
+  if (padding == Pad_t::LINEAR_FIT)
+    throw std::runtime_error("FarField: linear-fit padding is not implemented on the device");
+  const int pad_mode = padding == Pad_t::CONSTANT ? PXG_PAD_CONSTANT : PXG_PAD_ZERO;
   const arma::cx_mat &solution = solver.getLastSolution3D();
   const pxg_plan plan = solver.devicePlan();
   if (plan == nullptr) throw std::runtime_error("FarField: the solver holds no field on the device");
@@ -73,7 +74,7 @@ void post::FarField::result(const Solver &solver, arma::mat &res)
     }
   }
   res.set_size(dims[0], dims[1]);
-  const int rc = pxg_far_field(plan, 0, holder, 0, Nx, phiMin, phiMax, er, ec, res.memptr());
+  const int rc = pxg_far_field_padded(plan, 0, holder, 0, Nx, pad_mode, phiMin, phiMax, er, ec, res.memptr());
   if (holder) pxg_plan_destroy(holder);
   check(rc);
 }

@@ -1714,7 +1714,18 @@ int pxg_far_field_dims(pxg_plan P, int b, unsigned pad, double phi_min, double p
 int pxg_far_field(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, double phi_min,
                   double phi_max, unsigned export_rows, unsigned export_cols, double *out)
 {
+  return pxg_far_field_padded(P, b, R, rb, pad, PXG_PAD_ZERO, phi_min, phi_max, export_rows,
+                              export_cols, out);
+}
+
+int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, int padding,
+                         double phi_min, double phi_max, unsigned export_rows, unsigned export_cols,
+                         double *out)
+{
   return guarded([&] {
+    if (padding == PXG_PAD_LINEAR_FIT)
+      fail(PXG_ERR_UNSUPPORTED, "linear-fit padding is not implemented (constant or zero only)");
+    if (padding != PXG_PAD_ZERO && padding != PXG_PAD_CONSTANT) fail(PXG_ERR_INVALID, "unknown padding %d", padding);
     const FfWindow W = ff_window(P, b, pad, phi_min, phi_max, export_rows, export_cols);
     if (!out) fail(PXG_ERR_INVALID, "null output");
     if (!P->field_set) fail(PXG_ERR_STATE, "field not set (pxg_set_field)");
@@ -1761,6 +1772,7 @@ int pxg_far_field(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, double ph
     a.tmp = tmp.as<cd>();
     a.out = res.as<double>();
     a.inv_len = 1.0 / W.M;
+    a.pad_const = padding == PXG_PAD_CONSTANT;
     hip_check(launch_far_field(a, np, s), "far field");
     const double *src = res.as<double>();
     if (W.resize) {

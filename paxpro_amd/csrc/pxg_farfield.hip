@@ -6,7 +6,7 @@
 //   column pass: every column zero-padded at its end to the signal length M,
 //                forward FFT (unnormalised), fftshift, rows of the y window kept
 //   row pass:    every row of that, conjugated (Armadillo's t() of a complex
-//                row, src/farFieldPost.cpp:213), zero-padded to M, forward FFT,
+//                row, src/farFieldPost.cpp:227), zero-padded to M, forward FFT,
 //                fftshift, columns of the x window kept, |.|^2 / M
 //   resize:      block means down to the export size
 //                (PostProcessingModule::resizeMatrix, src/postProcessing.cpp:29-49)
@@ -20,6 +20,11 @@
 // Pad lengths far beyond the field (the reference's examples use 1024 and 32768
 // for 512- and 1024-point fields) then need no transform longer than the field's.
 // Everything after the field load is FP64.
+//
+// Constant padding (post::FarField::constantPadding, src/farFieldPost.cpp:314-321)
+// fills entries n .. M-1 of every line with its entry 0, x_0.  Its spectrum is
+// the zero-padded one plus x_0 G(f), G(f) = sum_{j=n}^{M-1} w_M^{f j}, a geometric
+// sum: M - n at f = 0, else (w_M^{f n} - 1) / (1 - w_M^f).
 #include "pxg_fft.hpp"
 #include "pxg_internal.hpp"
 
@@ -28,6 +33,23 @@ namespace pxg {
 __device__ __forceinline__ cd ff_load(const void *p, int c64, size_t i)
 {
   return c64 ? cvt<double>(static_cast<const cf *>(p)[i]) : static_cast<const cd *>(p)[i];
+}
+
+// x_0 G(f) of constant padding, f < M
+__device__ __forceinline__ cd ff_pad_term(const FfArgs &a, cd x0, long f)
+{
+  cd g;
+  if (f == 0) {
+    g = cd{static_cast<double>(a.M - a.n), 0.0};
+  } else {
+    const cd wfn = a.tw_m[(static_cast<size_t>(f) * a.n) & (a.M - 1)];
+    const cd wf = a.tw_m[f];
+    const cd num{wfn.x - 1.0, wfn.y};
+    const cd den{1.0 - wf.x, -wf.y};
+    const double inv = 1.0 / (den.x * den.x + den.y * den.y);
+    g = cd{(num.x * den.x + num.y * den.y) * inv, (num.y * den.x - num.x * den.y) * inv};
+  }
+  return x0 * g;
 }
 
 template <int NP>
@@ -52,13 +74,19 @@ __global__ __launch_bounds__(NP / 8) void k_ff_cols(FfArgs a)
       v[m] = cd{0.0, 0.0};
     }
   }
+  cd x0{0.0, 0.0};
+  if (a.pad_const) {
+    const size_t i0 = static_cast<size_t>(c) * a.n;
+    x0 = ff_load(a.field, a.field_c64, i0);
+    if (a.ref) x0 = x0 - ff_load(a.ref, a.ref_c64, i0);
+  }
   fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const long f = static_cast<long>(a.D) * (t + m * T) + h;
     const int r = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.ylo;  // fftshift, window
     if (static_cast<unsigned>(r) < static_cast<unsigned>(a.ny))
-      a.tmp[static_cast<size_t>(c) * a.ny + r] = v[m];
+      a.tmp[static_cast<size_t>(c) * a.ny + r] = a.pad_const ? v[m] + ff_pad_term(a, x0, f) : v[m];
   }
 }
 
@@ -82,12 +110,14 @@ __global__ __launch_bounds__(NP / 8) void k_ff_rows(FfArgs a)
       v[m] = cd{0.0, 0.0};
     }
   }
+  const cd x0 = a.pad_const ? conj(a.tmp[r]) : cd{0.0, 0.0};  // entry 0 of the row
   fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const long f = static_cast<long>(a.D) * (t + m * T) + h;
     const int col = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.xlo;
     if (static_cast<unsigned>(col) < static_cast<unsigned>(a.nx)) {
+      if (a.pad_const) v[m] = v[m] + ff_pad_term(a, x0, f);
       const double mag = hypot(v[m].x, v[m].y);
       a.out[static_cast<size_t>(col) * a.ny + r] = mag * mag * a.inv_len;
     }

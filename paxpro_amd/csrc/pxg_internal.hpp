@@ -72,6 +72,7 @@ struct FfArgs {
   cd *tmp;                 // column-pass output, ny x n, column-major
   double *out;             // ny x nx intensities, column-major
   double inv_len;          // 1 / M
+  int pad_const;           // constant padding: entries n .. M-1 of a line repeat its entry 0
 };
 hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
 hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,

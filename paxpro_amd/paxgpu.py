@@ -40,12 +40,13 @@ EXPORTS = [
     "pxg_server_create", "pxg_server_submit", "pxg_server_status", "pxg_server_wait",
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
-    "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8",
+    "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
 SRC_FIELD, SRC_SOLUTION = 0, 1
 Q_ABS, Q_ABS2, Q_ARG, Q_REAL = 0, 1, 2, 3
+PAD_ZERO, PAD_LINEAR_FIT, PAD_CONSTANT = 0, 1, 2
 
 
 class PxgError(RuntimeError):
@@ -140,6 +141,7 @@ def lib():
         L.pxg_job_solution_dims.argtypes = [P, I, P]
         L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
         L.pxg_far_field.argtypes = [P, I, P, I, U, D, D, U, U, P]
+        L.pxg_far_field_padded.argtypes = [P, I, P, I, U, I, D, D, U, U, P]
         L.pxg_intensity_trapezoid.argtypes = [P, I, I, P, P, P, P]
         L.pxg_field_quantity_dims.argtypes = [P, I, I, U, U, P]
         L.pxg_field_quantity.argtypes = [P, I, I, I, U, U, P]
@@ -300,15 +302,17 @@ class Plan:
         return out
 
     def far_field(self, b=0, ref=None, ref_b=0, pad=1024, phi_min=-90.0, phi_max=90.0,
-                  export_rows=0, export_cols=0):
+                  export_rows=0, export_cols=0, padding=PAD_ZERO):
         """Scattered far-field intensity of field b minus field ref_b of plan `ref`
-        (post::FarField::result, src/farFieldPost.cpp:141-230): rows follow y, columns x."""
+        (post::FarField::result, src/farFieldPost.cpp:141-230): rows follow y, columns x;
+        padding PAD_ZERO or PAD_CONSTANT (post::FarField::setPadding)."""
         dims = np.zeros(2, dtype=np.int32)
         _check(lib().pxg_far_field_dims(self.h, b, pad, phi_min, phi_max, export_rows,
                                         export_cols, _p(dims)))
         out = np.empty((int(dims[0]), int(dims[1])), dtype=np.float64, order="F")
-        _check(lib().pxg_far_field(self.h, b, None if ref is None else ref.h, ref_b, pad, phi_min,
-                                   phi_max, export_rows, export_cols, _p(out)))
+        _check(lib().pxg_far_field_padded(self.h, b, None if ref is None else ref.h, ref_b, pad,
+                                          padding, phi_min, phi_max, export_rows, export_cols,
+                                          _p(out)))
         return out
 
     def material_none(self):

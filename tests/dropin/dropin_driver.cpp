@@ -206,7 +206,19 @@ static int run3d(const std::string &mode, int n, int nsteps, const char *out, in
   write(out, last.memptr(), last.n_elem);
   const arma::cx_cube &cube = solver->getSolution3D();
   write((std::string(out) + ".cube").c_str(), cube.memptr(), cube.n_elem);
-  if (mode == "fft3d") post3d(*solver, out);
+  if (mode == "fft3d") {
+    post3d(*solver, out);
+    // post::FarField with constant padding (no reference), pad length 256, +-1 degree
+    post::FarField ff;
+    ff.linkParaxialSim(sim);
+    ff.setPadding(post::FarField::Pad_t::CONSTANT);
+    ff.setPadLength(256);
+    ff.setAngleRange(-1.0, 1.0);
+    arma::mat far;
+    ff.result(*solver, far);
+    write_raw(std::string(out) + ".ffconst", far.memptr(), far.n_elem * 8);
+    std::printf("ffconst %zux%zu\n", (size_t)far.n_rows, (size_t)far.n_cols);
+  }
   std::printf("%s n=%d steps=%d cube=%zux%zux%zu\n", mode.c_str(), n, nsteps, (size_t)cube.n_rows,
               (size_t)cube.n_cols, (size_t)cube.n_slices);
   return 0;

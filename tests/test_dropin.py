@@ -63,6 +63,10 @@ def test_dropin_fft3d(tmp_path, mode):
     assert rel_l2(load(out + ".cube", cube.shape), cube) < 1e-11
     if mode == "fft3d":
         _check_post3d(out, load(out, (n, n)), load(out + ".cube", cube.shape))
+        # post::FarField::setPadding(CONSTANT), pad length 256, +-1 degree
+        exp = O.far_field(load(out, (n, n)), None, 256, g.k, g.dx, g.dy, -1.0, 1.0, padding="constant")
+        got = np.fromfile(out + ".ffconst", dtype=np.float64).reshape(exp.shape, order="F")
+        assert rel_l2(got, exp) < 1e-11
 
 
 def _dbl(path, shape):

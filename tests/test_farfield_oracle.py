@@ -129,3 +129,38 @@ def test_sphere_form_factor_first_minimum():
         qr = q[1:40] * R
         fac = (3 * (np.sin(qr) - qr * np.cos(qr)) / qr ** 3) ** 2
         assert np.allclose(prof[1:40] / prof[0], fac, rtol=2e-2, atol=2e-4)
+
+
+def test_constant_padding_of_a_constant_field_is_a_delta():
+    # post::FarField::constantPadding (src/farFieldPost.cpp:314-321) in both passes: a
+    # constant field c padded with its own entry 0 is constant over all M entries of
+    # every column and then of every row, so the spectrum is c M^2 at the centre and
+    # zero elsewhere: |c M^2|^2 / M.  Zero padding of the same field gives a sinc.
+    n, M, dx, c = 8, 32, 0.5, 0.7 - 0.2j
+    sol = np.full((n, n), c)
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0, padding="constant")
+    exp = np.zeros((M, M))
+    exp[M // 2, M // 2] = abs(c) ** 2 * M ** 3
+    assert np.allclose(ff, exp, rtol=1e-13, atol=1e-9 * exp.max())
+    fz = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0)
+    assert np.count_nonzero(fz > 1e-6 * fz.max()) > 1
+
+
+def test_constant_padding_repeats_entry_zero():
+    # only row 0 non-zero (ones): each column pads with its entry 0 = 1, so its
+    # transform is 1 + G(f) with G(f) = sum_{j=n}^{M-1} exp(-2 pi i f j / M); each row of
+    # the column pass is then constant (conjugated and padded with itself), so only the
+    # centre column is lit, with M |1 + G(f)|^2.  Padding with the last entry (0) would
+    # give M everywhere in that column.
+    n, M, dx = 8, 32, 0.5
+    sol = np.zeros((n, n), complex)
+    sol[0, :] = 1.0
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0, padding="constant")
+    f = np.arange(M)
+    G = np.array([sum(np.exp(-2j * np.pi * ff_ * j / M) for j in range(n, M)) for ff_ in f])
+    col = M * np.abs(1.0 + G) ** 2
+    col = np.roll(col, M // 2)  # fftshift of an even length
+    assert ff[M // 2 + 0, M // 2] == pytest.approx(M * (M - n + 1) ** 2, rel=1e-13)
+    assert np.allclose(ff[:, M // 2], col, rtol=1e-12, atol=1e-9 * col.max())
+    off = np.delete(ff, M // 2, axis=1)
+    assert np.max(off) < 1e-12 * col.max()

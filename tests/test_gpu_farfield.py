@@ -43,6 +43,40 @@ def _random_field(n, seed):
 
 
 @pytest.mark.parametrize("kind,n,prec,pad,phi,export", [
+    (P.FFT3D, 64, P.PREC_FP64, 64, 90.0, (0, 0)),         # M = n: nothing is padded
+    (P.FFT3D, 64, P.PREC_FP64, 256, 90.0, (0, 0)),        # whole spectrum, M = 4 NP
+    (P.FFT3D, 64, P.PREC_FP64, 1024, 1.0, (0, 0)),
+    (P.FFT3D, 128, P.PREC_MIXED, 2048, 0.5, (16, 12)),
+    (P.ADI, 48, P.PREC_FP64, 256, 2.0, (0, 0)),           # n below NP = 64
+])
+def test_far_field_constant_padding_vs_oracle(ctx, kind, n, prec, pad, phi, export):
+    # setPadding(CONSTANT): the device adds x_0 G(f) in closed form; the oracle pads
+    # every line explicitly and transforms M points
+    announce("far field constant padding", kind, n, prec, pad, phi, export)
+    g = _grid(n)
+    prob = P.problem(kind, n, 3, g.dx, g.dz, K, xmin=g.xmin, ymin=g.ymin, dy=g.dy, zmin=g.zmin,
+                     precision=prec)
+    plan, refp = P.Plan(ctx, prob), P.Plan(ctx, prob)
+    plan.set_field(_random_field(n, 3) + 2.0)  # a mean that the padding carries
+    refp.set_field(_random_field(n, 4))
+    sol, ref = plan.get_field(), refp.get_field()
+    out = plan.far_field(0, refp, 0, pad, -phi, phi, *export, padding=P.PAD_CONSTANT)
+    exp = O.far_field(sol, ref, pad, K, g.dx, g.dy, -phi, phi, *export, padding="constant")
+    assert out.shape == exp.shape
+    err = rel_l2(out, exp)
+    announce("shape", out.shape, "rel_l2", err)
+    assert err < 1e-11
+    zero = plan.far_field(0, refp, 0, pad, -phi, phi, *export)
+    if pad > n:
+        assert rel_l2(zero, exp) > 1e-3  # the padding changes the pattern
+    with pytest.raises(P.PxgError) as e:
+        plan.far_field(0, refp, 0, pad, -phi, phi, *export, padding=P.PAD_LINEAR_FIT)
+    assert e.value.code == P.ERR_UNSUPPORTED
+    plan.close()
+    refp.close()
+
+
+@pytest.mark.parametrize("kind,n,prec,pad,phi,export", [
     (P.FFT3D, 64, P.PREC_FP64, 64, 90.0, (0, 0)),         # whole spectrum, M = n
     (P.FFT3D, 64, P.PREC_FP64, 1024, 1.0, (0, 0)),        # M = 16 NP: residue split
     (P.FFT3D, 128, P.PREC_MIXED, 2048, 0.5, (16, 12)),    # complex64 fields, export resize
