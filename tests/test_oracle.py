@@ -166,7 +166,9 @@ def test_gaussian_closed_form_at_waist():
 def test_node_count_truncation():
     assert O.nodes(-500, 500, 1000 / 1023) == 1024
     assert O.nodes(-10000, -10000 + 511 * 39.0625, 39.0625) == 512
-    assert O.nodes(0, 20000, 20000 / 511) in (511, 512)  # truncating formula, see SURVEY
+    # (max - min)/step + 1 truncated to unsigned (src/paraxialSimulation.cpp:107-110): a
+    # quotient that rounds just below 511 gives 511 nodes
+    assert O.nodes(0, 20000, 20000 / 511) in (511, 512)
 
 
 # ------------------------------------------------------------ cube storage
