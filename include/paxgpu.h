@@ -236,11 +236,14 @@ int pxg_far_field_dims(pxg_plan plan, int b, unsigned pad, double phi_min, doubl
                        unsigned export_rows, unsigned export_cols, int dims[2]);
 int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, double phi_min,
                   double phi_max, unsigned export_rows, unsigned export_cols, double *out);
-/* The same with post::FarField::setPadding (src/farFieldPost.cpp:249-321): PXG_PAD_ZERO (the
- * default, = pxg_far_field) or PXG_PAD_CONSTANT (entries n .. M-1 of every column, and of
- * every conjugated row of the column pass's window, repeat that line's entry 0).
- * PXG_PAD_LINEAR_FIT (a GSL least-squares fit the reference marks untested) returns
- * PXG_ERR_UNSUPPORTED. */
+/* The same with post::FarField::setPadding (src/farFieldPost.cpp:249-321), applied to every
+ * column and to every conjugated row of the column pass's window: PXG_PAD_ZERO (the default,
+ * = pxg_far_field); PXG_PAD_CONSTANT (entries n .. M-1 repeat the line's entry 0);
+ * PXG_PAD_LINEAR_FIT (linearPadding, :265-312, which the reference marks untested: least-
+ * squares lines through entries 0..9 (gsl_fit_linear) extend the line over its last
+ * (M-n)/2 entries and over (M-n)/2 entries after n, the second with the fit of the same
+ * entries against the grid's last ten x positions, as the reference computes it; needs
+ * nx >= 10). */
 enum { PXG_PAD_ZERO = 0, PXG_PAD_LINEAR_FIT = 1, PXG_PAD_CONSTANT = 2 };
 int pxg_far_field_padded(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, int padding,
                          double phi_min, double phi_max, unsigned export_rows, unsigned export_cols,

@@ -419,10 +419,66 @@ def _constant_pad(lines, M, axis):
     return out
 
 
+def _gsl_fit_linear(x, y):
+    """gsl_fit_linear (GSL fit/linear.c, the library the reference links; its published
+    algorithm): running means of x and y, then running means of dx^2 and dx dy;
+    c1 = m_dxdy / m_dx2, c0 = m_y - m_x c1."""
+    mx = my = 0.0
+    for i in range(len(x)):
+        mx += (x[i] - mx) / (i + 1.0)
+        my += (y[i] - my) / (i + 1.0)
+    mdx2 = mdxdy = 0.0
+    for i in range(len(x)):
+        ddx, ddy = x[i] - mx, y[i] - my
+        mdx2 += (ddx * ddx - mdx2) / (i + 1.0)
+        mdxdy += (ddx * ddy - mdxdy) / (i + 1.0)
+    c1 = mdxdy / mdx2
+    return my - mx * c1, c1
+
+
+def _arma_linspace(start, end, num):
+    """arma::linspace: start + i (end - start)/(num - 1), the last point set to end."""
+    delta = (end - start) / (num - 1.0)
+    x = [start + i * delta for i in range(num - 1)]
+    return np.array(x + [end])
+
+
+def _linear_pad(lines, M, axis, xmin, dx):
+    """post::FarField::linearPadding (src/farFieldPost.cpp:265-312), line by line:
+    real and imaginary parts of entries 0..9 fitted against linspace(0, x_10 - x_0, 10)
+    fill the last (M - n)/2 entries with a0 + a1 (x[0] + dxf (i - M + 1)); the same
+    entries 0..9 (the reference resets its data pointer to the start) fitted against
+    linspace(x_{n-10}, x_n, 10) fill entries n .. n + (M - n)/2 - 1 with
+    a0 + a1 (x[9] + (i - n) dxf); dxf is the first grid's step, x_i = xmin + i dx."""
+    L2 = np.moveaxis(np.asarray(lines, dtype=np.complex128), axis, 0)
+    n = L2.shape[0]
+    out = np.zeros((M,) + L2.shape[1:], dtype=np.complex128)
+    out[:n] = L2
+    added = M - n
+    getx = lambda i: xmin + i * dx
+    x1 = _arma_linspace(0.0, getx(10) - getx(0), 10)
+    x2 = _arma_linspace(getx(n - 10), getx(n), 10)
+    dxf = x1[1] - x1[0]
+    for j in np.ndindex(*L2.shape[1:]):
+        y = L2[(slice(0, 10),) + j]
+        c0, c1 = _gsl_fit_linear(x1, y.real)
+        d0, d1 = _gsl_fit_linear(x1, y.imag)
+        a0, a1 = complex(c0, d0), complex(c1, d1)
+        for i in range(M - added // 2, M):
+            out[(i,) + j] = a0 + a1 * (x1[0] + dxf * (i - M + 1))
+        c0, c1 = _gsl_fit_linear(x2, y.real)
+        d0, d1 = _gsl_fit_linear(x2, y.imag)
+        a0, a1 = complex(c0, d0), complex(c1, d1)
+        for i in range(n, n + added // 2):
+            out[(i,) + j] = a0 + a1 * (x2[-1] + (i - n) * dxf)
+    return np.moveaxis(out, 0, axis)
+
+
 def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_cols=0,
-              padding="zero"):
+              padding="zero", xmin=0.0):
     """post::FarField::result for a 3D field (src/farFieldPost.cpp:141-230) with zero
-    padding (the default) or constant padding: subtract the reference field, FFT every column zero-padded
+    padding (the default), constant or linear-fit padding (whose fit grids use the
+    transverse grid xmin, dx, as the reference's do): subtract the reference field, FFT every column zero-padded
     at its end to the signal length M, fftshift, keep the rows of the y window; then
     the same along the rows of that, keeping the x window; |.|^2 / M; then the
     export resize when export sizes are set (GenericScattering sets 512 x 512).
@@ -435,9 +491,14 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     M, (ylo, yhi), (xlo, xhi) = far_field_window(n, pad, k, dx, dy, phi_min, phi_max)
     if yhi < ylo or xhi < xlo:
         raise ValueError("angle window is empty")
-    if padding not in ("zero", "constant"):
+    if padding not in ("zero", "constant", "linear"):
         raise ValueError(padding)
-    cols = _constant_pad(s, M, 0) if padding == "constant" else s
+    if padding == "constant":
+        cols = _constant_pad(s, M, 0)
+    elif padding == "linear":
+        cols = _linear_pad(s, M, 0, xmin, dx)
+    else:
+        cols = s
     ft = _ref_fftshift(np.fft.fft(cols, n=M, axis=0), 0)[ylo:yhi + 1, :]
     # the row pass copies temporary.row(i).t(): Armadillo's t() conjugates complex
     # values (src/farFieldPost.cpp:227), which mirrors the x axis of the pattern;
@@ -445,6 +506,8 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     rows = np.conj(ft)
     if padding == "constant":
         rows = _constant_pad(rows, M, 1)
+    elif padding == "linear":
+        rows = _linear_pad(rows, M, 1, xmin, dx)
     ft = _ref_fftshift(np.fft.fft(rows, n=M, axis=1), 1)[:, xlo:xhi + 1]
     res = np.asfortranarray(np.abs(ft) ** 2 / M)
     if export_rows and export_cols:

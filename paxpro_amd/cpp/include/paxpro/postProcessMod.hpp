@@ -75,7 +75,7 @@ class FarField : public ProjectionQuantity {
   FarField() : ProjectionQuantity("farField") {}
   using PostProcessingModule::result;
 
-  /** Padding of the signal; the device path implements ZERO (the default). */
+  /** Padding of the signal: ZERO (the default), CONSTANT or LINEAR_FIT, all on the device. */
   void setPadding(Pad_t padType) { padding = padType; }
   /** Far field in the 3D case (src/farFieldPost.cpp:141-230): rows follow y, columns x. */
   void result(const Solver &solver, arma::mat &res) override;

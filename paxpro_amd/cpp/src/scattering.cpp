@@ -31,8 +31,10 @@ void post::FarField::result(const Solver &solver, arma::mat &res)
 {
   if (sim == nullptr) throw std::runtime_error("FarField: no ParaxialSimulation linked");
   if (padding == Pad_t::LINEAR_FIT)
-    throw std::runtime_error("FarField: linear-fit padding is not implemented on the device");
-  const int pad_mode = padding == Pad_t::CONSTANT ? PXG_PAD_CONSTANT : PXG_PAD_ZERO;
+    std::clog << "Warning! Linear padding has not been tested properly and may not work!\n";
+  const int pad_mode = padding == Pad_t::CONSTANT     ? PXG_PAD_CONSTANT
+                       : padding == Pad_t::LINEAR_FIT ? PXG_PAD_LINEAR_FIT
+                                                      : PXG_PAD_ZERO;
   const arma::cx_mat &solution = solver.getLastSolution3D();
   const pxg_plan plan = solver.devicePlan();
   if (plan == nullptr) throw std::runtime_error("FarField: the solver holds no field on the device");

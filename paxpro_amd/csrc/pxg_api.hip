@@ -1723,9 +1723,10 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
                          double *out)
 {
   return guarded([&] {
-    if (padding == PXG_PAD_LINEAR_FIT)
-      fail(PXG_ERR_UNSUPPORTED, "linear-fit padding is not implemented (constant or zero only)");
-    if (padding != PXG_PAD_ZERO && padding != PXG_PAD_CONSTANT) fail(PXG_ERR_INVALID, "unknown padding %d", padding);
+    if (padding != PXG_PAD_ZERO && padding != PXG_PAD_CONSTANT && padding != PXG_PAD_LINEAR_FIT)
+      fail(PXG_ERR_INVALID, "unknown padding %d", padding);
+    if (padding == PXG_PAD_LINEAR_FIT && P->p.nx < 10)
+      fail(PXG_ERR_UNSUPPORTED, "linear-fit padding fits 10 entries: needs at least 10 points per side");
     const FfWindow W = ff_window(P, b, pad, phi_min, phi_max, export_rows, export_cols);
     if (!out) fail(PXG_ERR_INVALID, "null output");
     if (!P->field_set) fail(PXG_ERR_STATE, "field not set (pxg_set_field)");
@@ -1773,6 +1774,35 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
     a.out = res.as<double>();
     a.inv_len = 1.0 / W.M;
     a.pad_const = padding == PXG_PAD_CONSTANT;
+    a.lin_L = static_cast<int>((W.M - static_cast<unsigned>(n)) / 2);
+    a.pad_linear = padding == PXG_PAD_LINEAR_FIT && a.lin_L > 0;
+    if (a.pad_linear) {
+      // the fit grids of post::FarField::linearPadding (src/farFieldPost.cpp:271-273,
+      // 296-298): getX(i) = xmin + i dx, Armadillo linspace (last point = end)
+      auto getx = [&](int i) { return P->p.xmin + i * P->p.dx; };
+      auto grid = [](double start, double end, double *x) {
+        const double delta = (end - start) / 9.0;
+        for (int k = 0; k < 9; ++k) x[k] = start + k * delta;
+        x[9] = end;
+      };
+      // gsl_fit_linear's x-only running means: mean(x), then deviations and the mean square
+      auto stats = [](const double *x, double *dxk, double &mx, double &mdx2) {
+        mx = 0.0;
+        for (int k = 0; k < 10; ++k) mx += (x[k] - mx) / (k + 1.0);
+        mdx2 = 0.0;
+        for (int k = 0; k < 10; ++k) {
+          dxk[k] = x[k] - mx;
+          mdx2 += (dxk[k] * dxk[k] - mdx2) / (k + 1.0);
+        }
+      };
+      double x1[10], x2[10];
+      grid(0.0, getx(10) - getx(0), x1);
+      grid(getx(n - 10), getx(n), x2);
+      stats(x1, a.fit1_dx, a.fit1_mx, a.fit1_mdx2);
+      stats(x2, a.fit2_dx, a.fit2_mx, a.fit2_mdx2);
+      a.lin_dxf = x1[1] - x1[0];
+      a.lin_x2end = x2[9];
+    }
     hip_check(launch_far_field(a, np, s), "far field");
     const double *src = res.as<double>();
     if (W.resize) {

@@ -25,7 +25,19 @@
 // fills entries n .. M-1 of every line with its entry 0, x_0.  Its spectrum is
 // the zero-padded one plus x_0 G(f), G(f) = sum_{j=n}^{M-1} w_M^{f j}, a geometric
 // sum: M - n at f = 0, else (w_M^{f n} - 1) / (1 - w_M^f).
+//
+// Linear-fit padding (post::FarField::linearPadding, src/farFieldPost.cpp:265-312)
+// fits y = c0 + c1 x to the real and to the imaginary parts of entries 0..9 of a
+// line (gsl_fit_linear: running means) twice: against x = linspace(0, x_10 - x_0)
+// for the L = (M - n)/2 entries M-L .. M-1, valued a1 + b1 dxf (i - M + 1), and
+// against x = linspace(x_{n-10}, x_n) for entries n .. n+L-1, valued
+// a2 + b2 (x_n + (i - n) dxf); dxf is the first grid's step, x_i the transverse grid.
+// (The second fit reads entries 0..9 again, as the reference does.)  Each segment
+// is P + Q k over k = 0..L-1, so its spectrum is a phase times P S0 + Q S1 with
+// S0 = sum_k w^{fk} = e^{i c t} D(t) and S1 = sum_k k w^{fk} = e^{i c t} (c D(t) - i D'(t)),
+// t = -2 pi f / M, c = (L-1)/2 and D the Dirichlet kernel sin(L t/2) / sin(t/2).
 #include "pxg_fft.hpp"
+#include "pxg_fp64math.hpp"
 #include "pxg_internal.hpp"
 
 namespace pxg {
@@ -50,6 +62,63 @@ __device__ __forceinline__ cd ff_pad_term(const FfArgs &a, cd x0, long f)
     g = cd{(num.x * den.x + num.y * den.y) * inv, (num.y * den.x - num.x * den.y) * inv};
   }
   return x0 * g;
+}
+
+struct LinFit {
+  cd a, b;  // intercept and slope (real and imaginary parts fitted separately)
+};
+
+// gsl_fit_linear's running means over the 10 points y of one grid
+__device__ __forceinline__ LinFit lin_fit(const double *dxk, double mx, double mdx2, const cd *y)
+{
+  double myr = 0.0, myi = 0.0;
+  for (int k = 0; k < 10; ++k) {
+    myr += (y[k].x - myr) / (k + 1.0);
+    myi += (y[k].y - myi) / (k + 1.0);
+  }
+  double mxyr = 0.0, mxyi = 0.0;
+  for (int k = 0; k < 10; ++k) {
+    mxyr += (dxk[k] * (y[k].x - myr) - mxyr) / (k + 1.0);
+    mxyi += (dxk[k] * (y[k].y - myi) - mxyi) / (k + 1.0);
+  }
+  const double br = mxyr / mdx2, bi = mxyi / mdx2;
+  return {cd{myr - mx * br, myi - mx * bi}, cd{br, bi}};
+}
+
+// sin and cos of pi m / M for an integer m, reduced modulo 2M first
+__device__ __forceinline__ void sincos_pi_frac(long long m, int M, double &s, double &c)
+{
+  const long long r = m % (2LL * M);
+  sincos_fp64(3.141592653589793 * static_cast<double>(r) / M, s, c);
+}
+
+// spectrum at f of the two linear-fit segments of a line
+__device__ __forceinline__ cd ff_linear_term(const FfArgs &a, const LinFit &B, const LinFit &E, long f)
+{
+  const int L = a.lin_L;
+  cd S0, S1;
+  if (f == 0) {
+    S0 = cd{static_cast<double>(L), 0.0};
+    S1 = cd{0.5 * L * (L - 1.0), 0.0};
+  } else {
+    double sh, ch, sl, cl, se, ce;
+    sincos_pi_frac(f, a.M, sh, ch);                                  // sin(t/2) = -sh
+    sincos_pi_frac(static_cast<long long>(f) * L, a.M, sl, cl);      // sin(L t/2) = -sl
+    sincos_pi_frac(static_cast<long long>(f) * (L - 1), a.M, se, ce);  // e^{i c t} = ce - i se
+    const double D = sl / sh;
+    const double Dp = (0.5 * L * cl * (-sh) + 0.5 * sl * ch) / (sh * sh);
+    const cd e{ce, -se};
+    const double chalf = 0.5 * (L - 1.0);
+    S0 = e * cd{D, 0.0};
+    S1 = e * cd{chalf * D, -Dp};
+  }
+  const cd PB = B.a + B.b * cd{a.lin_dxf * (1.0 - L), 0.0};
+  const cd QB = B.b * cd{a.lin_dxf, 0.0};
+  const cd PE = E.a + E.b * cd{a.lin_x2end, 0.0};
+  const cd QE = E.b * cd{a.lin_dxf, 0.0};
+  const cd phB = a.tw_m[(static_cast<size_t>(f) * static_cast<size_t>(a.M - L)) & (a.M - 1)];
+  const cd phE = a.tw_m[(static_cast<size_t>(f) * a.n) & (a.M - 1)];
+  return phB * (PB * S0 + QB * S1) + phE * (PE * S0 + QE * S1);
 }
 
 template <int NP>
@@ -80,13 +149,25 @@ __global__ __launch_bounds__(NP / 8) void k_ff_cols(FfArgs a)
     x0 = ff_load(a.field, a.field_c64, i0);
     if (a.ref) x0 = x0 - ff_load(a.ref, a.ref_c64, i0);
   }
+  LinFit fb{}, fe{};
+  if (a.pad_linear) {
+    cd y[10];
+    for (int k = 0; k < 10; ++k) {
+      const size_t i = static_cast<size_t>(c) * a.n + k;
+      y[k] = ff_load(a.field, a.field_c64, i);
+      if (a.ref) y[k] = y[k] - ff_load(a.ref, a.ref_c64, i);
+    }
+    fb = lin_fit(a.fit1_dx, a.fit1_mx, a.fit1_mdx2, y);
+    fe = lin_fit(a.fit2_dx, a.fit2_mx, a.fit2_mdx2, y);
+  }
   fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const long f = static_cast<long>(a.D) * (t + m * T) + h;
     const int r = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.ylo;  // fftshift, window
     if (static_cast<unsigned>(r) < static_cast<unsigned>(a.ny))
-      a.tmp[static_cast<size_t>(c) * a.ny + r] = a.pad_const ? v[m] + ff_pad_term(a, x0, f) : v[m];
+      a.tmp[static_cast<size_t>(c) * a.ny + r] =
+          a.pad_const ? v[m] + ff_pad_term(a, x0, f) : (a.pad_linear ? v[m] + ff_linear_term(a, fb, fe, f) : v[m]);
   }
 }
 
@@ -111,6 +192,13 @@ __global__ __launch_bounds__(NP / 8) void k_ff_rows(FfArgs a)
     }
   }
   const cd x0 = a.pad_const ? conj(a.tmp[r]) : cd{0.0, 0.0};  // entry 0 of the row
+  LinFit fb{}, fe{};
+  if (a.pad_linear) {
+    cd y[10];
+    for (int k = 0; k < 10; ++k) y[k] = conj(a.tmp[static_cast<size_t>(k) * a.ny + r]);
+    fb = lin_fit(a.fit1_dx, a.fit1_mx, a.fit1_mdx2, y);
+    fe = lin_fit(a.fit2_dx, a.fit2_mx, a.fit2_mdx2, y);
+  }
   fft_line<NP, -1, double, WL, false>(v, lds, t, twb);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -118,6 +206,7 @@ __global__ __launch_bounds__(NP / 8) void k_ff_rows(FfArgs a)
     const int col = static_cast<int>((f + a.M / 2) & (a.M - 1)) - a.xlo;
     if (static_cast<unsigned>(col) < static_cast<unsigned>(a.nx)) {
       if (a.pad_const) v[m] = v[m] + ff_pad_term(a, x0, f);
+      if (a.pad_linear) v[m] = v[m] + ff_linear_term(a, fb, fe, f);
       const double mag = hypot(v[m].x, v[m].y);
       a.out[static_cast<size_t>(col) * a.ny + r] = mag * mag * a.inv_len;
     }

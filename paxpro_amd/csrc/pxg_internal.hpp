@@ -73,6 +73,15 @@ struct FfArgs {
   double *out;             // ny x nx intensities, column-major
   double inv_len;          // 1 / M
   int pad_const;           // constant padding: entries n .. M-1 of a line repeat its entry 0
+  // linear-fit padding: two least-squares lines through a line's entries 0..9, one per
+  // padded segment; per fit grid the deviations x_k - mean(x), mean(x) and mean of the
+  // squared deviations (the fit's x-only terms), the grid step and the end point
+  int pad_linear;
+  int lin_L;               // entries per padded segment, (M - n) / 2
+  double fit1_dx[10], fit1_mx, fit1_mdx2;
+  double fit2_dx[10], fit2_mx, fit2_mdx2;
+  double lin_dxf;          // step of the first fit grid
+  double lin_x2end;        // last point of the second fit grid
 };
 hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
 hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,

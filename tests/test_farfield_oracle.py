@@ -164,3 +164,43 @@ def test_constant_padding_repeats_entry_zero():
     assert np.allclose(ff[:, M // 2], col, rtol=1e-12, atol=1e-9 * col.max())
     off = np.delete(ff, M // 2, axis=1)
     assert np.max(off) < 1e-12 * col.max()
+
+
+@pytest.mark.parametrize("n", [16, 15])
+def test_linear_padding_of_ramp_columns(n):
+    # post::FarField::linearPadding (src/farFieldPost.cpp:265-312) on columns p + q j:
+    # the fits reproduce the ramp, so the last (M - n)/2 entries continue it backwards
+    # to p at index M - 1 (p + q (i - M + 1)), and the entries after n carry
+    # p + q (9 + i - n), because the second fit reads entries 0..9 again against the
+    # grid's last ten positions; for odd M - n the middle entry stays 0.  Identical
+    # columns make every row of the column pass constant (its fit is flat), so only
+    # the centre column is lit: M |FFT(column)|^2.
+    M, dx, xmin = 64, 0.5, -3.0
+    p, q = 0.3 + 0.1j, -0.05 + 0.02j
+    j = np.arange(n)
+    sol = np.tile((p + q * j)[:, None], (1, n))
+    L = (M - n) // 2
+    col = np.zeros(M, complex)
+    col[:n] = p + q * j
+    i = np.arange(n, n + L)
+    col[n:n + L] = p + q * (9 + i - n)
+    i = np.arange(M - L, M)
+    col[M - L:] = p + q * (i - M + 1)
+    exp_col = M * np.abs(np.roll(np.fft.fft(col), M // 2)) ** 2
+    if n % 2:  # each constant row also keeps one zero: its sum is (M - 1) times the value
+        exp_col *= ((M - 1.0) / M) ** 2
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0, padding="linear", xmin=xmin)
+    assert np.allclose(ff[:, M // 2], exp_col, rtol=1e-9, atol=1e-9 * exp_col.max())
+    if n % 2 == 0:  # with the zero, other columns are lit as well
+        assert np.max(np.delete(ff, M // 2, axis=1)) < 1e-9 * exp_col.max()
+
+
+def test_gsl_fit_linear_exact_on_a_line():
+    x = O._arma_linspace(-2.0, 7.0, 10)
+    c0, c1 = O._gsl_fit_linear(x, 1.5 - 0.25 * x)
+    assert c0 == pytest.approx(1.5, abs=1e-14) and c1 == pytest.approx(-0.25, abs=1e-15)
+    # least squares of a non-linear set: the normal equations' solution
+    y = x ** 2
+    A = np.vstack([np.ones_like(x), x]).T
+    ref = np.linalg.lstsq(A, y, rcond=None)[0]
+    assert np.allclose(O._gsl_fit_linear(x, y), ref, rtol=1e-13)
