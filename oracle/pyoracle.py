@@ -26,6 +26,7 @@ SPHERE, BOX, CYLINDER = 0, 1, 2
 UNION, DIFFERENCE = 0, 1
 FIX_ADI_Y_DIAG = 1
 FIX_REFR_SAMPLES = 2
+FIX_LINEAR_PAD = 4
 BC_DIRICHLET, BC_TRANSPARENT = 0, 1
 
 
@@ -282,8 +283,12 @@ def num_threads():
 
 
 def rel_l2(a, ref):
-    a = np.asarray(a).ravel(order="K")
-    ref = np.asarray(ref).ravel(order="K")
+    """Relative L2 distance.  Arrays of one shape are compared element by
+    element whatever their memory order; otherwise both are flattened in their
+    memory order (a flat column-major buffer against a Fortran matrix)."""
+    a, ref = np.asarray(a), np.asarray(ref)
+    if a.shape != ref.shape:
+        a, ref = a.ravel(order="K"), ref.ravel(order="K")
     return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
 
 
@@ -443,13 +448,17 @@ def _arma_linspace(start, end, num):
     return np.array(x + [end])
 
 
-def _linear_pad(lines, M, axis, xmin, dx):
+def _linear_pad(lines, M, axis, xmin, dx, signed_ramp=False):
     """post::FarField::linearPadding (src/farFieldPost.cpp:265-312), line by line:
     real and imaginary parts of entries 0..9 fitted against linspace(0, x_10 - x_0, 10)
     fill the last (M - n)/2 entries with a0 + a1 (x[0] + dxf (i - M + 1)); the same
     entries 0..9 (the reference resets its data pointer to the start) fitted against
     linspace(x_{n-10}, x_n, 10) fill entries n .. n + (M - n)/2 - 1 with
-    a0 + a1 (x[9] + (i - n) dxf); dxf is the first grid's step, x_i = xmin + i dx."""
+    a0 + a1 (x[9] + (i - n) dxf); dxf is the first grid's step, x_i = xmin + i dx.
+    The reference's i - M + 1 mixes an int with Armadillo's unsigned 64-bit n_elem
+    (:288), so it is taken modulo 2^64 before the conversion to double (correctly
+    rounded, as the conversion is); signed_ramp=True takes the signed difference
+    instead (PXG_FIX_LINEAR_PAD)."""
     L2 = np.moveaxis(np.asarray(lines, dtype=np.complex128), axis, 0)
     n = L2.shape[0]
     out = np.zeros((M,) + L2.shape[1:], dtype=np.complex128)
@@ -465,7 +474,8 @@ def _linear_pad(lines, M, axis, xmin, dx):
         d0, d1 = _gsl_fit_linear(x1, y.imag)
         a0, a1 = complex(c0, d0), complex(c1, d1)
         for i in range(M - added // 2, M):
-            out[(i,) + j] = a0 + a1 * (x1[0] + dxf * (i - M + 1))
+            ramp = float(i - M + 1) if signed_ramp else float((i - M + 1) % (1 << 64))
+            out[(i,) + j] = a0 + a1 * (x1[0] + dxf * ramp)
         c0, c1 = _gsl_fit_linear(x2, y.real)
         d0, d1 = _gsl_fit_linear(x2, y.imag)
         a0, a1 = complex(c0, d0), complex(c1, d1)
@@ -475,7 +485,7 @@ def _linear_pad(lines, M, axis, xmin, dx):
 
 
 def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_cols=0,
-              padding="zero", xmin=0.0):
+              padding="zero", xmin=0.0, quirks=0):
     """post::FarField::result for a 3D field (src/farFieldPost.cpp:141-230) with zero
     padding (the default), constant or linear-fit padding (whose fit grids use the
     transverse grid xmin, dx, as the reference's do): subtract the reference field, FFT every column zero-padded
@@ -496,7 +506,7 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     if padding == "constant":
         cols = _constant_pad(s, M, 0)
     elif padding == "linear":
-        cols = _linear_pad(s, M, 0, xmin, dx)
+        cols = _linear_pad(s, M, 0, xmin, dx, signed_ramp=bool(quirks & FIX_LINEAR_PAD))
     else:
         cols = s
     ft = _ref_fftshift(np.fft.fft(cols, n=M, axis=0), 0)[ylo:yhi + 1, :]
@@ -507,7 +517,7 @@ def far_field(sol, ref, pad, k, dx, dy, phi_min, phi_max, export_rows=0, export_
     if padding == "constant":
         rows = _constant_pad(rows, M, 1)
     elif padding == "linear":
-        rows = _linear_pad(rows, M, 1, xmin, dx)
+        rows = _linear_pad(rows, M, 1, xmin, dx, signed_ramp=bool(quirks & FIX_LINEAR_PAD))
     ft = _ref_fftshift(np.fft.fft(rows, n=M, axis=1), 1)[:, xlo:xhi + 1]
     res = np.asfortranarray(np.abs(ft) ** 2 / M)
     if export_rows and export_cols:
