@@ -62,7 +62,8 @@ enum { PXG_BC_DIRICHLET = 0, PXG_BC_TRANSPARENT = 1 };   /* Solver2D::BC_t */
 /* Quirk switches: 0 reproduces the reference; a set bit selects the physics form. */
 enum {
   PXG_FIX_ADI_Y_DIAG = 1u << 0,    /* src/alternatingDirectionSolver.cpp:103 parenthesis */
-  PXG_FIX_REFR_SAMPLES = 1u << 1   /* src/fftSolver3D.cpp:262 skipped trapezoid sample */
+  PXG_FIX_REFR_SAMPLES = 1u << 1,  /* src/fftSolver3D.cpp:262 skipped trapezoid sample */
+  PXG_FIX_LINEAR_PAD = 1u << 2     /* src/farFieldPost.cpp:288 unsigned index difference */
 };
 
 typedef struct {
@@ -242,8 +243,10 @@ int pxg_far_field(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, d
  * PXG_PAD_LINEAR_FIT (linearPadding, :265-312, which the reference marks untested: least-
  * squares lines through entries 0..9 (gsl_fit_linear) extend the line over its last
  * (M-n)/2 entries and over (M-n)/2 entries after n, the second with the fit of the same
- * entries against the grid's last ten x positions, as the reference computes it; needs
- * nx >= 10). */
+ * entries against the grid's last ten x positions; needs nx >= 10).  The first segment's
+ * abscissa is dxf*(i - n_elem + 1) with Armadillo's unsigned 64-bit n_elem (:288): by
+ * default it wraps as in the reference (entries M-L .. M-2 sit near dxf*2^64, entry M-1
+ * at 0); the plan's quirk bit PXG_FIX_LINEAR_PAD takes the signed ramp dxf*(i - M + 1). */
 enum { PXG_PAD_ZERO = 0, PXG_PAD_LINEAR_FIT = 1, PXG_PAD_CONSTANT = 2 };
 int pxg_far_field_padded(pxg_plan plan, int b, pxg_plan ref, int ref_b, unsigned pad, int padding,
                          double phi_min, double phi_max, unsigned export_rows, unsigned export_cols,
@@ -327,7 +330,11 @@ typedef struct {
   double *result;                /* exit field (complex128); written at completion */
   unsigned first_step, nsteps;
   int inject_failure_after_chunks;  /* < 0: none; >= 0 fails the job from the host (tests) */
+  int inject_plan_failure;          /* PXG_INJECT_*: a plan failure forced from the host (tests) */
 } pxg_job;
+/* Host-forced plan failures of one job: its plan creation, or the recycling
+ * (pxg_plan_reset) of a pooled plan for it, fails with PXG_ERR_INJECTED. */
+enum { PXG_INJECT_NONE = 0, PXG_INJECT_CREATE = 1, PXG_INJECT_RESET = 2 };
 typedef struct {
   uint64_t submitted, completed, failed, cancelled, rejected, running, queued;
   double p50_ms, p99_ms, mean_ms, max_ms;   /* submission -> result on host, completed jobs */
@@ -336,7 +343,13 @@ typedef struct {
   size_t hbm_reserved, hbm_budget;
 } pxg_server_stats;
 /* slots: jobs running at once; hbm_budget: bytes of plans (0 = 64 GiB);
- * chunk_steps: z-steps enqueued per job before the next job is served. */
+ * chunk_steps: z-steps enqueued per job before the next job is served.
+ * A job whose plan cannot be made or recycled fails with that error (its
+ * reserved bytes are released); a job that does not fit the budget now waits
+ * for running jobs to free memory.  The worker thread takes the server lock
+ * only to move jobs between the queue and the running set: copies, plan
+ * creation and synchronisation run without it, so submit/status/wait/cancel
+ * never wait for device work. */
 int pxg_server_create(pxg_ctx ctx, int slots, size_t hbm_budget, unsigned chunk_steps, pxg_server *out);
 int pxg_server_submit(pxg_server s, const pxg_job *job, uint64_t *id);
 int pxg_server_status(pxg_server s, uint64_t id, int *state);

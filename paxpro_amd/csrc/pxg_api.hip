@@ -1802,6 +1802,7 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
       stats(x2, a.fit2_dx, a.fit2_mx, a.fit2_mdx2);
       a.lin_dxf = x1[1] - x1[0];
       a.lin_x2end = x2[9];
+      a.lin_wrap = (P->p.quirks & PXG_FIX_LINEAR_PAD) == 0;
     }
     hip_check(launch_far_field(a, np, s), "far field");
     const double *src = res.as<double>();

@@ -36,6 +36,15 @@
 // is P + Q k over k = 0..L-1, so its spectrum is a phase times P S0 + Q S1 with
 // S0 = sum_k w^{fk} = e^{i c t} D(t) and S1 = sum_k k w^{fk} = e^{i c t} (c D(t) - i D'(t)),
 // t = -2 pi f / M, c = (L-1)/2 and D the Dirichlet kernel sin(L t/2) / sin(t/2).
+//
+// The reference forms the first segment's abscissa as dxf * (i - n_elem + 1) with
+// an int i and Armadillo's unsigned 64-bit n_elem (src/farFieldPost.cpp:288), so the
+// difference wraps: for j = M - 1 - i >= 1 it is the double nearest to 2^64 - j,
+// i.e. 2^64 - 2048 q(j) with q(j) = j / 2048 rounded half to even (doubles in
+// [2^63, 2^64) are 2048 apart), and 0 at i = M - 1.  That segment is therefore
+// C = a + b dxf 2^64 on k = 0..L-2 and a at k = L-1, less b dxf 2048 q(j) on the
+// runs of constant q, each a geometric sum (none while L <= 1024).  This is the
+// default; PXG_FIX_LINEAR_PAD selects the signed ramp P + Q k above.
 #include "pxg_fft.hpp"
 #include "pxg_fp64math.hpp"
 #include "pxg_internal.hpp"
@@ -92,6 +101,43 @@ __device__ __forceinline__ void sincos_pi_frac(long long m, int M, double &s, do
   sincos_fp64(3.141592653589793 * static_cast<double>(r) / M, s, c);
 }
 
+// sum_{k=0}^{m-1} w_M^{f k}, f in [0, M): m at f = 0, else (1 - w^{fm}) / (1 - w^f)
+__device__ __forceinline__ cd ff_geometric(long f, long long m, int M)
+{
+  if (f == 0) return cd{static_cast<double>(m), 0.0};
+  double s1, c1, sm, cm;
+  sincos_pi_frac(2LL * f, M, s1, c1);                        // w^f = c1 - i s1
+  sincos_pi_frac(2LL * f * m, M, sm, cm);                    // w^{fm}
+  const cd num{1.0 - cm, sm};
+  const cd den{1.0 - c1, s1};
+  const double inv = 1.0 / (den.x * den.x + den.y * den.y);
+  return cd{(num.x * den.x + num.y * den.y) * inv, (num.y * den.x - num.x * den.y) * inv};
+}
+
+// w_M^{x} for an integer x
+__device__ __forceinline__ cd ff_w(long long x, int M)
+{
+  double s, c;
+  sincos_pi_frac(2LL * x, M, s, c);
+  return cd{c, -s};
+}
+
+// sum_{k=0}^{L-2} q(L-1-k) w^{fk} of the wrapped first segment: over the runs
+// q >= 1 of j = L-1-k (q even: j in [2048q - 1024, 2048q + 1024], odd: the open
+// range), each clipped to j <= L-1
+__device__ __forceinline__ cd ff_wrap_staircase(long f, int L, int M)
+{
+  cd t{0.0, 0.0};
+  for (long long q = 1;; ++q) {
+    const long long jlo = 2048 * q - 1024 + (q & 1), jhi = 2048 * q + 1024 - (q & 1);
+    if (jlo > L - 1) break;
+    const long long k0 = (L - 1) - std::min<long long>(jhi, L - 1), k1 = (L - 1) - jlo;
+    t = t + cd{static_cast<double>(q), 0.0} * ff_w(static_cast<long long>(f) * k0, M) *
+                ff_geometric(f, k1 - k0 + 1, M);
+  }
+  return t;
+}
+
 // spectrum at f of the two linear-fit segments of a line
 __device__ __forceinline__ cd ff_linear_term(const FfArgs &a, const LinFit &B, const LinFit &E, long f)
 {
@@ -112,13 +158,22 @@ __device__ __forceinline__ cd ff_linear_term(const FfArgs &a, const LinFit &B, c
     S0 = e * cd{D, 0.0};
     S1 = e * cd{chalf * D, -Dp};
   }
-  const cd PB = B.a + B.b * cd{a.lin_dxf * (1.0 - L), 0.0};
-  const cd QB = B.b * cd{a.lin_dxf, 0.0};
   const cd PE = E.a + E.b * cd{a.lin_x2end, 0.0};
   const cd QE = E.b * cd{a.lin_dxf, 0.0};
   const cd phB = a.tw_m[(static_cast<size_t>(f) * static_cast<size_t>(a.M - L)) & (a.M - 1)];
   const cd phE = a.tw_m[(static_cast<size_t>(f) * a.n) & (a.M - 1)];
-  return phB * (PB * S0 + QB * S1) + phE * (PE * S0 + QE * S1);
+  cd segB;
+  if (a.lin_wrap) {
+    const cd C = B.a + B.b * cd{a.lin_dxf * 18446744073709551616.0, 0.0};   // 2^64
+    const cd wl = ff_w(static_cast<long long>(f) * (L - 1), a.M);
+    segB = C * (S0 - wl) + B.a * wl;
+    if (L > 1025) segB = segB - B.b * cd{a.lin_dxf * 2048.0, 0.0} * ff_wrap_staircase(f, L, a.M);
+  } else {
+    const cd PB = B.a + B.b * cd{a.lin_dxf * (1.0 - L), 0.0};
+    const cd QB = B.b * cd{a.lin_dxf, 0.0};
+    segB = PB * S0 + QB * S1;
+  }
+  return phB * segB + phE * (PE * S0 + QE * S1);
 }
 
 template <int NP>

@@ -82,6 +82,7 @@ struct FfArgs {
   double fit2_dx[10], fit2_mx, fit2_mdx2;
   double lin_dxf;          // step of the first fit grid
   double lin_x2end;        // last point of the second fit grid
+  int lin_wrap;            // first segment's ramp with the reference's unsigned index difference
 };
 hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
 hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -62,13 +63,17 @@ struct pxg_server_s {
   int slots;
   size_t budget;
   unsigned chunk;
+  // mu guards queue, all, the job fields that other threads read (state, err,
+  // msg, t_done, cancel) and the statistics.  running, pool and the plans are
+  // the worker's alone; the worker takes mu only to move jobs and record results.
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Job>> queue;
-  std::vector<std::shared_ptr<Job>> running;
   std::map<uint64_t, std::shared_ptr<Job>> all;
+  std::vector<std::shared_ptr<Job>> running;
   std::vector<Pooled> pool;
-  size_t reserved = 0;  // running + pooled plan bytes
+  std::atomic<size_t> reserved{0};  // running + pooled plan bytes
+  std::atomic<uint64_t> n_running{0};
   uint64_t next_id = 1;
   bool stop = false;
   std::thread worker;
@@ -80,7 +85,8 @@ struct pxg_server_s {
   clk::time_point busy_since;
   std::vector<hipEvent_t> free_events;
 
-  void finish(const std::shared_ptr<Job> &j, int state, int err, const std::string &msg)
+  // Records the end of a job (called with mu held).
+  void finish_locked(const std::shared_ptr<Job> &j, int state, int err, const std::string &msg)
   {
     j->state = state;
     j->err = err;
@@ -100,52 +106,104 @@ struct pxg_server_s {
     } else {
       ++n_fail;
     }
+    cv_done.notify_all();
+  }
+
+  // Worker only, without mu: returns the job's plan to the pool (or frees it)
+  // and its event, then records the end of the job.
+  void finish(const std::shared_ptr<Job> &j, int state, int err, const std::string &msg)
+  {
     if (j->plan) release_plan(j);
     if (j->ev) {
       free_events.push_back(j->ev);
       j->ev = nullptr;
     }
-    cv_done.notify_all();
+    std::lock_guard<std::mutex> lk(mu);
+    finish_locked(j, state, err, msg);
+  }
+
+  void drop_plan(pxg_plan plan, size_t bytes)
+  {
+    pxg_plan_destroy(plan);
+    reserved -= bytes;
   }
 
   void release_plan(const std::shared_ptr<Job> &j)
   {
-    (void)pxg_sync(j->plan);
-    if (static_cast<int>(pool.size()) < slots) {
+    if (pxg_sync(j->plan) == PXG_OK && static_cast<int>(pool.size()) < slots)
       pool.push_back({j->plan, j->spec.prob, j->bytes});
-    } else {
-      pxg_plan_destroy(j->plan);
-      reserved -= j->bytes;
-    }
+    else
+      drop_plan(j->plan, j->bytes);
     j->plan = nullptr;
   }
 
+  size_t pooled_bytes() const
+  {
+    size_t b = 0;
+    for (const Pooled &p : pool) b += p.bytes;
+    return b;
+  }
+
+  bool pooled_shape(const pxg_problem &p) const
+  {
+    for (const Pooled &q : pool)
+      if (same_shape(q.prob, p)) return true;
+    return false;
+  }
+
+  enum Acquire { ACQ_OK, ACQ_NO_ROOM, ACQ_ERROR };
+
   // A plan for the job: recycle a pooled plan of the same shape, else create
-  // one (evicting idle pooled plans when the budget needs room).
-  bool acquire_plan(const std::shared_ptr<Job> &j)
+  // one (evicting idle pooled plans when the budget needs room).  ACQ_NO_ROOM:
+  // the job waits for memory; ACQ_ERROR: *rc and *msg say why it cannot run
+  // (a failed recycle has destroyed the pooled plan and released its bytes).
+  Acquire acquire_plan(const std::shared_ptr<Job> &j, int *rc, std::string *msg)
   {
     const pxg_problem &p = j->spec.prob;
     for (size_t i = 0; i < pool.size(); ++i) {
-      if (same_shape(pool[i].prob, p)) {
-        j->plan = pool[i].plan;
-        j->bytes = pool[i].bytes;
-        pool.erase(pool.begin() + i);
-        if (pxg_plan_reset(j->plan, &p) != PXG_OK) return false;
-        return true;
+      if (!same_shape(pool[i].prob, p)) continue;
+      const Pooled got = pool[i];
+      pool.erase(pool.begin() + static_cast<long>(i));
+      const int r = j->spec.inject_plan_failure == PXG_INJECT_RESET ? PXG_ERR_INJECTED
+                                                                    : pxg_plan_reset(got.plan, &p);
+      if (r != PXG_OK) {
+        *msg = r == PXG_ERR_INJECTED ? "plan recycling failure injected by the host" : pxg_last_error();
+        *rc = r;
+        drop_plan(got.plan, got.bytes);
+        return ACQ_ERROR;
       }
+      j->plan = got.plan;
+      j->bytes = got.bytes;
+      return ACQ_OK;
     }
     size_t need = 0;
-    if (pxg_plan_bytes(&p, &need) != PXG_OK) return false;
+    if ((*rc = pxg_plan_bytes(&p, &need)) != PXG_OK) {
+      *msg = pxg_last_error();
+      return ACQ_ERROR;
+    }
     while (reserved + need > budget && !pool.empty()) {
-      pxg_plan_destroy(pool.front().plan);
-      reserved -= pool.front().bytes;
+      drop_plan(pool.front().plan, pool.front().bytes);
       pool.erase(pool.begin());
     }
-    if (reserved + need > budget) return false;
-    if (pxg_plan_create(ctx, &p, &j->plan) != PXG_OK) return false;
+    if (reserved + need > budget) return ACQ_NO_ROOM;
+    if (j->spec.inject_plan_failure == PXG_INJECT_CREATE) {
+      *rc = PXG_ERR_INJECTED;
+      *msg = "plan creation failure injected by the host";
+      return ACQ_ERROR;
+    }
+    const int r = pxg_plan_create(ctx, &p, &j->plan);
+    if (r != PXG_OK) {
+      j->plan = nullptr;
+      // the context's own budget is shared with plans outside this server:
+      // wait while jobs of ours still hold memory, else the job cannot run
+      if (r == PXG_ERR_BUDGET && !running.empty()) return ACQ_NO_ROOM;
+      *rc = r;
+      *msg = pxg_last_error();
+      return ACQ_ERROR;
+    }
     j->bytes = need;
     reserved += need;
-    return true;
+    return ACQ_OK;
   }
 
   int setup(const std::shared_ptr<Job> &j)
@@ -167,110 +225,173 @@ struct pxg_server_s {
     return rc;
   }
 
+  // Next queued job to admit (FIFO with backfill): the first one that is
+  // cancelled, cannot ever fit (both finished here), or fits now, i.e. a
+  // pooled plan of its shape exists or its bytes fit once idle pooled plans are
+  // evicted.  Called with mu held; the job is taken out of the queue.
+  std::shared_ptr<Job> next_admission_locked()
+  {
+    const size_t evictable = pooled_bytes();
+    for (auto it = queue.begin(); it != queue.end();) {
+      std::shared_ptr<Job> j = *it;
+      if (j->cancel) {
+        it = queue.erase(it);
+        finish_locked(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled while queued");
+        continue;
+      }
+      size_t need = 0;
+      const int brc = pxg_plan_bytes(&j->spec.prob, &need);
+      if (brc != PXG_OK || need > budget) {
+        it = queue.erase(it);
+        finish_locked(j, PXG_JOB_FAILED, brc != PXG_OK ? brc : PXG_ERR_BUDGET,
+                      brc != PXG_OK ? pxg_last_error() : "job larger than the HBM budget");
+        continue;
+      }
+      if (pooled_shape(j->spec.prob) || reserved - evictable + need <= budget) {
+        queue.erase(it);
+        return j;
+      }
+      ++it;
+    }
+    return nullptr;
+  }
+
+  // Admits queued jobs into free slots; plan work runs without mu.
+  bool admit()
+  {
+    bool progressed = false;
+    while (static_cast<int>(running.size()) < slots) {
+      std::shared_ptr<Job> j;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        j = next_admission_locked();
+      }
+      if (!j) break;
+      int rc = PXG_OK;
+      std::string msg;
+      const Acquire a = acquire_plan(j, &rc, &msg);
+      if (a == ACQ_NO_ROOM) {
+        std::lock_guard<std::mutex> lk(mu);
+        queue.push_front(j);  // keeps its turn; waits for running jobs to free memory
+        break;
+      }
+      if (a == ACQ_OK && (rc = setup(j)) != PXG_OK) msg = pxg_last_error();
+      if (rc != PXG_OK) {
+        finish(j, PXG_JOB_FAILED, rc, msg);
+        progressed = true;
+        continue;
+      }
+      if (free_events.empty()) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+          finish(j, PXG_JOB_FAILED, PXG_ERR_DEVICE, "hipEventCreate failed");
+          continue;
+        }
+        free_events.push_back(e);
+      }
+      j->ev = free_events.back();
+      free_events.pop_back();
+      const bool first = running.empty();
+      running.push_back(j);
+      ++n_running;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first) busy_since = clk::now();
+        j->state = PXG_JOB_RUNNING;
+      }
+      progressed = true;
+    }
+    return progressed;
+  }
+
+  void retire(size_t i, int state, int err, const std::string &msg)
+  {
+    std::shared_ptr<Job> j = running[i];
+    running.erase(running.begin() + static_cast<long>(i));
+    --n_running;
+    finish(j, state, err, msg);
+  }
+
+  // Advances running jobs by one chunk each; finished jobs copy their result
+  // out and leave.  Runs without mu except to read cancellation flags.
+  bool advance()
+  {
+    bool progressed = false;
+    for (size_t i = 0; i < running.size();) {
+      std::shared_ptr<Job> j = running[i];
+      if (j->in_flight) {
+        if (hipEventQuery(j->ev) != hipSuccess) {
+          ++i;
+          continue;
+        }
+        j->done += j->in_flight;
+        j->in_flight = 0;
+        ++j->chunks;
+        progressed = true;
+      }
+      bool cancel;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        cancel = j->cancel;
+      }
+      if (cancel) {
+        retire(i, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled");
+        continue;
+      }
+      if (j->spec.inject_failure_after_chunks >= 0 && j->chunks >= j->spec.inject_failure_after_chunks) {
+        retire(i, PXG_JOB_FAILED, PXG_ERR_INJECTED, "failure injected by the host");
+        continue;
+      }
+      if (j->done >= j->spec.nsteps) {
+        const int rc = pxg_get_field(j->plan, 0, j->spec.result);
+        retire(i, rc == PXG_OK ? PXG_JOB_DONE : PXG_JOB_FAILED, rc, rc == PXG_OK ? "" : pxg_last_error());
+        continue;
+      }
+      const unsigned c = std::min(chunk, j->spec.nsteps - j->done);
+      int rc = pxg_step(j->plan, j->spec.first_step + j->done, c);
+      if (rc == PXG_OK) rc = hipEventRecord(j->ev, pxg::plan_stream(j->plan)) == hipSuccess ? PXG_OK : PXG_ERR_DEVICE;
+      if (rc != PXG_OK) {
+        retire(i, PXG_JOB_FAILED, rc, pxg_last_error());
+        continue;
+      }
+      j->in_flight = c;
+      progressed = true;
+      ++i;
+    }
+    if (running.empty() && busy_since != clk::time_point()) {
+      std::lock_guard<std::mutex> lk(mu);
+      busy_s += std::chrono::duration<double>(clk::now() - busy_since).count();
+      busy_since = clk::time_point();
+    }
+    return progressed;
+  }
+
   void loop()
   {
     (void)hipSetDevice(pxg::ctx_device(ctx));
-    std::unique_lock<std::mutex> lk(mu);
-    while (!stop) {
-      bool progressed = false;
-      // admission: FIFO with backfill of jobs that fit
-      for (auto it = queue.begin(); it != queue.end() && static_cast<int>(running.size()) < slots;) {
-        std::shared_ptr<Job> j = *it;
-        if (j->cancel) {
-          it = queue.erase(it);
-          finish(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled while queued");
-          continue;
-        }
-        size_t need = 0;
-        const int brc = pxg_plan_bytes(&j->spec.prob, &need);
-        if (brc != PXG_OK || need > budget) {
-          it = queue.erase(it);
-          finish(j, PXG_JOB_FAILED, brc != PXG_OK ? brc : PXG_ERR_BUDGET,
-                 brc != PXG_OK ? pxg_last_error() : "job larger than the HBM budget");
-          continue;
-        }
-        if (!acquire_plan(j)) {
-          ++it;  // does not fit now; later jobs may
-          continue;
-        }
-        const int rc = setup(j);
-        it = queue.erase(it);
-        if (rc != PXG_OK) {
-          finish(j, PXG_JOB_FAILED, rc, pxg_last_error());
-          continue;
-        }
-        if (running.empty()) busy_since = clk::now();
-        j->state = PXG_JOB_RUNNING;
-        if (free_events.empty()) {
-          hipEvent_t e;
-          (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
-          free_events.push_back(e);
-        }
-        j->ev = free_events.back();
-        free_events.pop_back();
-        running.push_back(j);
-        progressed = true;
+    for (;;) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (stop) break;
       }
-      // progress of running jobs
-      for (size_t i = 0; i < running.size();) {
-        std::shared_ptr<Job> j = running[i];
-        if (j->in_flight) {
-          if (hipEventQuery(j->ev) != hipSuccess) {
-            ++i;
-            continue;
-          }
-          j->done += j->in_flight;
-          j->in_flight = 0;
-          ++j->chunks;
-          progressed = true;
-        }
-        if (j->cancel) {
-          running.erase(running.begin() + i);
-          finish(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled");
-          continue;
-        }
-        if (j->spec.inject_failure_after_chunks >= 0 && j->chunks >= j->spec.inject_failure_after_chunks) {
-          running.erase(running.begin() + i);
-          finish(j, PXG_JOB_FAILED, PXG_ERR_INJECTED, "failure injected by the host");
-          continue;
-        }
-        if (j->done >= j->spec.nsteps) {
-          const int rc = pxg_get_field(j->plan, 0, j->spec.result);
-          running.erase(running.begin() + i);
-          finish(j, rc == PXG_OK ? PXG_JOB_DONE : PXG_JOB_FAILED, rc, rc == PXG_OK ? "" : pxg_last_error());
-          continue;
-        }
-        const unsigned c = std::min(chunk, j->spec.nsteps - j->done);
-        int rc = pxg_step(j->plan, j->spec.first_step + j->done, c);
-        if (rc == PXG_OK) rc = hipEventRecord(j->ev, pxg::plan_stream(j->plan)) == hipSuccess ? PXG_OK : PXG_ERR_DEVICE;
-        if (rc != PXG_OK) {
-          running.erase(running.begin() + i);
-          finish(j, PXG_JOB_FAILED, rc, pxg_last_error());
-          continue;
-        }
-        j->in_flight = c;
-        progressed = true;
-        ++i;
-      }
-      if (running.empty() && busy_since != clk::time_point()) {
-        busy_s += std::chrono::duration<double>(clk::now() - busy_since).count();
-        busy_since = clk::time_point();
-      }
+      bool progressed = admit();
+      progressed = advance() || progressed;
       if (!progressed) {
-        if (running.empty() && queue.empty())
+        std::unique_lock<std::mutex> lk(mu);
+        if (running.empty())
           cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-        else {
-          lk.unlock();
-          std::this_thread::sleep_for(std::chrono::microseconds(20));
-          lk.lock();
-        }
+        else
+          cv_work.wait_for(lk, std::chrono::microseconds(20), [&] { return stop; });
       }
     }
-    for (auto &j : running) finish(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
-    running.clear();
-    for (auto &j : queue) finish(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
-    queue.clear();
-    for (auto &p : pool) pxg_plan_destroy(p.plan);
+    while (!running.empty()) retire(0, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
+    std::deque<std::shared_ptr<Job>> left;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      left.swap(queue);
+      for (auto &j : left) finish_locked(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
+    }
+    for (auto &p : pool) drop_plan(p.plan, p.bytes);
     pool.clear();
     for (auto e : free_events) (void)hipEventDestroy(e);
     free_events.clear();
@@ -367,7 +488,7 @@ int pxg_server_stats_get(pxg_server s, pxg_server_stats *st)
   st->failed = s->n_fail;
   st->cancelled = s->n_cancel;
   st->rejected = s->n_rej;
-  st->running = s->running.size();
+  st->running = s->n_running.load();
   st->queued = s->queue.size();
   std::vector<double> v = s->lat_ms;
   if (!v.empty()) {

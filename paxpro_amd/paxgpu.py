@@ -17,7 +17,7 @@ OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_BUDGET, ERR_DEVICE, ERR_STATE, ERR_CANCELL
 FFT2D, FFT3D, CN, ADI = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 BC_DIRICHLET, BC_TRANSPARENT = 0, 1
-FIX_ADI_Y_DIAG, FIX_REFR_SAMPLES = 1, 2
+FIX_ADI_Y_DIAG, FIX_REFR_SAMPLES, FIX_LINEAR_PAD = 1, 2, 4
 LAYOUT_CONTIGUOUS, LAYOUT_INTERLEAVED = 0, 1
 SHAPE_SPHERE, SHAPE_BOX, SHAPE_CYLINDER = 0, 1, 2
 OP_UNION, OP_DIFFERENCE = 0, 1
@@ -43,6 +43,7 @@ EXPORTS = [
     "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+INJECT_NONE, INJECT_CREATE, INJECT_RESET = range(3)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
 SRC_FIELD, SRC_SOLUTION = 0, 1
 Q_ABS, Q_ABS2, Q_ARG, Q_REAL = 0, 1, 2, 3
@@ -78,7 +79,8 @@ class Job(C.Structure):
                 ("n_shapes", C.c_int), ("parts", C.c_void_p), ("n_parts", C.c_int),
                 ("modules", C.c_void_p), ("n_modules", C.c_int), ("wg", CurvedWG),
                 ("field", C.c_void_p), ("result", C.c_void_p), ("first_step", C.c_uint),
-                ("nsteps", C.c_uint), ("inject_failure_after_chunks", C.c_int)]
+                ("nsteps", C.c_uint), ("inject_failure_after_chunks", C.c_int),
+                ("inject_plan_failure", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -436,7 +438,8 @@ class Server:
         _check(lib().pxg_server_create(ctx.h, slots, budget, chunk_steps, C.byref(self.h)))
         self._keep = {}
 
-    def submit(self, prob, field, nsteps, first_step=1, csg=None, curved=None, inject=-1):
+    def submit(self, prob, field, nsteps, first_step=1, csg=None, curved=None, inject=-1,
+               inject_plan=INJECT_NONE):
         is3d = prob.kind in (FFT3D, ADI)
         f = np.asarray(field, dtype=np.complex128)
         f = np.asfortranarray(f) if is3d else np.ascontiguousarray(f)
@@ -462,6 +465,7 @@ class Server:
         job.field, job.result = _p(f), _p(out)
         job.first_step, job.nsteps = first_step, nsteps
         job.inject_failure_after_chunks = inject
+        job.inject_plan_failure = inject_plan
         jid = C.c_uint64()
         _check(lib().pxg_server_submit(self.h, C.byref(job), C.byref(jid)))
         self._keep[jid.value] = (keep, out)

@@ -197,6 +197,18 @@ static int run3d(const std::string &mode, int n, int nsteps, const char *out, in
   sim.material = (mode == "fft3d_user") ? static_cast<const MaterialFunction *>(&user) : &csg;
   sim.setBoundaryConditions(gb);
   if (mode == "fft3d_abs") fft.absorbingBC(600.0, 300.0);
+  if (mode == "fft3d_reset" || mode == "adi_reset") {
+    // a first propagation through another material, then ParaxialSimulation::reset
+    // (src/paraxialSimulation.cpp:50-54: the solver's reset, currentStep back to 1),
+    // new boundary conditions and the material of the checked run, as
+    // GenericScattering::solve does between its two runs (src/genericScattering.cpp:148-213)
+    UserSphere other(2500.0);
+    sim.material = &other;
+    sim.solve();
+    sim.reset();
+    sim.material = &csg;
+    sim.setBoundaryConditions(gb);
+  }
   if (mode == "fft3d_step") {
     for (int i = 1; i < n * 0 + nsteps + 1; ++i) sim.step();
   } else {

@@ -43,7 +43,7 @@ def _grid3d(n, nsteps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["fft3d", "fft3d_user", "fft3d_step", "fft3d_abs"])
+@pytest.mark.parametrize("mode", ["fft3d", "fft3d_user", "fft3d_step", "fft3d_abs", "fft3d_reset"])
 def test_dropin_fft3d(tmp_path, mode):
     n, nsteps = 64, 8
     out = run(tmp_path, mode, n, nsteps)
@@ -93,9 +93,10 @@ def _check_post3d(out, last, cube):
 
 
 @pytest.mark.gpu
-def test_dropin_adi(tmp_path):
+@pytest.mark.parametrize("mode", ["adi", "adi_reset"])
+def test_dropin_adi(tmp_path, mode):
     n, nsteps = 64, 4
-    out = run(tmp_path, "adi", n, nsteps)
+    out = run(tmp_path, mode, n, nsteps)
     g = _grid3d(n, nsteps)
     u0 = gaussian_field(g, n, 3000.0)
     csg = O.CSGMaterial(*spheres_scene([(0.0, 0.0, 0.0, 4000.0, 5e-6, 1e-7)]))

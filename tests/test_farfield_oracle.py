@@ -167,14 +167,19 @@ def test_constant_padding_repeats_entry_zero():
 
 
 @pytest.mark.parametrize("n", [16, 15])
-def test_linear_padding_of_ramp_columns(n):
+@pytest.mark.parametrize("signed", [False, True])
+def test_linear_padding_of_ramp_columns(n, signed):
     # post::FarField::linearPadding (src/farFieldPost.cpp:265-312) on columns p + q j:
-    # the fits reproduce the ramp, so the last (M - n)/2 entries continue it backwards
-    # to p at index M - 1 (p + q (i - M + 1)), and the entries after n carry
-    # p + q (9 + i - n), because the second fit reads entries 0..9 again against the
-    # grid's last ten positions; for odd M - n the middle entry stays 0.  Identical
-    # columns make every row of the column pass constant (its fit is flat), so only
-    # the centre column is lit: M |FFT(column)|^2.
+    # the fits reproduce the ramp, so the last L = (M - n)/2 entries continue it
+    # backwards to p at index M - 1, and the entries after n carry p + q (9 + i - n),
+    # because the second fit reads entries 0..9 again against the grid's last ten
+    # positions; for odd M - n the middle entry stays 0.  The backward ramp's
+    # abscissa is i - M + 1 taken modulo 2^64 in the reference (an int minus an
+    # unsigned 64-bit n_elem): 2^64 - j for j = M - 1 - i < 1024 is a double only as
+    # 2^64, so those entries are p + q 2^64 and entry M - 1 is p; with the signed
+    # difference (PXG_FIX_LINEAR_PAD) they are p + q (i - M + 1).  Identical columns
+    # make every row of the column pass constant (its fit is flat), so only the
+    # centre column is lit: M |FFT(column)|^2.
     M, dx, xmin = 64, 0.5, -3.0
     p, q = 0.3 + 0.1j, -0.05 + 0.02j
     j = np.arange(n)
@@ -185,11 +190,12 @@ def test_linear_padding_of_ramp_columns(n):
     i = np.arange(n, n + L)
     col[n:n + L] = p + q * (9 + i - n)
     i = np.arange(M - L, M)
-    col[M - L:] = p + q * (i - M + 1)
+    col[M - L:] = p + q * ((i - M + 1) if signed else np.where(i < M - 1, 2.0 ** 64, 0.0))
     exp_col = M * np.abs(np.roll(np.fft.fft(col), M // 2)) ** 2
     if n % 2:  # each constant row also keeps one zero: its sum is (M - 1) times the value
         exp_col *= ((M - 1.0) / M) ** 2
-    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0, padding="linear", xmin=xmin)
+    ff = O.far_field(sol, None, M, full_window_k(dx), dx, dx, -90.0, 90.0, padding="linear", xmin=xmin,
+                     quirks=O.FIX_LINEAR_PAD if signed else 0)
     assert np.allclose(ff[:, M // 2], exp_col, rtol=1e-9, atol=1e-9 * exp_col.max())
     if n % 2 == 0:  # with the zero, other columns are lit as well
         assert np.max(np.delete(ff, M // 2, axis=1)) < 1e-9 * exp_col.max()
@@ -204,3 +210,28 @@ def test_gsl_fit_linear_exact_on_a_line():
     A = np.vstack([np.ones_like(x), x]).T
     ref = np.linalg.lstsq(A, y, rcond=None)[0]
     assert np.allclose(O._gsl_fit_linear(x, y), ref, rtol=1e-13)
+
+
+def test_linear_padding_wrap_staircase():
+    """Segments longer than 1024 entries: the wrapped abscissa 2^64 - j is the
+    double 2^64 - 2048 r, r = j/2048 rounded half to even (the doubles of
+    [2^63, 2^64) are 2048 apart), so a ramp line p + q j pads entry M - 1 - j with
+    p + (q/dxf) dxf (2^64 - 2048 r): constant runs, ties at j = 1024, 3072, 5120
+    going to r = 0, 2, 2."""
+    n, M, dx, xmin = 16, 8192, 0.5, -3.0
+    p, q = 0.25 - 0.5j, 0.125 + 0.0625j
+    line = (p + q * np.arange(n))[:, None]
+    out = O._linear_pad(line, M, 0, xmin, dx)[:, 0]
+    L = (M - n) // 2
+    dxf = O._arma_linspace(0.0, 10 * dx, 10)[1]
+    jj = np.arange(1, L)
+    r = np.array([round(j / 2048) for j in jj], dtype=float)  # Python rounds half to even
+    for j, want in ((1024, 0), (1025, 1), (3071, 1), (3072, 2), (4000, 2)):
+        assert r[j - 1] == want
+    exp = p + (q / dxf) * (dxf * (2.0 ** 64 - 2048.0 * r))
+    got = out[M - 1 - jj]
+    assert np.allclose(got, exp, rtol=1e-15, atol=0)
+    assert out[M - 1] == pytest.approx(p, abs=1e-12)
+    # the oracle converts (i - M + 1) mod 2^64 directly; the staircase form the device
+    # sums in closed form is that conversion exactly, for every j of the segment
+    assert all(float((-int(j)) % (1 << 64)) == 2.0 ** 64 - 2048.0 * rj for j, rj in zip(jj, r))

@@ -48,23 +48,29 @@ def _random_field(n, seed):
     (P.FFT3D, 64, P.PREC_FP64, 1024, 1.0, (0, 0)),
     (P.FFT3D, 128, P.PREC_MIXED, 2048, 0.5, (16, 12)),
     (P.ADI, 48, P.PREC_FP64, 256, 2.0, (0, 0)),           # n below NP = 64
+    (P.FFT3D, 64, P.PREC_FP64, 8192, 0.3, (0, 0)),        # linear fit: 4064-entry segments
 ])
-@pytest.mark.parametrize("padding", ["constant", "linear"])
+@pytest.mark.parametrize("padding", ["constant", "linear", "linear_signed"])
 def test_far_field_padding_vs_oracle(ctx, kind, n, prec, pad, phi, export, padding):
     # setPadding(CONSTANT / LINEAR_FIT): the device adds the padded segments' spectra in
     # closed form (geometric and arithmetico-geometric sums); the oracle pads every line
     # explicitly, as the reference does, and transforms M points
     announce("far field padding", padding, kind, n, prec, pad, phi, export)
+    # the linear fit's first segment wraps its abscissa near 2^64 as the reference does
+    # (runs of a 2048-step staircase beyond 1024 entries); "linear_signed" sets the
+    # plan's PXG_FIX_LINEAR_PAD
     mode = P.PAD_CONSTANT if padding == "constant" else P.PAD_LINEAR_FIT
+    quirks = P.FIX_LINEAR_PAD if padding == "linear_signed" else 0
     g = _grid(n)
     prob = P.problem(kind, n, 3, g.dx, g.dz, K, xmin=g.xmin, ymin=g.ymin, dy=g.dy, zmin=g.zmin,
-                     precision=prec)
+                     precision=prec, quirks=quirks)
     plan, refp = P.Plan(ctx, prob), P.Plan(ctx, prob)
     plan.set_field(_random_field(n, 3) + 2.0)  # a mean that the padding carries
     refp.set_field(_random_field(n, 4))
     sol, ref = plan.get_field(), refp.get_field()
     out = plan.far_field(0, refp, 0, pad, -phi, phi, *export, padding=mode)
-    exp = O.far_field(sol, ref, pad, K, g.dx, g.dy, -phi, phi, *export, padding=padding, xmin=g.xmin)
+    exp = O.far_field(sol, ref, pad, K, g.dx, g.dy, -phi, phi, *export, padding=padding.replace("_signed", ""),
+                      xmin=g.xmin, quirks=quirks)
     assert out.shape == exp.shape
     err = rel_l2(out, exp)
     announce("shape", out.shape, "rel_l2", err)

@@ -122,3 +122,39 @@ def test_server_failures_are_isolated(ctx):
     st = srv.stats()
     assert st["rejected"] == 1 and st["cancelled"] == 1 and st["completed"] == 2
     srv.close()
+
+
+def test_server_plan_failures_release_memory(ctx):
+    """A job whose plan cannot be recycled (pxg_plan_reset failure forced from
+    the host) or created (creation failure forced from the host) fails with
+    that code; the pooled plan of the failed recycle is destroyed and its bytes
+    released; later jobs of both shapes finish bit-identically to the same jobs
+    served alone; the context holds no bytes once the server is closed."""
+    jobs = _jobs()[:2]   # 64^2 FP64 and 128^2 mixed FFT3D with CSG phantoms
+    alone = [_served_alone(ctx, j, 4) for j in jobs]
+    base = ctx.reserved()
+    srv = P.Server(ctx, slots=1, chunk_steps=4)
+
+    def run(j, **kw):
+        jid = srv.submit(j["prob"], j["field"], j["nsteps"], csg=j["csg"], **kw)
+        return srv.wait(jid, timeout=120.0)
+
+    rc, _, out = run(jobs[0])
+    assert rc == P.OK
+    np.testing.assert_array_equal(out, alone[0])
+    assert srv.stats()["hbm_reserved"] > 0            # its plan waits in the pool
+    rc, _, _ = run(jobs[0], inject_plan=P.INJECT_RESET)
+    announce("forced recycle failure", rc, srv.stats()["hbm_reserved"])
+    assert rc == P.ERR_INJECTED
+    assert srv.stats()["hbm_reserved"] == 0           # destroyed, bytes released
+    rc, _, _ = run(jobs[1], inject_plan=P.INJECT_CREATE)
+    assert rc == P.ERR_INJECTED
+    assert srv.stats()["hbm_reserved"] == 0
+    for j, ref in zip(jobs, alone):
+        rc, _, out = run(j)
+        assert rc == P.OK
+        np.testing.assert_array_equal(out, ref)
+    st = srv.stats()
+    assert st["failed"] == 2 and st["completed"] == 3
+    srv.close()
+    assert ctx.reserved() == base
