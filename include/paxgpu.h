@@ -305,6 +305,19 @@ int pxg_field_uint8(pxg_plan plan, int b, int source, int log_scale, const doubl
 enum { PXG_LAYOUT_CONTIGUOUS = 0, PXG_LAYOUT_INTERLEAVED = 1 };
 int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
                        double *rhs);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pxg_thomas_batched on DEVICE arrays diag/sub/rhs (same
+ * layouts and results, bit for bit), enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) and returned without waiting.  Its scratch (the eliminated
+ * super-diagonal, 16 bytes per row) is a static device array of
+ * PXG_THOMAS_NOALLOC_MAX_ROWS rows (16 MiB) that each device holds for the life
+ * of the process, so the call allocates nothing; nsys * n above that returns
+ * PXG_ERR_UNSUPPORTED.  All calls on one device share that array: calls that
+ * overlap on one device (other streams, other threads) race. */
+#define PXG_THOMAS_NOALLOC_MAX_ROWS (1u << 20)
+int pxg_thomas_batched_noalloc(pxg_ctx ctx, int layout, int nsys, int n, double *diag,
+                               const double *sub, double *rhs, void *stream);
 
 /* ---- serving ------------------------------------------------------------------ */
 /* A persistent request loop (no counterpart in the reference, which runs one

@@ -21,7 +21,18 @@
 
 using namespace pxg;
 
+// Scratch of pxg_thomas_batched_noalloc: the eliminated super-diagonal of every
+// row of one call (16 bytes per row), PXG_THOMAS_NOALLOC_MAX_ROWS rows, 16 MiB
+// per device for the life of the process.  Host code never uses this name as a
+// device address: pxg_ctx_create looks up each device's copy once
+// (hipGetSymbolAddress with that device current) into g_scratch_dev, and only
+// those addresses reach kernels.
+static __device__ __align__(256) char scratch[static_cast<size_t>(PXG_THOMAS_NOALLOC_MAX_ROWS) * 16];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_dev;  // per device: address of its copy of scratch (null: not looked up)
 
 thread_local std::string g_err;
 
@@ -994,6 +1005,17 @@ int pxg_ctx_create(int device, size_t hbm_budget_bytes, pxg_ctx *out)
     hip_check(hipSetDevice(device), "hipSetDevice");
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    {
+      // each device holds its own copy of the no-alloc scratch at its own address
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      if (g_scratch_dev.size() < static_cast<size_t>(count)) g_scratch_dev.resize(count, nullptr);
+      if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) fail(PXG_ERR_DEVICE, "no device address for the scratch array");
+        g_scratch_dev[device] = ptr;
+      }
+    }
     auto *c = new pxg_ctx_s;
     c->device = device;
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<size_t>(free_b * 0.9);
@@ -2010,6 +2032,35 @@ int pxg_thomas_batched(pxg_ctx ctx, int layout, int nsys, int n, double *diag, c
     hip_check(hipDeviceSynchronize(), "thomas sync");
     hip_check(hipMemcpy(diag, d.p, N * 16, hipMemcpyDeviceToHost), "thomas download");
     hip_check(hipMemcpy(rhs, r.p, N * 16, hipMemcpyDeviceToHost), "thomas download");
+  });
+}
+
+int pxg_thomas_batched_noalloc(pxg_ctx ctx, int layout, int nsys, int n, double *diag, const double *sub,
+                               double *rhs, void *stream)
+{
+  return guarded([&] {
+    if (!ctx) fail(PXG_ERR_INVALID, "null context");
+    if (nsys < 1 || n < 1) fail(PXG_ERR_INVALID, "need nsys >= 1 and n >= 1");
+    if (layout != PXG_LAYOUT_CONTIGUOUS && layout != PXG_LAYOUT_INTERLEAVED)
+      fail(PXG_ERR_INVALID, "unknown layout %d", layout);
+    if (!diag || !rhs || (n > 1 && !sub)) fail(PXG_ERR_INVALID, "null array");
+    const size_t rows = static_cast<size_t>(nsys) * static_cast<size_t>(n);
+    if (rows > PXG_THOMAS_NOALLOC_MAX_ROWS)
+      fail(PXG_ERR_UNSUPPORTED, "%zu rows (nsys x n) exceed the %u rows of the no-alloc scratch", rows,
+           static_cast<unsigned>(PXG_THOMAS_NOALLOC_MAX_ROWS));
+    void *work = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      if (static_cast<size_t>(ctx->device) < g_scratch_dev.size()) work = g_scratch_dev[ctx->device];
+    }
+    if (!work) fail(PXG_ERR_STATE, "scratch of device %d not set up", ctx->device);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    // the kernel writes every row's scratch entry before reading it: nothing to clear
+    hip_check(launch_thomas_batched(layout, nsys, n, reinterpret_cast<cx<double> *>(diag),
+                                    reinterpret_cast<const cx<double> *>(sub),
+                                    reinterpret_cast<cx<double> *>(rhs), static_cast<cx<double> *>(work),
+                                    static_cast<hipStream_t>(stream)),
+              "thomas kernel");
   });
 }
 

@@ -41,7 +41,9 @@ EXPORTS = [
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
     "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
+    "pxg_thomas_batched_noalloc",
 ]
+THOMAS_NOALLOC_MAX_ROWS = 1 << 20
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 INJECT_NONE, INJECT_CREATE, INJECT_RESET = range(3)
 MAT_NONE, MAT_CSG, MAT_CURVED_WG = 0, 1, 2
@@ -127,6 +129,7 @@ def lib():
         L.pxg_step_profiled.argtypes = [P, U, U, P, I, P, P]
         L.pxg_field_device_ptr.argtypes = [P, I, P]
         L.pxg_thomas_batched.argtypes = [P, I, I, I, P, P, P]
+        L.pxg_thomas_batched_noalloc.argtypes = [P, I, I, I, P, P, P, P]
         L.pxg_plan_create_jobs.argtypes = [P, P, I, P]
         L.pxg_plan_reset.argtypes = [P, P]
         L.pxg_plan_set_lines.argtypes = [P, I, I]
@@ -199,6 +202,12 @@ class Ctx:
             n, nsys = d.shape
         _check(lib().pxg_thomas_batched(self.h, layout, nsys, n, _p(d), _p(s), _p(r)))
         return d, r
+
+    def thomas_batched_noalloc(self, d_diag, d_sub, d_rhs, nsys, n, layout=LAYOUT_CONTIGUOUS,
+                               stream=None):
+        """pxg_thomas_batched_noalloc on device addresses (ints): enqueues the
+        solve on `stream` (None = the null stream) and returns without waiting."""
+        _check(lib().pxg_thomas_batched_noalloc(self.h, layout, nsys, n, d_diag, d_sub, d_rhs, stream))
 
     def close(self):
         if self.h:
