@@ -1,16 +1,21 @@
 // ParaxialSimulation (include/paraxialSimulation.hpp:20-236 of the reference):
-// grid, wavenumber, material dispatch and the initial field; HDF5 output and
-// post-processing modules are not part of this build.
+// grid, wavenumber, material dispatch, the initial field, the post-processing
+// modules and their HDF5 output.
 #pragma once
 
 #include <complex>
 #include <string>
+#include <vector>
 
 #include "arma_lite.hpp"
 #include "materialFunction.hpp"
 
 class Solver;
 class ParaxialSource;
+namespace post {
+class PostProcessingModule;
+class FarField;
+}  // namespace post
 
 struct Disctretization {
   double min{0.0};
@@ -57,8 +62,17 @@ class ParaxialSimulation {
   virtual void setBoundaryConditions(const ParaxialSource &src);
   virtual void getXrayMatProp(double x, double z, double &delta, double &beta) const;
   virtual void getXrayMatProp(double x, double y, double z, double &delta, double &beta) const;
-  /** HDF5 output is outside the GPU hot path of this build: throws. */
+  /** Writes every module's result to an HDF5 file (src/paraxialSimulation.cpp:137-193):
+   * group /data/ with the grid, wavenumber, solver and description attributes, one
+   * dataset /data/<module name> per module.  Needs libpaxpro_h5.so (built where the
+   * HDF5 C library is found); throws std::runtime_error without it. */
   virtual void save(const char *fname);
+  void save(const std::string &fname) { save(fname.c_str()); }
+  /** Adds a post-processing module whose result save() writes (src/paraxialSimulation.cpp:431-452) */
+  ParaxialSimulation &operator<<(post::PostProcessingModule &module);
+  ParaxialSimulation &operator<<(post::FarField &farfield);
+  ParaxialSimulation &addPostProcessingModule(post::PostProcessingModule &module) { return *this << module; }
+  ParaxialSimulation &addPostProcessingModule(post::FarField &farfield) { return *this << farfield; }
 
   std::string description{""};
   const MaterialFunction *material{nullptr};
@@ -69,4 +83,6 @@ class ParaxialSimulation {
   double wavenumber{1.0};
   std::string name;
   const ParaxialSource *src{nullptr};
+  std::vector<post::PostProcessingModule *> postProcess;
+  std::string groupname{"/data/"};
 };

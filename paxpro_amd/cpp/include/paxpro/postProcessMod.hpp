@@ -88,6 +88,8 @@ class FarField : public ProjectionQuantity {
   void linkParaxialSim(const ParaxialSimulation &simulation) { sim = &simulation; }
   double getMaxScatteringAngle() const { return phiMax; }
   double getMinScatteringAngle() const { return phiMin; }
+  /** phiMin, phiMax and qmin, qmax = phi k pi / 180 (src/farFieldPost.cpp:131-139) */
+  void addAttrib(std::vector<H5Attr> &attrs) const override;
 
  private:
   unsigned int signalLength{0};

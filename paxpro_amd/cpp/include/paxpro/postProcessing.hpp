@@ -1,12 +1,14 @@
 // post::PostProcessingModule and its two families (include/postProcessing.hpp:9-73,
 // src/postProcessing.cpp:7-56 of the reference).  The modules of this build compute
-// their results on the device from the solver's plan; the HDF5 attribute hook
-// (addAttrib) belongs to ParaxialSimulation::save, which is not part of this build.
+// their results on the device from the solver's plan; addAttrib lists the
+// attributes ParaxialSimulation::save attaches to a module's dataset.
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "arma_lite.hpp"
+#include "h5Attribute.hpp"
 
 class Solver;
 
@@ -25,6 +27,9 @@ class PostProcessingModule {
   virtual void result(const Solver &solver, arma::vec &res) {}
 
   std::string getName() const { return name; }
+
+  /** Attributes of this module's dataset in the HDF5 file (none by default) */
+  virtual void addAttrib(std::vector<H5Attr> &attrs) const {}
 
   /** Sets the size of exported matrices: block means down to rows x cols
    * (resizeMatrix, src/postProcessing.cpp:29-56), for the modules that resize. */

@@ -17,6 +17,16 @@ void post::FarField::setAngleRange(double angMin, double angMax)
   phiMax = angMax;
 }
 
+void post::FarField::addAttrib(std::vector<H5Attr> &attrs) const
+{
+  if (sim == nullptr) throw std::runtime_error("FarField: no ParaxialSimulation linked");
+  const double pi = std::acos(-1.0);
+  attrs.push_back(makeAttr("phiMin", phiMin));
+  attrs.push_back(makeAttr("phiMax", phiMax));
+  attrs.push_back(makeAttr("qmin", phiMin * sim->getWavenumber() * pi / 180.0));
+  attrs.push_back(makeAttr("qmax", phiMax * sim->getWavenumber() * pi / 180.0));
+}
+
 void post::FarField::verifyConsistentAngles(double dx, double dy, double wavenumber) const
 {
   const double d = dx > dy ? dy : dx;

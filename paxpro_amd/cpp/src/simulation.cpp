@@ -1,9 +1,16 @@
 // Problem definition of the drop-in API: discretisation, sources, CSG geometry
 // and waveguide materials.  Behaviour follows the reference files cited per
 // function; the solvers in solvers.cpp hand these to libpaxgpu.
-#include <cmath>
-#include <stdexcept>
+#include <dlfcn.h>
 
+#include <cmath>
+#include <iostream>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "paxpro/h5writer.h"
 #include "paxpro/paxpro.hpp"
 
 namespace {
@@ -106,9 +113,134 @@ void ParaxialSimulation::getXrayMatProp(double x, double y, double z, double &de
   }
 }
 
-void ParaxialSimulation::save(const char *)
+// src/paraxialSimulation.cpp:431-452
+ParaxialSimulation &ParaxialSimulation::operator<<(post::PostProcessingModule &module)
 {
-  throw std::runtime_error("ParaxialSimulation::save: HDF5 output is not part of this build");
+  postProcess.push_back(&module);
+  return *this;
+}
+
+ParaxialSimulation &ParaxialSimulation::operator<<(post::FarField &farfield)
+{
+  farfield.linkParaxialSim(*this);
+  postProcess.push_back(&farfield);
+  return *this;
+}
+
+namespace {
+
+typedef int (*pxh5_write_fn)(const char *, const char *, const pxh5_group_attr *, int, const pxh5_dataset *,
+                             int, char *, size_t);
+
+// pxh5_write of libpaxpro_h5.so, looked up next to this library on first use
+pxh5_write_fn h5_writer()
+{
+  static std::mutex mu;
+  static pxh5_write_fn fn = nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (fn) return fn;
+  Dl_info info{};
+  std::string dir = ".";
+  if (dladdr(reinterpret_cast<void *>(&h5_writer), &info) && info.dli_fname) {
+    dir = info.dli_fname;
+    const size_t slash = dir.rfind('/');
+    dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+  }
+  const std::string path = dir + "/libpaxpro_h5.so";
+  void *h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+  if (!h)
+    throw std::runtime_error("ParaxialSimulation::save: the HDF5 writer " + path +
+                             " is not available (built only where the HDF5 C library is found): " +
+                             std::string(dlerror()));
+  fn = reinterpret_cast<pxh5_write_fn>(dlsym(h, "pxh5_write"));
+  if (!fn) throw std::runtime_error("ParaxialSimulation::save: " + path + " has no pxh5_write");
+  return fn;
+}
+
+}  // namespace
+
+// src/paraxialSimulation.cpp:137-193 (the module loop), :454-511 (setGroupAttributes),
+// :286-317 (saveArray) with the dataspaces of include/hdf5DataspaceCreator.hpp:30-66:
+// a vector is {n_elem}, a matrix {n_cols, n_rows} and a cube {n_rows, n_cols, n_slices},
+// each written from the column-major buffer as it is stored.  The 8-bit cubes go out as
+// NATIVE_UINT8 without attributes (:180-182).
+void ParaxialSimulation::save(const char *h5fname)
+{
+  if (solver == nullptr) throw std::runtime_error("No solver specified!\n");
+  if (src == nullptr) throw std::runtime_error("No source specified!\n");
+  const pxh5_write_fn write = h5_writer();
+
+  std::vector<pxh5_group_attr> ga = {
+      {"xmin", nullptr, xDisc.min}, {"xmax", nullptr, xDisc.max}, {"dx", nullptr, xDisc.step},
+      {"zmin", nullptr, zDisc.min}, {"zmax", nullptr, zDisc.max}, {"dz", nullptr, zDisc.step},
+      {"ymin", nullptr, yDisc.min}, {"ymax", nullptr, yDisc.max}, {"dy", nullptr, yDisc.step},
+      {"wavenumber", nullptr, wavenumber}};
+  const std::string solvername = solver->getName();
+  ga.push_back({"solver", solvername.c_str(), 0.0});
+  if (description != "") ga.push_back({"description", description.c_str(), 0.0});
+
+  // results and attributes stay alive until the file is written
+  const size_t nm = postProcess.size();
+  std::vector<arma::vec> r1(nm);
+  std::vector<arma::mat> r2(nm);
+  std::vector<arma::cube> r3(nm);
+  std::vector<arma::Cube<unsigned char>> r8(nm);
+  std::vector<std::vector<H5Attr>> attrs(nm);
+  std::vector<std::vector<pxh5_attr>> cattrs(nm);
+  std::vector<std::string> names(nm);
+  std::vector<pxh5_dataset> ds(nm);
+  for (size_t i = 0; i < nm; ++i) {
+    post::PostProcessingModule &m = *postProcess[i];
+    names[i] = m.getName();
+    pxh5_dataset &d = ds[i];
+    d = pxh5_dataset{};
+    d.name = names[i].c_str();
+    m.addAttrib(attrs[i]);
+    switch (m.getReturnType(*solver)) {
+      case post::PostProcessingModule::ReturnType_t::vector1D:
+        m.result(*solver, r1[i]);
+        d.rank = 1;
+        d.dims[0] = r1[i].n_elem;
+        d.data = r1[i].memptr();
+        break;
+      case post::PostProcessingModule::ReturnType_t::matrix2D:
+        m.result(*solver, r2[i]);
+        d.rank = 2;
+        d.dims[0] = r2[i].n_cols;
+        d.dims[1] = r2[i].n_rows;
+        d.data = r2[i].memptr();
+        break;
+      case post::PostProcessingModule::ReturnType_t::cube3D:
+        d.rank = 3;
+        if (m.isUint8) {
+          m.result(*solver, r8[i]);
+          d.dims[0] = r8[i].n_rows;
+          d.dims[1] = r8[i].n_cols;
+          d.dims[2] = r8[i].n_slices;
+          d.uint8 = 1;
+          d.data = r8[i].memptr();
+          attrs[i].clear();
+        } else {
+          m.result(*solver, r3[i]);
+          d.dims[0] = r3[i].n_rows;
+          d.dims[1] = r3[i].n_cols;
+          d.dims[2] = r3[i].n_slices;
+          d.data = r3[i].memptr();
+        }
+        break;
+    }
+    for (const H5Attr &a : attrs[i])
+      cattrs[i].push_back({a.name.c_str(), a.dtype == H5Attr::Type::NATIVE_INT, a.value});
+    d.attrs = cattrs[i].data();
+    d.n_attrs = static_cast<int>(cattrs[i].size());
+  }
+  std::string group = groupname;
+  while (group.size() > 1 && group.back() == '/') group.pop_back();
+  char err[512] = {0};
+  if (write(h5fname, group.c_str(), ga.data(), static_cast<int>(ga.size()), ds.data(), static_cast<int>(ds.size()),
+            err, sizeof err) != 0)
+    throw std::runtime_error(std::string("ParaxialSimulation::save: ") + err);
+  for (size_t i = 0; i < nm; ++i) std::clog << "Dataset " << names[i] << " added to HDF5 file\n";
 }
 
 // ------------------------------------------------------------------- sources

@@ -230,6 +230,21 @@ static int run3d(const std::string &mode, int n, int nsteps, const char *out, in
     ff.result(*solver, far);
     write_raw(std::string(out) + ".ffconst", far.memptr(), far.n_elem * 8);
     std::printf("ffconst %zux%zu\n", (size_t)far.n_rows, (size_t)far.n_cols);
+    // the same modules through ParaxialSimulation::save
+    post::ExitIntensity ei;
+    post::ExitPhase ep;
+    ep.setExportDimensions(20, 30);
+    post::Intensity in;
+    post::LogIntensityUint8 lu8;  // IntensityUint8 shares Intensity's dataset name "amplitude"
+    lu8.setMinValue(1e-3);
+    lu8.setMaxValue(0.9);
+    post::FarField ff2;
+    ff2.setPadding(post::FarField::Pad_t::CONSTANT);
+    ff2.setPadLength(256);
+    ff2.setAngleRange(-1.0, 1.0);
+    sim.description = "drop-in sphere";
+    sim << ei << ep << in << lu8 << ff2;
+    sim.save(std::string(out) + ".h5");
   }
   std::printf("%s n=%d steps=%d cube=%zux%zux%zu\n", mode.c_str(), n, nsteps, (size_t)cube.n_rows,
               (size_t)cube.n_cols, (size_t)cube.n_slices);
@@ -354,12 +369,115 @@ static int scatter(const std::string &mode, int n, int nsteps, const char *out, 
   return 0;
 }
 
+// User-defined modules with known results (the reference lets any
+// PostProcessingModule subclass be added): a vector, a matrix with attributes, a
+// cube and an 8-bit cube, to check ParaxialSimulation::save without a device.
+struct KnownVector : post::ProjectionQuantity {
+  KnownVector() : post::ProjectionQuantity("knownVector") {}
+  using post::PostProcessingModule::result;
+  void result(const Solver &, arma::vec &res) override
+  {
+    res.set_size(5);
+    for (unsigned i = 0; i < 5; ++i) res[i] = 0.5 * i - 1.0;
+  }
+};
+struct KnownMatrix : post::ProjectionQuantity {
+  KnownMatrix() : post::ProjectionQuantity("knownMatrix") {}
+  using post::PostProcessingModule::result;
+  void result(const Solver &, arma::mat &res) override
+  {
+    res.set_size(3, 4);  // element (r, c) = 10 r + c, column-major
+    for (unsigned c = 0; c < 4; ++c)
+      for (unsigned r = 0; r < 3; ++r) res(r, c) = 10.0 * r + c;
+  }
+  void addAttrib(std::vector<H5Attr> &attrs) const override
+  {
+    attrs.push_back(makeAttr("scale", 2.5));
+    attrs.push_back(makeAttr("count", 7));
+  }
+};
+struct KnownCube : post::FieldQuantity {
+  KnownCube() : post::FieldQuantity("knownCube") {}
+  using post::PostProcessingModule::result;
+  void result(const Solver &, arma::cube &res) override
+  {
+    res.set_size(2, 3, 4);  // element (r, c, s) = 100 r + 10 c + s
+    for (unsigned sl = 0; sl < 4; ++sl)
+      for (unsigned c = 0; c < 3; ++c)
+        for (unsigned r = 0; r < 2; ++r) res(r, c, sl) = 100.0 * r + 10.0 * c + sl;
+  }
+  void addAttrib(std::vector<H5Attr> &attrs) const override { attrs.push_back(makeAttr("slices", 4)); }
+};
+struct KnownBytes : post::FieldQuantity {
+  KnownBytes() : post::FieldQuantity("knownBytes") { isUint8 = true; }
+  using post::PostProcessingModule::result;
+  void result(const Solver &, arma::Cube<unsigned char> &res) override
+  {
+    res.set_size(2, 2, 3);
+    for (unsigned i = 0; i < res.n_elem; ++i) res.memptr()[i] = static_cast<unsigned char>(20 * i + 1);
+  }
+  // the reference writes 8-bit cubes without attributes, whatever the module lists
+  void addAttrib(std::vector<H5Attr> &attrs) const override { attrs.push_back(makeAttr("dropped", 3)); }
+};
+
+// A field quantity of a 2D simulation is a matrix: the same values as KnownMatrix.
+struct KnownField2D : post::FieldQuantity {
+  KnownField2D() : post::FieldQuantity("knownField2D") {}
+  using post::PostProcessingModule::result;
+  void result(const Solver &s, arma::mat &res) override { KnownMatrix().result(s, res); }
+};
+
+// ParaxialSimulation::save of a 3D simulation (FFTSolver3D, Gaussian source, not
+// propagated) and of a 2D one (CrankNicholson, plane wave) holding the known
+// modules: no device work is needed.  A projection quantity is a matrix in 3D and
+// a vector in 2D (ProjectionQuantity::getReturnType), a field quantity a cube in
+// 3D and a matrix in 2D.
+static int h5_known(const std::string &out)
+{
+  {
+    ParaxialSimulation sim2("h5test2d");
+    sim2.setTransverseDiscretization(-40.0, 40.0, 2.5);
+    sim2.setLongitudinalDiscretization(0.0, 100.0, 10.0);
+    PlaneWave pw;
+    pw.setWavelength(0.1);
+    CrankNicholson cn;
+    sim2.setSolver(cn);
+    sim2.setBoundaryConditions(pw);
+    KnownVector v;
+    KnownField2D f;
+    KnownMatrix m;  // a projection quantity: a vector in 2D, and it has no vector result
+    sim2 << v << f << m;
+    sim2.save(out + ".2d.h5");
+  }
+  ParaxialSimulation sim("h5test");
+  sim.setTransverseDiscretization(-100.0, 100.0, 12.5, 2);
+  sim.setVerticalDiscretization(-100.0, 100.0, 12.5);
+  sim.setLongitudinalDiscretization(0.0, 50.0, 5.0, 5);
+  sim.description = "known modules";
+  GaussianBeam gb;
+  gb.setWaist(50.0);
+  gb.setWavelength(0.2);
+  gb.setDim(ParaxialSource::Dim_t::THREE_D);
+  FFTSolver3D fft;
+  sim.setSolver(fft);
+  sim.setBoundaryConditions(gb);
+  KnownMatrix m;
+  KnownCube c;
+  KnownBytes b;
+  sim << m;
+  sim.addPostProcessingModule(c).addPostProcessingModule(b);
+  sim.save(out);
+  std::printf("h5 saved %s\n", out.c_str());
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   try {
     if (argc < 2) return 2;
     const std::string mode = argv[1];
     if (mode == "kat") return kats();
+    if (mode == "h5known" && argc >= 3) return h5_known(argv[2]);
     if (argc < 5) return 2;
     const int n = std::atoi(argv[2]), nsteps = std::atoi(argv[3]);
     const int prec = argc > 5 ? std::atoi(argv[5]) : PXG_PREC_FP64;

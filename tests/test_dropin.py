@@ -2,6 +2,7 @@
 examples, compared with the CPU oracle on the same inputs.  The driver binary
 is built by __graft_entry__.build() (tests/dropin/Makefile)."""
 import os
+import pathlib
 import subprocess
 
 import numpy as np
@@ -67,6 +68,40 @@ def test_dropin_fft3d(tmp_path, mode):
         exp = O.far_field(load(out, (n, n)), None, 256, g.k, g.dx, g.dy, -1.0, 1.0, padding="constant")
         got = np.fromfile(out + ".ffconst", dtype=np.float64).reshape(exp.shape, order="F")
         assert rel_l2(got, exp) < 1e-11
+        _check_saved(out, n, cube.shape, g.k)
+
+
+def _check_saved(out, n, cshape, k):
+    """ParaxialSimulation::save of the same run: each module's dataset in the
+    reference's layout (matrices as {n_cols, n_rows}, cubes {n_rows, n_cols, n_slices}
+    over the column-major buffer) holds the values the modules returned above, and the
+    far field carries phiMin, phiMax, qmin, qmax (src/farFieldPost.cpp:131-139)."""
+    import test_h5_save as H
+    if not (os.path.exists(H.H5LIB) and os.path.exists(H.H5DUMP)):
+        pytest.skip("HDF5 writer or h5dump not available")
+    f = out + ".h5"
+    tmp = pathlib.Path(out).parent
+    _, blocks = H.header(f)
+    assert H.dims(blocks["exitIntensity"]) == (n, n)
+    np.testing.assert_array_equal(H.data(f, "/data/exitIntensity", np.float64, tmp).reshape((n, n)).T,
+                                  _dbl(out + ".exitint", (n, n)))
+    assert H.dims(blocks["exitPhase"]) == (30, 20)
+    np.testing.assert_array_equal(H.data(f, "/data/exitPhase", np.float64, tmp).reshape((30, 20)).T,
+                                  _dbl(out + ".exitphase_rs", (20, 30)))
+    assert H.dims(blocks["amplitude"]) == cshape
+    np.testing.assert_array_equal(H.data(f, "/data/amplitude", np.float64, tmp),
+                                  np.fromfile(out + ".intcube", dtype=np.float64))
+    assert "H5T_STD_U8LE" in blocks["LogAmplitude"] and H.dims(blocks["LogAmplitude"]) == cshape
+    np.testing.assert_array_equal(H.data(f, "/data/LogAmplitude", np.uint8, tmp),
+                                  np.fromfile(out + ".logu8", dtype=np.uint8))
+    ff = np.fromfile(out + ".ffconst", dtype=np.float64)
+    fshape = H.dims(blocks["farField"])
+    np.testing.assert_array_equal(H.data(f, "/data/farField", np.float64, tmp), ff)
+    for a, v in (("phiMin", -1.0), ("phiMax", 1.0), ("qmin", -k * np.pi / 180), ("qmax", k * np.pi / 180)):
+        assert float(H.attr_value(f, "/data/farField/" + a)) == pytest.approx(v, rel=1e-5)
+    assert H.attr_value(f, "/data/solver") == '"FFTSolver3D"'
+    assert H.attr_value(f, "/data/description") == '"drop-in sphere"'
+    assert len(fshape) == 2
 
 
 def _dbl(path, shape):
