@@ -335,20 +335,35 @@ static void fft_pow2_inplace(orc_c *a, unsigned n, const orc_c *w)
   }
 }
 
-static void dft_naive(orc_c *a, unsigned n, int sign)
+/* The definition, term by term in order; the factor of term (j, k) is the
+ * table entry of (j k mod n), cos + i sin of sign 2 pi (j k mod n)/n, i.e. the
+ * same value as evaluating it in place. */
+static void dft_table(orc_c *w, unsigned n, int sign)
+{
+  for (unsigned m = 0; m < n; ++m) {
+    double ang = (double)sign * 2.0 * ORC_PI * (double)m / (double)n;
+    w[m] = cos(ang) + I * sin(ang);
+  }
+}
+
+static void dft_naive_w(orc_c *a, unsigned n, const orc_c *w)
 {
   orc_c *o = (orc_c *)malloc(sizeof(orc_c) * n);
   for (unsigned k = 0; k < n; ++k) {
     orc_c s = 0.0;
-    for (unsigned j = 0; j < n; ++j) {
-      unsigned long long jk = ((unsigned long long)j * k) % n;
-      double ang = (double)sign * 2.0 * ORC_PI * (double)jk / (double)n;
-      s += a[j] * (cos(ang) + I * sin(ang));
-    }
+    for (unsigned j = 0; j < n; ++j) s += a[j] * w[((unsigned long long)j * k) % n];
     o[k] = s;
   }
   memcpy(a, o, sizeof(orc_c) * n);
   free(o);
+}
+
+static void dft_naive(orc_c *a, unsigned n, int sign)
+{
+  orc_c *w = (orc_c *)malloc(sizeof(orc_c) * n);
+  dft_table(w, n, sign);
+  dft_naive_w(a, n, w);
+  free(w);
 }
 
 void orc_fft1d(orc_c *a, unsigned n, int sign)
@@ -367,14 +382,26 @@ void orc_fft1d(orc_c *a, unsigned n, int sign)
 void orc_fft2d(orc_c *a, unsigned nrows, unsigned ncols, int sign)
 {
   if (!is_pow2(nrows) || !is_pow2(ncols)) {
-    orc_c *line = (orc_c *)malloc(sizeof(orc_c) * (nrows > ncols ? nrows : ncols));
-    for (unsigned c = 0; c < ncols; ++c) dft_naive(a + (size_t)c * nrows, nrows, sign);
-    for (unsigned r = 0; r < nrows; ++r) {
-      for (unsigned c = 0; c < ncols; ++c) line[c] = a[(size_t)c * nrows + r];
-      dft_naive(line, ncols, sign);
-      for (unsigned c = 0; c < ncols; ++c) a[(size_t)c * nrows + r] = line[c];
+    /* lines are independent: OpenMP over them changes no result */
+    orc_c *wr = (orc_c *)malloc(sizeof(orc_c) * nrows);
+    orc_c *wc = (orc_c *)malloc(sizeof(orc_c) * ncols);
+    dft_table(wr, nrows, sign);
+    dft_table(wc, ncols, sign);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long c = 0; c < (long)ncols; ++c) dft_naive_w(a + (size_t)c * nrows, nrows, wr);
+#pragma omp parallel
+    {
+      orc_c *line = (orc_c *)malloc(sizeof(orc_c) * ncols);
+#pragma omp for schedule(dynamic, 4)
+      for (long r = 0; r < (long)nrows; ++r) {
+        for (unsigned c = 0; c < ncols; ++c) line[c] = a[(size_t)c * nrows + r];
+        dft_naive_w(line, ncols, wc);
+        for (unsigned c = 0; c < ncols; ++c) a[(size_t)c * nrows + r] = line[c];
+      }
+      free(line);
     }
-    free(line);
+    free(wr);
+    free(wc);
     return;
   }
   orc_c *wr = (orc_c *)malloc(sizeof(orc_c) * (nrows / 2 + 1));

@@ -133,6 +133,9 @@ struct pxg_plan_s {
   size_t reserved = 0;
   int lines_x = 1, lines_y = 1;
   DBuf field, tw, kx, ky, absx, absy, kvec;
+  // FFT kinds with nx not a power of two in [16, 4096]: Bluestein length and tables
+  int blue_m = 0;
+  DBuf chirp, blue_bf, blue_bi;
   DBuf mcode, mxdb;  // FFT3D: material map of the current z-step (CSG materials)
   long map_step = -1;  // step whose map mcode holds (-1: none)
   std::vector<double> k_host;
@@ -241,8 +244,8 @@ void validate(const pxg_problem &p)
   if (p.nz < 1) fail(PXG_ERR_INVALID, "nz must be >= 1");
   if (p.batch < 1) fail(PXG_ERR_INVALID, "batch must be >= 1");
   if (!(p.dx > 0) || !(p.dz > 0) || !(p.k > 0)) fail(PXG_ERR_INVALID, "dx, dz and k must be > 0");
-  if ((p.kind == PXG_FFT3D || p.kind == PXG_FFT2D) && !fft_size_supported(p.nx))
-    fail(PXG_ERR_UNSUPPORTED, "FFT solvers need a power-of-two nx in [16, 4096] (got %d)", p.nx);
+  if ((p.kind == PXG_FFT3D || p.kind == PXG_FFT2D) && p.nx > 4096)
+    fail(PXG_ERR_UNSUPPORTED, "FFT solvers take nx up to 4096 (got %d)", p.nx);
   if ((p.kind == PXG_FFT3D || p.kind == PXG_ADI) && !(p.dy > 0)) fail(PXG_ERR_INVALID, "dy must be > 0");
   if (p.store_dsz < 0 || p.store_dsx < 0) fail(PXG_ERR_INVALID, "negative storage ratio");
   if (p.store_dsz > 0 && p.store_dsx < 1) fail(PXG_ERR_INVALID, "store_dsx must be >= 1");
@@ -307,18 +310,87 @@ cx<double> screen_factor(double k, double dz, double delta, double beta, double 
   return {std::cos(ph) * amp * inv_norm, std::sin(ph) * amp * inv_norm};
 }
 
+// In-place FP64 radix-2 DFT of a power-of-two length, X_k = sum_j x_j e^{sign 2 pi i jk/n},
+// with directly evaluated twiddles: the Bluestein kernel tables.
+void host_fft(std::vector<cx<double>> &a, int sign)
+{
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t k = 0; k < len / 2; ++k) {
+      const double ang = sign * 2.0 * PI * (static_cast<double>(k) / len);
+      const cx<double> w{std::cos(ang), std::sin(ang)};
+      for (size_t i = 0; i < n; i += len) {
+        const cx<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+// Bluestein tables of an n-point DFT on M = blue_length(n) points: the chirp
+// c_j = e^{-i pi j^2/n} (j^2 reduced mod 2n, so the angle is exact before its
+// one rounding) and the transformed convolution kernels FFT_M(conj c)/M
+// (forward) and FFT_M(c)/M (inverse), each kernel placed cyclically (entries
+// m and M - m for |m| < n).
+void build_bluestein(pxg_plan P, unsigned n, hipStream_t s)
+{
+  const unsigned M = static_cast<unsigned>(blue_length(static_cast<int>(n)));
+  P->blue_m = static_cast<int>(M);
+  std::vector<cx<double>> tw(M), chirp(n), kf(M, cx<double>{0.0, 0.0}), ki(M, cx<double>{0.0, 0.0});
+  for (unsigned m = 0; m < M; ++m) {
+    const double a = 2.0 * PI * (static_cast<double>(m) / M);
+    tw[m] = {std::cos(a), -std::sin(a)};
+  }
+  for (unsigned j = 0; j < n; ++j) {
+    const uint64_t r = (static_cast<uint64_t>(j) * j) % (2ull * n);
+    const double a = PI * (static_cast<double>(r) / n);
+    chirp[j] = {std::cos(a), -std::sin(a)};
+  }
+  for (unsigned m = 0; m < n; ++m) {
+    const cx<double> c = chirp[m];
+    kf[m] = conj(c);
+    ki[m] = c;
+    if (m) {
+      kf[M - m] = conj(c);
+      ki[M - m] = c;
+    }
+  }
+  host_fft(kf, -1);
+  host_fft(ki, -1);
+  for (unsigned m = 0; m < M; ++m) {
+    kf[m] = {kf[m].x / M, kf[m].y / M};
+    ki[m] = {ki[m].x / M, ki[m].y / M};
+  }
+  upload_cx<double>(P->tw, tw, s);
+  upload_cx<double>(P->chirp, chirp, s);
+  upload_cx<double>(P->blue_bf, kf, s);
+  upload_cx<double>(P->blue_bi, ki, s);
+}
+
 void build_tables(pxg_plan P)
 {
   const pxg_problem &p = P->p;
   const unsigned n = p.nx;
   hipStream_t s = P->stream;
   if (p.kind == PXG_FFT2D || p.kind == PXG_FFT3D) {
-    std::vector<cx<double>> tw(n);
-    for (unsigned m = 0; m < n; ++m) {
-      const double a = 2.0 * PI * (static_cast<double>(m) / n);
-      tw[m] = {std::cos(a), -std::sin(a)};
+    if (fft_size_supported(static_cast<int>(n))) {
+      P->blue_m = 0;
+      std::vector<cx<double>> tw(n);
+      for (unsigned m = 0; m < n; ++m) {
+        const double a = 2.0 * PI * (static_cast<double>(m) / n);
+        tw[m] = {std::cos(a), -std::sin(a)};
+      }
+      upload_cx<double>(P->tw, tw, s);
+    } else {
+      build_bluestein(P, n, s);
     }
-    upload_cx<double>(P->tw, tw, s);
     std::vector<cx<double>> kx(static_cast<size_t>(n) * p.batch), ky(kx.size());
     for (int b = 0; b < p.batch; ++b)
       for (unsigned i = 0; i < n; ++i) {
@@ -435,7 +507,12 @@ FftArgs<R> fft_args(pxg_plan P)
   a.mxdb = P->mxdb.as<cd>();
   // the map is computed once per step for the batch: inside pass X when the
   // batch gives it many work-groups to spread over, else in its own launch
-  a.map_in_pass_x = p.batch >= MAP_IN_PASS_X_BATCH;
+  // (always for the Bluestein passes)
+  a.blue_m = P->blue_m;
+  a.chirp = P->chirp.as<cd>();
+  a.bf = P->blue_bf.as<cd>();
+  a.bi = P->blue_bi.as<cd>();
+  a.map_in_pass_x = p.batch >= MAP_IN_PASS_X_BATCH && !P->blue_m;
   return a;
 }
 
@@ -501,7 +578,7 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
 // up to 16, at most 1024 lanes, FP64 line storage within the LDS of one CU.
 bool lines_supported(int n, int l)
 {
-  return l >= 1 && l <= 16 && (l & (l - 1)) == 0 && l * (n / 8) <= 1024 && n % l == 0 &&
+  return fft_size_supported(n) && l >= 1 && l <= 16 && (l & (l - 1)) == 0 && l * (n / 8) <= 1024 && n % l == 0 &&
          static_cast<size_t>(l) * (n + n / 8 + 1) * sizeof(cd) <= 163840 - 1024;
 }
 
@@ -1082,8 +1159,11 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     if (prob->kind == PXG_FFT3D) {
       P->mcode.alloc(P->cells * sizeof(uint16_t));
       P->mxdb.alloc(P->cells * sizeof(cd));
+    }
+    if (prob->kind == PXG_FFT3D && fft_size_supported(prob->nx)) {
       // untuned defaults (pxg_plan_autotune measures): pass Y one wavefront of
-      // lines per work-group, pass X wide work-groups for coalescing
+      // lines per work-group, pass X wide work-groups for coalescing; the
+      // Bluestein passes of other lengths run one line per work-group
       P->lines_y = std::min(max_lines(prob->nx, 81920), std::max(1, 512 / prob->nx));
       P->lines_x = max_lines(prob->nx, 162816);
       // keep enough work-groups in flight for small batches
@@ -1521,6 +1601,9 @@ int pxg_plan_set_lines(pxg_plan P, int lines_x, int lines_y)
     check_plan(P);
     if (P->p.kind != PXG_FFT3D) fail(PXG_ERR_UNSUPPORTED, "work-group shapes apply to FFT3D plans only");
     const int n = P->p.nx;
+    if (P->blue_m && (lines_x > 1 || lines_y > 1))
+      fail(PXG_ERR_UNSUPPORTED, "nx=%d (Bluestein passes) runs one line per work-group", n);
+    if (P->blue_m) return;
     if ((lines_x && !lines_supported(n, lines_x)) || (lines_y && !lines_supported(n, lines_y)))
       fail(PXG_ERR_INVALID, "lines per work-group (%d, %d) not supported at nx=%d", lines_x, lines_y, n);
     if (lines_x) P->lines_x = lines_x;
@@ -1542,7 +1625,7 @@ int pxg_plan_autotune(pxg_plan P, int reps)
 {
   return guarded([&] {
     check_plan(P);
-    if (P->p.kind != PXG_FFT3D) return;  // the other kinds have one shape
+    if (P->p.kind != PXG_FFT3D || P->blue_m) return;  // the other kinds have one shape
     const std::array<int, 4> key = {P->p.nx, P->p.batch, P->p.precision, P->mat_mode};
     {
       std::lock_guard<std::mutex> lk(P->ctx->mu);

@@ -18,22 +18,11 @@
 #include "pxg_fft.hpp"
 #include "pxg_fp64math.hpp"
 #include "pxg_internal.hpp"
+#include "pxg_screen.hpp"
 
 namespace pxg {
 
 constexpr int MAX_MASK_WORDS = 16;  // CSG parts per material: up to 1024
-
-// Phase-screen factor exp(-k (beta + i delta) dz) / N^2 from FP64 material
-// values, in FP64 (src/fftSolver3D.cpp:154).  Its callers are rare per-cell
-// loops; exp_fp64/sincos_fp64 keep it inline without many registers.
-__device__ __forceinline__ cd screen_from_values(double k, double dz, double delta, double beta,
-                                                   double inv_norm)
-{
-  const double amp = exp_fp64((-k * beta) * dz);
-  double s, c;
-  sincos_fp64((-k * delta) * dz, s, c);
-  return {c * amp * inv_norm, s * amp * inv_norm};
-}
 
 // Border-crossing integral of FFTSolver3D::refractionIntegral
 // (src/fftSolver3D.cpp:253-273): f(z0) + f(z1) + 2 sum_{i=1}^{n-2} f(z0 + i dz/n),
@@ -366,6 +355,23 @@ __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes);
 }
 
+// The same for line lengths n that are not a multiple of the strip: cells past
+// the last row are clamped to row n - 1, so those lanes repeat that cell's work
+// and write the same code (and border values) to it.
+template <class R, int CPL>
+__global__ __launch_bounds__(64) void k_material_map_clamped(FftArgs<R> a, int n)
+{
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  __shared__ __align__(16) char scr[map_scratch_bytes<CPL>(64)];
+  const int lanes = static_cast<int>(blockDim.x);
+  const int c = blockIdx.y, r0 = blockIdx.x * lanes * CPL;
+  const int r = r0 + static_cast<int>(threadIdx.x);
+  const int rlast = r0 + lanes * CPL - 1 < n - 1 ? r0 + lanes * CPL - 1 : n - 1;
+  material_map_cells<CPL>(a, mask, n, c, c, r0, rlast,
+                          [&](int m) { return make_int2(c, r + m * lanes < n ? r + m * lanes : n - 1); }, scr,
+                          lanes);
+}
+
 // ---------------------------------------------------------------------------
 // Pass X: strided x-lines (matrix rows).  Global loads and stores run with the
 // lines fastest across lanes (mapping A), so that each instruction touches L
@@ -430,24 +436,8 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 // ---------------------------------------------------------------------------
 // FFTSolver2D step (src/fftSolver2D.cpp:15-74): one field per line,
 // FFT -> x exp(-i dz/(2k) kx^2) -> IFFT -> x exp(-k(beta + i delta) dz)/N with
-// the material at z_step + dz/2; the result is also stored as solution column.
-__device__ __forceinline__ cd screen_2d(const MatDev &m, const uint64_t *mask, int nw, unsigned step,
-                                     int i, double x, double z, double k, double dz, double inv_norm)
-{
-  double d = 0.0, be = 0.0;
-  if (m.mode == MAT_CSG) {
-    part_values(m, csg_eval(m, mask, nw, x, 0.0, z), d, be);
-  } else if (m.mode == MAT_CURVED) {
-    curved_material(m.wg, x, z, d, be);
-  } else if (m.mode == MAT_PLANES) {
-    const int p = m.z_invariant ? 0 : static_cast<int>(step) - m.first_plane;
-    const double *pl = m.planes + static_cast<size_t>(p) * m.plane_stride;
-    d = pl[2 * i];
-    be = pl[2 * i + 1];
-  }
-  return screen_from_values(k, dz, d, be, inv_norm);
-}
-
+// the material at z_step + dz/2 (screen_2d); the result is also stored as
+// solution column.
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_fft2d_step(FftArgs<R> a)
 {
@@ -519,6 +509,7 @@ bool fft_size_supported(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) ==
 // storage (gfx950: 160 KiB of LDS per CU and per work-group).
 int max_lines(int n, int lds_cap)
 {
+  if (n < 8) return 1;
   int l = 1024 / (n / 8);
   while (l > 1 && l * (n + n / 8 + 1) * static_cast<int>(sizeof(cd)) > lds_cap) l /= 2;
   return l > 16 ? 16 : l;
@@ -563,6 +554,7 @@ int max_lines(int n, int lds_cap)
 template <class R>
 hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s)
 {
+  if (a.blue_m) return launch_blue_pass_y<R>(n, a, batch, s);
   auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, batch); };
   PXG_SIZE_SWITCH(k_pass_y)
 }
@@ -570,6 +562,7 @@ hipError_t launch_pass_y(int n, int lines, const FftArgs<R> &a, int batch, hipSt
 template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s)
 {
+  if (a.blue_m) return launch_blue_pass_x<R>(n, a, batch, s);
   auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, batch); };
   PXG_SIZE_SWITCH(k_pass_x)
 }
@@ -581,7 +574,10 @@ hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
   // lists short.  From n = 128 on a strip is 128 rows, two cells per lane:
   // one culling per 128 cells instead of 64.
   // (four cells per lane took 98 registers, 4 waves per SIMD, and was slower)
-  if (n >= 128) {
+  if (a.blue_m) {
+    // other lengths: strips of 128 rows, the last one clamped to row n - 1
+    hipLaunchKernelGGL((k_material_map_clamped<R, 2>), dim3((n + 127) / 128, n), dim3(64), 0, s, a, n);
+  } else if (n >= 128) {
     hipLaunchKernelGGL((k_material_map<R, 2>), dim3(n / 128, n), dim3(64), 0, s, a, n);
   } else {
     const int rows = n < 64 ? n : 64;  // n is a power of two >= 16
@@ -593,6 +589,7 @@ hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
+  if (a.blue_m) return launch_blue_fft2d<R>(n, a, batch, s);
   // one field per line; 4 lines per work-group when the batch divides evenly
   const int lines = (batch % 4 == 0 && max_lines(n, 81920) >= 4) ? 4 : 1;
   auto grid_of = [&](int, int ll) { return dim3(batch / ll, 1); };

@@ -35,6 +35,11 @@ struct FftArgs {
   cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
   int map_in_pass_x;       // build the map inside pass X (large batches) or in its own launch
   int map_prev_valid;      // mcode holds the map of step - 1: reuse its parts as the z0 plane
+  // Line lengths without a power-of-two kernel (pxg_bluestein.hip): N-point DFTs as
+  // Bluestein convolutions of M = blue_m points (tw then holds the M-point table)
+  int blue_m;              // 0: power-of-two path
+  const cd *chirp;         // [N] exp(-i pi j^2 / N), j^2 reduced mod 2N
+  const cd *bf, *bi;       // [M] FFT_M of the forward / inverse convolution kernel, / M
 };
 
 // Bit of a material-map code marking a border cell (material differs between
@@ -52,6 +57,16 @@ template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
+// Bluestein variants of the passes for other line lengths (pxg_bluestein.hip);
+// the launchers above hand over to them when n has no power-of-two kernel.
+template <class R>
+hipError_t launch_blue_pass_y(int n, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_blue_pass_x(int n, const FftArgs<R> &a, int batch, hipStream_t s);
+template <class R>
+hipError_t launch_blue_fft2d(int n, const FftArgs<R> &a, int batch, hipStream_t s);
+// Bluestein transform length for n points: the power of two >= 2n - 1 (at least 16)
+int blue_length(int n);
 template <class R>
 hipError_t launch_cube_store(const cx<R> *field, size_t fstride, int n, int cube_n,
                              cx<double> *cube, size_t cube_fstride, int slice, int batch,

@@ -33,7 +33,7 @@ def test_plan_bytes_and_validation_without_gpu():
     prob = P.problem(P.FFT3D, 512, 1001, 39.0625, 10.0, 62.83, batch=4, precision=P.PREC_MIXED)
     b = P.plan_bytes(prob)
     assert b >= 512 * 512 * 4 * 8
-    bad = P.problem(P.FFT3D, 500, 1001, 40.0, 10.0, 62.83)
+    bad = P.problem(P.FFT3D, 5000, 1001, 4.0, 10.0, 62.83)   # beyond 4096 points per side
     with pytest.raises(P.PxgError) as e:
         P.plan_bytes(bad)
     assert e.value.code == P.ERR_UNSUPPORTED

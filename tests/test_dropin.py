@@ -44,9 +44,10 @@ def _grid3d(n, nsteps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["fft3d", "fft3d_user", "fft3d_step", "fft3d_abs", "fft3d_reset"])
-def test_dropin_fft3d(tmp_path, mode):
-    n, nsteps = 64, 8
+@pytest.mark.parametrize("mode,n", [("fft3d", 64), ("fft3d_user", 64), ("fft3d_step", 64), ("fft3d_abs", 64),
+                                    ("fft3d_reset", 64), ("fft3d", 100), ("fft3d_user", 100)])
+def test_dropin_fft3d(tmp_path, mode, n):
+    nsteps = 8
     out = run(tmp_path, mode, n, nsteps)
     g = _grid3d(n, nsteps)
     u0 = gaussian_field(g, n, 3000.0)
