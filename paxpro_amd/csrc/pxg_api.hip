@@ -136,6 +136,9 @@ struct pxg_plan_s {
   // FFT kinds with nx not a power of two in [16, 4096]: Bluestein length and tables
   int blue_m = 0;
   DBuf chirp, blue_bf, blue_bi;
+  // far field: twiddle tables of the last signal length and transform length used
+  DBuf ff_twm, ff_twn;
+  unsigned ff_twm_len = 0, ff_twn_len = 0;
   DBuf mcode, mxdb;  // FFT3D: material map of the current z-step (CSG materials)
   long map_step = -1;  // step whose map mcode holds (-1: none)
   std::vector<double> k_host;
@@ -1774,8 +1777,7 @@ FfWindow ff_window(pxg_plan P, int b, unsigned pad, double phi_min, double phi_m
   const unsigned n = static_cast<unsigned>(P->p.nx);
   // signalLength = max(pad, rows, cols) (src/farFieldPost.cpp:146-150)
   W.M = std::max(pad, n);
-  if (W.M < 8 || W.M > (1u << 24) || (W.M & (W.M - 1)))
-    fail(PXG_ERR_UNSUPPORTED, "far-field signal length %u: powers of two from 8 to 2^24 only", W.M);
+  if (W.M > (1u << 24)) fail(PXG_ERR_UNSUPPORTED, "far-field signal length %u above 2^24", W.M);
   const double k = P->k_host[b];
   W.ylo = ff_angle_index(phi_min, W.M, k, P->p.dy);
   W.yhi = ff_angle_index(phi_max, W.M, k, P->p.dy);
@@ -1854,9 +1856,18 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
     if (P->ctx->reserved.load() + need > P->ctx->budget)
       fail(PXG_ERR_BUDGET, "far field needs %zu bytes of scratch", need);
     hipStream_t s = P->stream;
-    DBuf twm, twn, tmp, res, rsz;
-    upload_cx<double>(twm, twiddle_table(W.M), s);
-    upload_cx<double>(twn, twiddle_table(static_cast<unsigned>(np)), s);
+    // the M- and NP-entry twiddle tables stay with the plan for later calls of the same lengths
+    if (P->ff_twm_len != W.M) {
+      P->ff_twm_len = 0;
+      upload_cx<double>(P->ff_twm, twiddle_table(W.M), s);
+      P->ff_twm_len = W.M;
+    }
+    if (P->ff_twn_len != static_cast<unsigned>(np)) {
+      P->ff_twn_len = 0;
+      upload_cx<double>(P->ff_twn, twiddle_table(static_cast<unsigned>(np)), s);
+      P->ff_twn_len = static_cast<unsigned>(np);
+    }
+    DBuf tmp, res, rsz;
     tmp.alloc(static_cast<size_t>(W.rows) * n * sizeof(cd));
     res.alloc(static_cast<size_t>(W.rows) * W.cols * sizeof(double));
     FfArgs a{};
@@ -1869,8 +1880,8 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
     a.n = n;
     a.M = static_cast<int>(W.M);
     a.D = static_cast<int>(W.M) / np;
-    a.tw_m = twm.as<cd>();
-    a.tw_np = twn.as<cd>();
+    a.tw_m = P->ff_twm.as<cd>();
+    a.tw_np = P->ff_twn.as<cd>();
     a.ylo = static_cast<int>(W.ylo);
     a.ny = W.rows;
     a.xlo = static_cast<int>(W.xlo);
@@ -1909,7 +1920,11 @@ int pxg_far_field_padded(pxg_plan P, int b, pxg_plan R, int rb, unsigned pad, in
       a.lin_x2end = x2[9];
       a.lin_wrap = (P->p.quirks & PXG_FIX_LINEAR_PAD) == 0;
     }
-    hip_check(launch_far_field(a, np, s), "far field");
+    const bool pow2 = (W.M & (W.M - 1)) == 0 && W.M >= static_cast<unsigned>(np);
+    if (pow2)
+      hip_check(launch_far_field(a, np, s), "far field");
+    else
+      hip_check(launch_far_field_direct(a, s), "far field");
     const double *src = res.as<double>();
     if (W.resize) {
       rsz.alloc(static_cast<size_t>(W.out_rows) * W.out_cols * sizeof(double));

@@ -63,7 +63,7 @@ __device__ __forceinline__ cd ff_pad_term(const FfArgs &a, cd x0, long f)
   if (f == 0) {
     g = cd{static_cast<double>(a.M - a.n), 0.0};
   } else {
-    const cd wfn = a.tw_m[(static_cast<size_t>(f) * a.n) & (a.M - 1)];
+    const cd wfn = a.tw_m[(static_cast<size_t>(f) * a.n) % static_cast<size_t>(a.M)];
     const cd wf = a.tw_m[f];
     const cd num{wfn.x - 1.0, wfn.y};
     const cd den{1.0 - wf.x, -wf.y};
@@ -160,8 +160,8 @@ __device__ __forceinline__ cd ff_linear_term(const FfArgs &a, const LinFit &B, c
   }
   const cd PE = E.a + E.b * cd{a.lin_x2end, 0.0};
   const cd QE = E.b * cd{a.lin_dxf, 0.0};
-  const cd phB = a.tw_m[(static_cast<size_t>(f) * static_cast<size_t>(a.M - L)) & (a.M - 1)];
-  const cd phE = a.tw_m[(static_cast<size_t>(f) * a.n) & (a.M - 1)];
+  const cd phB = a.tw_m[(static_cast<size_t>(f) * static_cast<size_t>(a.M - L)) % static_cast<size_t>(a.M)];
+  const cd phE = a.tw_m[(static_cast<size_t>(f) * a.n) % static_cast<size_t>(a.M)];
   cd segB;
   if (a.lin_wrap) {
     const cd C = B.a + B.b * cd{a.lin_dxf * 18446744073709551616.0, 0.0};   // 2^64
@@ -287,6 +287,89 @@ __global__ __launch_bounds__(256) void k_ff_resize(const double *in, int rows, i
     s += cs / (r1 - r0 + 1);
   }
   out[idx] = s / (c1 - c0 + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Signal lengths M that are not a power of two (FFTW takes any length,
+// src/farFieldPost.cpp:53): every output inside the window is its DFT sum,
+// X[f] = sum_{j<n} x_j w_M^{f j} (w_M from the M-entry table, the index f j mod M
+// advanced by f per term), plus the padded segments' closed-form terms.  The
+// window gives the frequencies: the reference's fftshift puts X[f] at position
+// f + h for f < h, f - h for h <= f < 2h and leaves position 2h (odd M) in place,
+// h = M/2 (src/farFieldPost.cpp:60-71).  One work-group per line, its values in
+// LDS, 256 lanes over the window's outputs.
+__device__ __forceinline__ long ff_unshift(long p, long M)
+{
+  const long h = M / 2;
+  return p < h ? p + h : (p < 2 * h ? p - h : p);
+}
+
+__device__ __forceinline__ cd ff_direct_sum(const cd *xs, int n, long f, const cd *tw, long M)
+{
+  cd s{0.0, 0.0};
+  long idx = 0;
+  for (int j = 0; j < n; ++j) {
+    s = s + xs[j] * tw[idx];
+    idx += f;
+    if (idx >= M) idx -= M;
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_ff_cols_direct(FfArgs a)
+{
+  __shared__ cd xs[4096];
+  const int c = blockIdx.x;
+  for (int j = threadIdx.x; j < a.n; j += blockDim.x) {
+    const size_t i = static_cast<size_t>(c) * a.n + j;
+    cd u = ff_load(a.field, a.field_c64, i);
+    if (a.ref) u = u - ff_load(a.ref, a.ref_c64, i);
+    xs[j] = u;
+  }
+  __syncthreads();
+  LinFit fb{}, fe{};
+  if (a.pad_linear) {
+    fb = lin_fit(a.fit1_dx, a.fit1_mx, a.fit1_mdx2, xs);
+    fe = lin_fit(a.fit2_dx, a.fit2_mx, a.fit2_mdx2, xs);
+  }
+  for (int r = threadIdx.x; r < a.ny; r += blockDim.x) {
+    const long f = ff_unshift(a.ylo + r, a.M);
+    cd v = ff_direct_sum(xs, a.n, f, a.tw_m, a.M);
+    if (a.pad_const) v = v + ff_pad_term(a, xs[0], f);
+    if (a.pad_linear) v = v + ff_linear_term(a, fb, fe, f);
+    a.tmp[static_cast<size_t>(c) * a.ny + r] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ff_rows_direct(FfArgs a)
+{
+  __shared__ cd xs[4096];
+  const int r = blockIdx.x;
+  for (int j = threadIdx.x; j < a.n; j += blockDim.x) xs[j] = conj(a.tmp[static_cast<size_t>(j) * a.ny + r]);
+  __syncthreads();
+  LinFit fb{}, fe{};
+  if (a.pad_linear) {
+    fb = lin_fit(a.fit1_dx, a.fit1_mx, a.fit1_mdx2, xs);
+    fe = lin_fit(a.fit2_dx, a.fit2_mx, a.fit2_mdx2, xs);
+  }
+  for (int col = threadIdx.x; col < a.nx; col += blockDim.x) {
+    const long f = ff_unshift(a.xlo + col, a.M);
+    cd v = ff_direct_sum(xs, a.n, f, a.tw_m, a.M);
+    if (a.pad_const) v = v + ff_pad_term(a, xs[0], f);
+    if (a.pad_linear) v = v + ff_linear_term(a, fb, fe, f);
+    const double mag = hypot(v.x, v.y);
+    a.out[static_cast<size_t>(col) * a.ny + r] = mag * mag * a.inv_len;
+  }
+}
+
+hipError_t launch_far_field_direct(const FfArgs &a, hipStream_t s)
+{
+  if (a.n > 4096) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ff_cols_direct, dim3(a.n), dim3(256), 0, s, a);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_ff_rows_direct, dim3(a.ny), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 #define PXG_FF_CASE(NN)                                                                   \

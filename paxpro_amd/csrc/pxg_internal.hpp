@@ -100,6 +100,8 @@ struct FfArgs {
   int lin_wrap;            // first segment's ramp with the reference's unsigned index difference
 };
 hipError_t launch_far_field(const FfArgs &a, int np, hipStream_t s);
+// Signal lengths that are not a power of two: the window's DFT sums directly.
+hipError_t launch_far_field_direct(const FfArgs &a, hipStream_t s);
 hipError_t launch_ff_resize(const double *in, int rows, int cols, double *out, int er, int ec,
                             hipStream_t s);
 // Trapezoid sums of |u|^2 over per-step index ranges of a stored 2D solution

@@ -49,6 +49,8 @@ def _random_field(n, seed):
     (P.FFT3D, 128, P.PREC_MIXED, 2048, 0.5, (16, 12)),
     (P.ADI, 48, P.PREC_FP64, 256, 2.0, (0, 0)),           # n below NP = 64
     (P.FFT3D, 64, P.PREC_FP64, 8192, 0.3, (0, 0)),        # linear fit: 4064-entry segments
+    (P.FFT3D, 100, P.PREC_FP64, 333, 3.0, (0, 0)),        # odd signal length, direct sums
+    (P.ADI, 50, P.PREC_FP64, 3000, 1.0, (0, 0)),          # even M, 1475-entry segments
 ])
 @pytest.mark.parametrize("padding", ["constant", "linear", "linear_signed"])
 def test_far_field_padding_vs_oracle(ctx, kind, n, prec, pad, phi, export, padding):
@@ -88,6 +90,9 @@ def test_far_field_padding_vs_oracle(ctx, kind, n, prec, pad, phi, export, paddi
     (P.FFT3D, 128, P.PREC_MIXED, 2048, 0.5, (16, 12)),    # complex64 fields, export resize
     (P.ADI, 48, P.PREC_FP64, 256, 2.0, (0, 0)),           # n below NP = 64, zero-filled
     (P.FFT3D, 512, P.PREC_FP64, 32768, 0.05, (512, 512)), # the Python example's pad length
+    (P.FFT3D, 100, P.PREC_FP64, 0, 90.0, (0, 0)),         # M = n = 100: direct window sums
+    (P.FFT3D, 99, P.PREC_MIXED, 0, 90.0, (20, 30)),       # odd M: fftshift leaves the last entry
+    (P.ADI, 64, P.PREC_FP64, 1000, 3.0, (0, 0)),          # pad length not a power of two
 ])
 def test_far_field_of_given_fields_vs_oracle(ctx, kind, n, prec, pad, phi, export):
     announce("far field", kind, n, prec, pad, phi, export)
@@ -174,7 +179,7 @@ def test_far_field_errors(ctx):
     assert e.value.code == 5
     plan.set_field(_random_field(64, 3))
     with pytest.raises(P.PxgError) as e:
-        plan.far_field(0, None, 0, 1000)  # not a power of two
+        plan.far_field(0, None, 0, (1 << 24) + 1)  # signal length beyond 2^24
     assert e.value.code == 2
     with pytest.raises(P.PxgError) as e:
         plan.far_field(0, None, 0, 1024, 1.0, -1.0)  # empty window
