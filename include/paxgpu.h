@@ -204,6 +204,15 @@ int pxg_set_material_curved_wg(pxg_plan plan, const pxg_curved_wg *wg);
 int pxg_set_material_planes(pxg_plan plan, const double *db, int n_planes, int first_plane,
                             int z_invariant);
 int pxg_set_material_none(pxg_plan plan);
+/* Replaces Solver2D::setEquation (src/solver2D.cpp:44-49) with a ParaxialEquation
+ * other than the default, as CrankNicholson::solveStep evaluates it
+ * (src/crancNicholson.cpp:41-60,83-88): the caller samples it on the host.  coef holds
+ * n_planes planes of 5 values per cell: F(x,z), G(x,z), H(x + dx/2, z), H(x - dx/2, z),
+ * J(x,z), x = xmin + i dx; plane p is z = zmin + p dz (step iz uses planes iz and
+ * iz - 1); z_invariant != 0: one plane for every z.  coef = NULL restores the default
+ * (F = G = H = 1, J = 0).  Single CN plans only. */
+int pxg_set_equation_planes(pxg_plan plan, const double *coef, int n_planes, int first_plane,
+                            int z_invariant);
 
 /* ---- stepping ----------------------------------------------------------------- */
 /* Replaces Solver2D::step/solve (src/solver2D.cpp:224-250) and Solver3D::step/solve

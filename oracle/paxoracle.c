@@ -594,20 +594,38 @@ void orc_fft2d_run(const orc_grid *g, const orc_material *m, orc_c *field, unsig
 }
 
 /* ========================================================================== */
-/* CrankNicholson::solveStep with the default ParaxialEquation F=G=H=1, J=0
- * (src/crancNicholson.cpp:17-105, src/paraxialEquation.cpp:3-21), followed by
- * applyTBC (:107-148) and the Thomas solve. */
-void orc_cn_step_planes(double rho, double r, int bc, const orc_c *prev, orc_c *cur, unsigned nx,
-                        const double *d1, const double *b1, const double *d0, const double *b0)
+/* CrankNicholson::solveStep (src/crancNicholson.cpp:17-105) followed by applyTBC
+ * (:107-148) and the Thomas solve.  Per row ix: diag = F + 0.25 (H+ + H-) G i rho
+ * + 0.5 (beta r + i delta r) with delta -= J; sub[ix] (the coupling of ix and ix+1,
+ * sup = sub) = -0.25 i rho H- G of row ix; rhs = 0.25 i rho (u_{ix-1} H-' G' +
+ * u_{ix+1} H+' G') - 0.25 u i rho (H+' + H-') G + (F' - 0.5 (beta' r + i delta' r)) u,
+ * primes at z - dz; note G (not G') in the centre term (:87).  The transparent
+ * boundary uses H-, G, H-', G' of the last row for both sides (:146-147). */
+void orc_poly_eqfn(const void *ctx, double x, double z, double out[4])
+{
+  const orc_poly_eq *e = (const orc_poly_eq *)ctx;
+  for (int q = 0; q < 4; ++q)
+    out[q] = e->c[q][0] + e->c[q][1] * x + e->c[q][2] * z + (e->c[q][3] != 0.0 ? e->c[q][3] / x : 0.0);
+}
+
+void orc_cn_step_eq(double rho, double r, int bc, const orc_c *prev, orc_c *cur, unsigned nx,
+                    const double *d1, const double *b1, const double *d0, const double *b0,
+                    const double *e1, const double *e0)
 {
   orc_c *sub = (orc_c *)calloc(nx > 1 ? nx - 1 : 1, sizeof(orc_c));
   orc_c *rhs = (orc_c *)malloc(sizeof(orc_c) * nx);
   orc_c *diag = (orc_c *)malloc(sizeof(orc_c) * nx);
-  const double Hp = 1.0, Hm = 1.0, gv = 1.0, fv = 1.0;
-  const double HpP = 1.0, HmP = 1.0, gP = 1.0, fP = 1.0;
+  double Hp = 1.0, Hm = 1.0, gv = 1.0, HpP = 1.0, HmP = 1.0, gP = 1.0;
   for (unsigned ix = 0; ix < nx; ++ix) {
-    double delta = d1[ix] - 0.0, beta = b1[ix];
-    double deltaP = d0[ix] - 0.0, betaP = b0[ix];
+    double fv = 1.0, jv = 0.0, fP = 1.0, jP = 0.0;
+    if (e1) {
+      fv = e1[5 * ix]; gv = e1[5 * ix + 1]; Hp = e1[5 * ix + 2]; Hm = e1[5 * ix + 3]; jv = e1[5 * ix + 4];
+    }
+    if (e0) {
+      fP = e0[5 * ix]; gP = e0[5 * ix + 1]; HpP = e0[5 * ix + 2]; HmP = e0[5 * ix + 3]; jP = e0[5 * ix + 4];
+    }
+    double delta = d1[ix] - jv, beta = b1[ix];
+    double deltaP = d0[ix] - jP, betaP = b0[ix];
     diag[ix] = fv * 1.0 + 0.25 * (Hp + Hm) * gv * I * rho + 0.5 * (beta * r + I * delta * r);
     if (ix < nx - 1) sub[ix] = -0.25 * I * rho * Hm * gv;
     orc_c left = ix > 0 ? prev[ix - 1] : 0.0;
@@ -639,13 +657,38 @@ void orc_cn_step_planes(double rho, double r, int bc, const orc_c *prev, orc_c *
   free(sub); free(rhs); free(diag);
 }
 
-void orc_cn_run(const orc_grid *g, const orc_material *m, int bc, orc_c *field, unsigned nx,
-                unsigned first_step, unsigned nsteps, orc_c *solution, unsigned nz,
-                unsigned absorb_npix, double absorb_inv)
+void orc_cn_step_planes(double rho, double r, int bc, const orc_c *prev, orc_c *cur, unsigned nx,
+                        const double *d1, const double *b1, const double *d0, const double *b0)
+{
+  orc_cn_step_eq(rho, r, bc, prev, cur, nx, d1, b1, d0, b0, NULL, NULL);
+}
+
+/* The equation values of one plane at z, 5 per row (orc_cn_step_eq). */
+static void eq_plane(const orc_equation *eq, const orc_grid *g, unsigned nx, double z, double *e)
+{
+  for (unsigned ix = 0; ix < nx; ++ix) {
+    double x = g->xmin + ix * g->dx, v[4], vp[4], vm[4];
+    eq->fn(eq->ctx, x, z, v);
+    eq->fn(eq->ctx, x + 0.5 * g->dx, z, vp);
+    eq->fn(eq->ctx, x - 0.5 * g->dx, z, vm);
+    e[5 * ix] = v[0];
+    e[5 * ix + 1] = v[1];
+    e[5 * ix + 2] = vp[2];
+    e[5 * ix + 3] = vm[2];
+    e[5 * ix + 4] = v[3];
+  }
+}
+
+void orc_cn_run_eq(const orc_grid *g, const orc_material *m, const orc_equation *eq, int bc,
+                   orc_c *field, unsigned nx, unsigned first_step, unsigned nsteps,
+                   orc_c *solution, unsigned nz, unsigned absorb_npix, double absorb_inv)
 {
   orc_c *cur = (orc_c *)malloc(sizeof(orc_c) * nx);
   double *d1 = (double *)malloc(sizeof(double) * nx * 4);
   double *b1 = d1 + nx, *d0 = d1 + 2 * nx, *b0 = d1 + 3 * nx;
+  const int custom = eq && eq->fn;
+  double *e1 = custom ? (double *)malloc(sizeof(double) * nx * 10) : NULL;
+  double *e0 = custom ? e1 + 5 * nx : NULL;
   const double rho = g->dz / (g->k * g->dx * g->dx);
   const double r = g->k * g->dz;
   for (unsigned st = first_step; st < first_step + nsteps; ++st) {
@@ -655,12 +698,23 @@ void orc_cn_run(const orc_grid *g, const orc_material *m, int bc, orc_c *field, 
       mat_eval(m, x, 0.0, z, &d1[ix], &b1[ix]);
       mat_eval(m, x, 0.0, z - g->dz, &d0[ix], &b0[ix]);
     }
-    orc_cn_step_planes(rho, r, bc, field, cur, nx, d1, b1, d0, b0);
+    if (custom) {
+      eq_plane(eq, g, nx, z, e1);
+      eq_plane(eq, g, nx, z - g->dz, e0);
+    }
+    orc_cn_step_eq(rho, r, bc, field, cur, nx, d1, b1, d0, b0, e1, e0);
     if (absorb_npix) orc_absorb_line(cur, nx, 1, absorb_npix, absorb_inv);
     memcpy(field, cur, sizeof(orc_c) * nx);
     if (solution && st < nz) memcpy(solution + (size_t)st * nx, cur, sizeof(orc_c) * nx);
   }
-  free(cur); free(d1);
+  free(cur); free(d1); free(e1);
+}
+
+void orc_cn_run(const orc_grid *g, const orc_material *m, int bc, orc_c *field, unsigned nx,
+                unsigned first_step, unsigned nsteps, orc_c *solution, unsigned nz,
+                unsigned absorb_npix, double absorb_inv)
+{
+  orc_cn_run_eq(g, m, NULL, bc, field, nx, first_step, nsteps, solution, nz, absorb_npix, absorb_inv);
 }
 
 /* ========================================================================== */

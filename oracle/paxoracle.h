@@ -169,6 +169,25 @@ void orc_cn_run(const orc_grid *g, const orc_material *m, int bc, orc_c *field, 
 void orc_cn_step_planes(double rho, double r, int bc, const orc_c *prev, orc_c *cur, unsigned nx,
                         const double *d1, const double *b1, const double *d0, const double *b0);
 
+/* ParaxialEquation (include/paraxialEquation.hpp:16-35): F, G, H, J at (x, z). */
+typedef void (*orc_eqfn)(const void *ctx, double x, double z, double out[4]);
+typedef struct {
+  orc_eqfn fn;     /* NULL: the default equation F = G = H = 1, J = 0 */
+  const void *ctx;
+} orc_equation;
+/* Coefficients c0 + cx x + cz z + cinv / x for F, G, H, J (rows of c[4][4]). */
+typedef struct { double c[4][4]; } orc_poly_eq;
+void orc_poly_eqfn(const void *ctx, double x, double z, double out[4]);
+/* Per-row equation values of one plane, 5 per row: F(x,z), G(x,z),
+ * H(x + dx/2, z), H(x - dx/2, z), J(x,z); NULL = defaults.  e1 at z, e0 at z - dz. */
+void orc_cn_step_eq(double rho, double r, int bc, const orc_c *prev, orc_c *cur, unsigned nx,
+                    const double *d1, const double *b1, const double *d0, const double *b0,
+                    const double *e1, const double *e0);
+/* orc_cn_run with a ParaxialEquation. */
+void orc_cn_run_eq(const orc_grid *g, const orc_material *m, const orc_equation *eq, int bc,
+                   orc_c *field, unsigned nx, unsigned first_step, unsigned nsteps,
+                   orc_c *solution, unsigned nz, unsigned absorb_npix, double absorb_inv);
+
 /* ---- ADI (src/alternatingDirectionSolver.cpp:8-234) ----------------------- */
 void orc_adi_run(const orc_grid *g, const orc_material *m, int use_tbc, unsigned quirks,
                  orc_c *field, unsigned n, unsigned first_step, unsigned nsteps,

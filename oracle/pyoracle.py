@@ -56,6 +56,25 @@ class Slab(C.Structure):
                 ("core_beta", C.c_double), ("clad_delta", C.c_double), ("clad_beta", C.c_double)]
 
 
+class Equation(C.Structure):
+    _fields_ = [("fn", C.c_void_p), ("ctx", C.c_void_p)]
+
+
+class PolyEq:
+    """A ParaxialEquation whose F, G, H, J are each c0 + cx x + cz z + cinv / x
+    (rows of `coef`, shape (4, 4)); evaluated by the oracle's orc_poly_eqfn."""
+
+    def __init__(self, coef):
+        self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(4, 4)
+        self.eq = Equation(C.cast(lib().orc_poly_eqfn, C.c_void_p), self.coef.ctypes.data_as(C.c_void_p))
+
+    def values(self, x, z):
+        """(F, G, H, J) at (x, z), as the oracle evaluates them."""
+        out = np.zeros(4)
+        lib().orc_poly_eqfn(self.coef.ctypes.data_as(C.c_void_p), x, z, _p(out))
+        return out
+
+
 class Absorber3D(C.Structure):
     _fields_ = [("npix_x", C.c_uint), ("npix_y", C.c_uint), ("inv_x", C.c_double),
                 ("inv_y", C.c_double)]
@@ -98,6 +117,9 @@ def lib():
                                  C.c_uint, C.c_double]
         L.orc_cn_step_planes.argtypes = [C.c_double, C.c_double, C.c_int, P, P, C.c_uint,
                                          P, P, P, P]
+        L.orc_cn_run_eq.argtypes = [P, P, P, C.c_int, P, C.c_uint, C.c_uint, C.c_uint, P, C.c_uint,
+                                    C.c_uint, C.c_double]
+        L.orc_poly_eqfn.argtypes = [P, C.c_double, C.c_double, P]
         L.orc_adi_run.argtypes = [P, P, C.c_int, C.c_uint, P, C.c_uint, C.c_uint, C.c_uint,
                                   P, C.c_uint, C.c_uint, C.c_uint]
         L.orc_gauss_2d_v.argtypes = [P, C.c_double, C.c_double, P]
@@ -248,11 +270,14 @@ def fft2d_run(g, material, field, first_step, nsteps, solution=None):
 
 
 def cn_run(g, material, field, first_step, nsteps, bc=BC_TRANSPARENT, solution=None,
-           absorb_npix=0, absorb_inv=0.0):
+           absorb_npix=0, absorb_inv=0.0, equation=None):
+    """equation: a PolyEq (any ParaxialEquation restated as orc_equation), or None
+    for the default F = G = H = 1, J = 0."""
     f = np.ascontiguousarray(field, dtype=np.complex128).copy()
     sp, nz = (None, 0) if solution is None else (_p(solution), solution.shape[1])
-    lib().orc_cn_run(C.byref(g), C.byref(material.mat), bc, _p(f), f.size, first_step, nsteps,
-                     sp, nz, absorb_npix, absorb_inv)
+    eq = None if equation is None else C.byref(equation.eq)
+    lib().orc_cn_run_eq(C.byref(g), C.byref(material.mat), eq, bc, _p(f), f.size, first_step, nsteps,
+                        sp, nz, absorb_npix, absorb_inv)
     return f
 
 

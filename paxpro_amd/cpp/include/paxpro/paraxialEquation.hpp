@@ -1,6 +1,7 @@
-// Coefficient hooks of the CN discretisation (include/paraxialEquation.hpp:16-35).
-// The GPU path implements the default equation (F = G = H = 1, J = 0); a
-// subclass with other coefficients is rejected when the plan is built.
+// Coefficient hooks of the CN discretisation (include/paraxialEquation.hpp:16-35):
+// F(x,z) dA/dz = a G(x,z) d/dx (H(x,z) dA/dx) - K A + i k J(x,z) A.  A subclass
+// with other coefficients is sampled on the host where CrankNicholson evaluates it
+// and handed to the device as coefficient planes (pxg_set_equation_planes).
 #pragma once
 
 #include <complex>

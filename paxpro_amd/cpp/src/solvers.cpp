@@ -318,8 +318,6 @@ void Solver2D::initValuesFromWaveGuide()
 void Solver2D::ensurePlan()
 {
   if (plan && !fieldDirty) return;
-  if (eq && typeid(*eq) != typeid(ParaxialEquation))
-    throw std::runtime_error("GPU Crank-Nicolson supports the default ParaxialEquation only");
   initValuesFromWaveGuide();
   if (!plan) {
     pxg_problem p{};
@@ -360,6 +358,26 @@ void Solver2D::runOnDevice(unsigned first, unsigned n)
                               db[(static_cast<size_t>(q) * Nx + i) * 2 + 1]);
     }
     check(pxg_set_material_planes(plan, db.data(), np, p0, 0));
+  }
+  if (gpuKind() == PXG_CN && eq && typeid(*eq) != typeid(ParaxialEquation)) {
+    // a ParaxialEquation of the user's: its coefficients sampled on the host where
+    // CrankNicholson::solveStep evaluates them (src/crancNicholson.cpp:30-60):
+    // F, G, J at x and H at x +- dx/2, on the planes z of steps first-1 .. first+n-1
+    const unsigned p0 = first - 1, np = n + 1;
+    std::vector<double> co(static_cast<size_t>(np) * 5 * Nx);
+    for (unsigned q = 0; q < np; ++q) {
+      const double z = guide->getZ(p0 + q);
+      for (unsigned i = 0; i < Nx; ++i) {
+        const double x = guide->getX(i);
+        double *c = &co[(static_cast<size_t>(q) * Nx + i) * 5];
+        c[0] = eq->F(x, z);
+        c[1] = eq->G(x, z);
+        c[2] = eq->H(x + 0.5 * stepX, z);
+        c[3] = eq->H(x - 0.5 * stepX, z);
+        c[4] = eq->J(x, z);
+      }
+    }
+    check(pxg_set_equation_planes(plan, co.data(), np, p0, 0));
   }
   check(pxg_step(plan, first, n));
   check(pxg_get_field(plan, 0, reinterpret_cast<double *>(currentSolution->memptr())));

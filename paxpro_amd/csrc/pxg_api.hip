@@ -136,6 +136,9 @@ struct pxg_plan_s {
   // FFT kinds with nx not a power of two in [16, 4096]: Bluestein length and tables
   int blue_m = 0;
   DBuf chirp, blue_bf, blue_bi;
+  // CN with a ParaxialEquation other than the default (pxg_set_equation_planes)
+  DBuf eq_planes;
+  int eq_n = 0, eq_first = 0, eq_inv = 0;
   // far field: twiddle tables of the last signal length and transform length used
   DBuf ff_twm, ff_twn;
   unsigned ff_twm_len = 0, ff_twn_len = 0;
@@ -711,6 +714,9 @@ void build_cn_jobs(pxg_plan P)
     J.dz = q.dz;
     J.k = P->k_host[b];
     J.mat = shared;
+    J.eq = P->eq_planes.p ? P->eq_planes.as<double>() : nullptr;
+    J.eq_first = P->eq_first;
+    J.eq_inv = P->eq_inv;
     if (P->is_jobs && P->job_mat[b] == MAT_CURVED) {
       J.mat.mode = MAT_CURVED;
       J.mat.wg = P->job_wg[b];
@@ -728,7 +734,11 @@ void build_cn_jobs(pxg_plan P)
   std::vector<CnJob> reg, big, glob;
   P->max_P = P->max_P_reg = P->max_P_big = 1;
   for (const CnJob &J : jobs) {
-    if (J.m <= cn_reg_rows()) {
+    if (J.eq) {
+      // a ParaxialEquation other than the default: the general march builds every row
+      glob.push_back(J);
+      P->max_P = std::max(P->max_P, J.P);
+    } else if (J.m <= cn_reg_rows()) {
       reg.push_back(J);
       P->max_P_reg = std::max(P->max_P_reg, J.P);
     } else if (J.m > 8 && J.m <= cn_big_rows()) {
@@ -1300,6 +1310,8 @@ int pxg_plan_reset(pxg_plan P, const pxg_problem *prob)
     if (P->p.n_refr == 0) P->p.n_refr = 10;
     P->k_host.assign(prob->batch, prob->k);
     P->mat_mode = MAT_NONE;
+    P->eq_planes.release();
+    P->eq_n = P->eq_first = P->eq_inv = 0;
     P->map_step = -1;
     P->field_set = false;
     P->jobs_dirty = true;
@@ -1475,6 +1487,28 @@ int pxg_get_solution(pxg_plan P, int b, double *c128, size_t n_complex)
         c128[2 * i + 1] = tmp[i].y;
       }
     }
+  });
+}
+
+int pxg_set_equation_planes(pxg_plan P, const double *coef, int n_planes, int first_plane, int z_invariant)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (P->p.kind != PXG_CN || P->is_jobs)
+      fail(PXG_ERR_UNSUPPORTED, "a ParaxialEquation applies to single CN plans (CrankNicholson)");
+    if (!coef) {
+      P->eq_planes.release();
+      P->eq_n = P->eq_first = P->eq_inv = 0;
+    } else {
+      if (n_planes < 1) fail(PXG_ERR_INVALID, "need at least one plane");
+      const size_t per = 5 * static_cast<size_t>(P->p.nx);
+      P->eq_planes.alloc(per * n_planes * sizeof(double));
+      hip_check(hipMemcpy(P->eq_planes.p, coef, P->eq_planes.bytes, hipMemcpyHostToDevice), "equation planes");
+      P->eq_n = n_planes;
+      P->eq_first = first_plane;
+      P->eq_inv = z_invariant != 0;
+    }
+    P->jobs_dirty = true;
   });
 }
 
@@ -1667,6 +1701,11 @@ int pxg_step(pxg_plan P, unsigned first_step, unsigned nsteps)
         fail(PXG_ERR_INVALID, "material planes [%d, %d) do not cover steps %u..%u", P->first_plane,
              P->first_plane + P->n_planes, first_step, first_step + nsteps - 1);
     }
+    if (P->eq_planes.p && !P->eq_inv &&
+        (static_cast<long>(first_step) - 1 < P->eq_first ||
+         static_cast<long>(first_step + nsteps) - 1 >= static_cast<long>(P->eq_first) + P->eq_n))
+      fail(PXG_ERR_INVALID, "equation planes [%d, %d) do not cover steps %u..%u", P->eq_first,
+           P->eq_first + P->eq_n, first_step, first_step + nsteps - 1);
     P->profiling = false;
     run(P, first_step, nsteps);
   });

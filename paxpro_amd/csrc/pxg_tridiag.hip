@@ -122,6 +122,45 @@ __device__ __forceinline__ void cn_row_m(int N, int bc, int i, double rho, doubl
   }
 }
 
+// Row i with a ParaxialEquation given as host-sampled planes (CnJob.eq): the
+// terms of src/crancNicholson.cpp:41-88 in the reference's order, e1 the values
+// at z, e0 at z - dz.  The off-diagonals are the reference's symmetric sub
+// diagonal, sub[i] = -i rho/4 H-(i) G(i) coupling rows i and i+1; the boundary
+// rows take H-, G (and H-', G') of row N-1, the last row of the reference's
+// loop (:146-147).
+__device__ __forceinline__ void cn_row_eq(int N, int bc, int i, double rho, double r, double d1, double b1,
+                                          double d0, double b0, const double *e1, const double *e0,
+                                          cx<double> ul, cx<double> u, cx<double> ur, cx<double> uin,
+                                          cx<double> &ai, cx<double> &bi, cx<double> &ci, cx<double> &di)
+{
+  const double *q = e1 + 5 * i, *qp = e0 + 5 * i;
+  const double fv = q[0], gv = q[1], hp = q[2], hm = q[3], jv = q[4];
+  const double fp = qp[0], gp = qp[1], hpp = qp[2], hmp = qp[3], jp = qp[4];
+  const double delta = d1 - jv, deltaP = d0 - jp;
+  auto sub = [&](int row) { return cx<double>{0.0, ((-0.25 * rho) * e1[5 * row + 3]) * e1[5 * row + 1]}; };
+  ai = i > 0 ? sub(i - 1) : cx<double>{0.0, 0.0};
+  ci = i < N - 1 ? sub(i) : cx<double>{0.0, 0.0};
+  // diag = F + 0.25 (H+ + H-) G i rho + 0.5 (beta r + i delta r)
+  bi = cx<double>{fv * 1.0, 0.0} + cx<double>{0.0, 0.25 * (hp + hm) * gv * rho} +
+       cx<double>{0.5 * (b1 * r), 0.5 * (delta * r)};
+  // rhs = (u_{i-1} H-' G' + u_{i+1} H+' G') i rho/4 - u i rho/4 (H+' + H-') G
+  //       + (F' - 0.5 (beta' r + i delta' r)) u
+  cx<double> v = scale(scale(ul, hmp), gp);
+  v = v + scale(scale(ur, hpp), gp);
+  v = v * cx<double>{0.0, 0.25 * rho};
+  v = v - u * cx<double>{0.0, 0.25 * rho * (hpp + hmp) * gv};
+  v = v + cx<double>{1.0 * fp - 0.5 * (b0 * r), -0.5 * (deltaP * r)} * u;
+  di = v;
+  if (bc == PXG_BC_TRANSPARENT && N >= 2 && (i == 0 || i == N - 1)) {
+    cx<double> e;
+    if (tbc_factor(u, uin, 1e-16, e)) {
+      const double *ql = e1 + 5 * (N - 1), *qlp = e0 + 5 * (N - 1);
+      bi = bi - cx<double>{0.0, 0.25 * rho * ql[3] * ql[1]} * e;
+      di = di + cx<double>{0.0, 0.25 * rho * qlp[3] * qlp[1]} * e * u;
+    }
+  }
+}
+
 __device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, double z, double rho, double r,
                                          const uint64_t *mask, int nw, cx<double> ul, cx<double> u,
                                          cx<double> ur, cx<double> uin, cx<double> &ai, cx<double> &bi,
@@ -131,6 +170,14 @@ __device__ __forceinline__ void cn_row_v(const CnJob &J, int i, unsigned st, dou
   double d1, b1, d0, b0;
   material_2d(J.mat, mask, nw, st, i, x, z, d1, b1);
   material_2d(J.mat, mask, nw, st - 1, i, x, z - J.dz, d0, b0);
+  if (J.eq) {
+    const size_t stride = 5 * static_cast<size_t>(J.nx);
+    const int p1 = J.eq_inv ? 0 : static_cast<int>(st) - J.eq_first;
+    const int p0 = J.eq_inv ? 0 : static_cast<int>(st) - 1 - J.eq_first;
+    cn_row_eq(J.nx, J.bc, i, rho, r, d1, b1, d0, b0, J.eq + p1 * stride, J.eq + p0 * stride, ul, u, ur, uin, ai,
+              bi, ci, di);
+    return;
+  }
   cn_row_m(J.nx, J.bc, i, rho, r, d1, b1, d0, b0, ul, u, ur, uin, ai, bi, ci, di);
 }
 

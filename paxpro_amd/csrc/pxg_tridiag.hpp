@@ -64,6 +64,11 @@ struct CnJob {
   double xmin, dx, zmin, dz, k;
   double absorb_inv;
   MatDev mat;
+  // a ParaxialEquation other than the default: per plane 5 values per row,
+  // F(x,z), G(x,z), H(x + dx/2, z), H(x - dx/2, z), J(x,z), plane p at
+  // z = zmin + p dz (eq_inv: one plane for every z); null for the default
+  const double *eq;
+  int eq_first, eq_inv;
 };
 // Scratch bytes of one CN job of nx rows (256-byte aligned blocks).
 size_t cn_job_scratch(int nx, int esz);

@@ -41,7 +41,7 @@ EXPORTS = [
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
     "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
-    "pxg_thomas_batched_noalloc",
+    "pxg_thomas_batched_noalloc", "pxg_set_equation_planes",
 ]
 THOMAS_NOALLOC_MAX_ROWS = 1 << 20
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
@@ -124,6 +124,7 @@ def lib():
         L.pxg_set_material_curved_wg.argtypes = [P, P]
         L.pxg_set_material_planes.argtypes = [P, P, I, I, I]
         L.pxg_set_material_none.argtypes = [P]
+        L.pxg_set_equation_planes.argtypes = [P, P, I, I, I]
         L.pxg_step.argtypes = [P, U, U]
         L.pxg_sync.argtypes = [P]
         L.pxg_step_profiled.argtypes = [P, U, U, P, I, P, P]
@@ -325,6 +326,15 @@ class Plan:
                                           padding, phi_min, phi_max, export_rows, export_cols,
                                           _p(out)))
         return out
+
+    def equation_planes(self, coef, first_plane=0, z_invariant=False):
+        """ParaxialEquation planes (n_planes, nx, 5): F, G, H(x+dx/2), H(x-dx/2), J; None
+        restores the default equation."""
+        if coef is None:
+            _check(lib().pxg_set_equation_planes(self.h, None, 0, 0, 0))
+            return
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        _check(lib().pxg_set_equation_planes(self.h, _p(c), c.shape[0], first_plane, int(z_invariant)))
 
     def material_none(self):
         _check(lib().pxg_set_material_none(self.h))

@@ -251,6 +251,16 @@ static int run3d(const std::string &mode, int n, int nsteps, const char *out, in
   return 0;
 }
 
+// A ParaxialEquation with coefficients varying in x and z (the same polynomials as
+// the GPU test of the equation planes): F = 1 + 1e-4 x + 1e-7 z, G = 1 + 2e-4 x,
+// H = 1 - 1e-4 x + 2e-8 z, J = 3e-6 + 1e-9 x.
+struct PolyEquation : ParaxialEquation {
+  double F(double x, double z) const override { return 1.0 + 1e-4 * x + 1e-7 * z; }
+  double G(double x, double z) const override { return 1.0 + 2e-4 * x + 0.0 * z; }
+  double H(double x, double z) const override { return 1.0 + -1e-4 * x + 2e-8 * z; }
+  double J(double x, double z) const override { return 3e-6 + 1e-9 * x + 0.0 * z; }
+};
+
 static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
 {
   const double dz = 500.0;
@@ -279,7 +289,11 @@ static int run2d(const std::string &mode, int nx, int nsteps, const char *out)
   FFTSolver2D f2;
   Solver2D *solver = mode == "fft2d" ? static_cast<Solver2D *>(&f2) : &cn;
   if (mode != "fft2d") cn.setBoundaryCondition(Solver2D::BC_t::TRANSPARENT);
+  PolyEquation poly;
   sim.setSolver(*solver);
+  // after setSolver: Solver2D::setSimulator installs a default equation
+  // (src/solver2D.cpp:41), so the reference's users set theirs afterwards
+  if (mode == "cn_eq") cn.setEquation(poly);
   sim.setBoundaryConditions(pw);
   if (mode == "cn_step") {
     for (int i = 1; i <= nsteps; ++i) sim.step();

@@ -141,7 +141,8 @@ def test_dropin_adi(tmp_path, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode,nx", [("cn", 1024), ("cn_step", 300), ("cn_curved", 777), ("fft2d", 512)])
+@pytest.mark.parametrize("mode,nx", [("cn", 1024), ("cn_step", 300), ("cn_curved", 777), ("fft2d", 512),
+                                     ("cn_eq", 300), ("fft2d", 500)])
 def test_dropin_2d(tmp_path, mode, nx):
     nsteps, dz = 40, 500.0
     out = run(tmp_path, mode, nx, nsteps)
@@ -163,7 +164,13 @@ def test_dropin_2d(tmp_path, mode, nx):
     if mode == "fft2d":
         ref = O.fft2d_run(g, mat, u0, 1, nsteps, solution=sol)
     else:
-        ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol)
+        # cn_eq: CrankNicholson::setEquation with a ParaxialEquation subclass whose
+        # coefficients vary in x and z (the driver's PolyEquation)
+        eq = O.PolyEq([[1.0, 1e-4, 1e-7, 0.0], [1.0, 2e-4, 0.0, 0.0], [1.0, -1e-4, 2e-8, 0.0],
+                       [3e-6, 1e-9, 0.0, 0.0]]) if mode == "cn_eq" else None
+        ref = O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT, solution=sol, equation=eq)
+        if eq is not None:
+            assert rel_l2(O.cn_run(g, mat, u0, 1, nsteps, bc=O.BC_TRANSPARENT), ref) > 1e-4
     assert rel_l2(load(out, (nx,)), ref) < 1e-10
     assert rel_l2(load(out + ".sol", sol.shape), sol) < 1e-10
     if mode == "cn":

@@ -319,3 +319,58 @@ def test_cn_mixed_long_curved_guide_within_bound(ctx):
     worst = max(rel_l2(m, r) for m, r in zip(out[1], out[0]))
     announce("cn mixed 1000 steps worst rel_l2", worst)
     assert worst < 1e-5
+
+
+POLY_EQ = [[1.0, 1e-4, 1e-7, 0.0],      # F = 1 + 1e-4 x + 1e-7 z
+           [1.0, 2e-4, 0.0, 0.0],       # G
+           [1.0, -1e-4, 2e-8, 0.0],     # H
+           [3e-6, 1e-9, 0.0, 0.0]]      # J
+
+
+def _eq_planes(eqo, xmin, dx, nx, zmin, dz, planes):
+    """Host samples of a ParaxialEquation where CrankNicholson evaluates it: F, G, J
+    at x, H at x +- dx/2, one plane per z = zmin + p dz."""
+    out = np.empty((len(planes), nx, 5))
+    for q, p in enumerate(planes):
+        z = zmin + p * dz
+        for i in range(nx):
+            x = xmin + i * dx
+            F, G, _, J = eqo.values(x, z)
+            out[q, i] = (F, G, eqo.values(x + 0.5 * dx, z)[2], eqo.values(x - 0.5 * dx, z)[2], J)
+    return out
+
+
+@pytest.mark.parametrize("nx,nsteps,bc", [(300, 40, P.BC_TRANSPARENT), (1024, 25, P.BC_TRANSPARENT),
+                                          (77, 30, P.BC_DIRICHLET)])
+def test_cn_custom_equation_vs_oracle(ctx, nx, nsteps, bc):
+    """A ParaxialEquation with F, G, H, J varying in x and z (src/crancNicholson.cpp:41-88),
+    host-sampled into coefficient planes, against the oracle evaluating the same
+    equation per row; then the default equation restored."""
+    announce("cn custom equation", nx, nsteps, bc)
+    k, dz = K_01, 500.0
+    xmin, dx = -500.0, 1000.0 / (nx - 1)
+    g = O.grid(xmin, xmin + (nx - 1) * dx, dx, 0.0, 1e6, dz, k)
+    x = xmin + dx * np.arange(nx)
+    planes = slab_planes(x, -50.0, 50.0)
+    u0 = (1.0 + 0.2 * np.exp(-(x / 120.0) ** 2)).astype(complex)
+    eqo = O.PolyEq(POLY_EQ)
+    ref = O.cn_run(g, O.SlabMaterial(-50, 50, 0.0, 0.0, 4.5e-5, 3.5e-6), u0, 1, nsteps, bc=bc, equation=eqo)
+    plan = P.Plan(ctx, P.problem(P.CN, nx, nsteps + 1, dx, dz, k, xmin=xmin, bc=bc))
+    plan.material_planes(planes, z_invariant=True)
+    plan.equation_planes(_eq_planes(eqo, xmin, dx, nx, 0.0, dz, range(nsteps + 1)))
+    plan.set_field(u0)
+    plan.step(1, nsteps)
+    err = rel_l2(plan.get_field(), ref)
+    announce("rel_l2", err)
+    assert err < 1e-10
+    default = O.cn_run(g, O.SlabMaterial(-50, 50, 0.0, 0.0, 4.5e-5, 3.5e-6), u0, 1, nsteps, bc=bc)
+    assert rel_l2(default, ref) > 1e-4   # the equation changes the result
+    plan.equation_planes(None)
+    plan.set_field(u0)
+    plan.step(1, nsteps)
+    assert rel_l2(plan.get_field(), default) < 1e-10
+    with pytest.raises(P.PxgError) as e:   # planes 0..3 do not cover step 5
+        plan.equation_planes(_eq_planes(eqo, xmin, dx, nx, 0.0, dz, range(4)))
+        plan.step(5, 1)
+    assert e.value.code == P.ERR_INVALID
+    plan.close()

@@ -223,3 +223,81 @@ def test_adi_gaussian_beam_width_anisotropic():
     assert abs(_width(y, inten.sum(axis=1)) / (wy * np.sqrt(1 + (z / zry) ** 2)) - 1) < 5e-3
     exact = np.outer(_beam_1d(y, wy, z), _beam_1d(x, wx, z))
     assert O.rel_l2(u, exact) < 2e-2
+
+
+# ------------------------------------------------------------- ParaxialEquation
+DEFAULT_EQ = [[1, 0, 0, 0], [1, 0, 0, 0], [1, 0, 0, 0], [0, 0, 0, 0]]
+
+
+def _cn_grid(n, dz):
+    dx = 1000 / (n - 1)
+    return O.grid(-500, 500, dx, 0, 1e6, dz, K), dx
+
+
+def test_cn_equation_default_coefficients_are_the_default_path():
+    """F = G = H = 1, J = 0 given explicitly reproduces the default equation bit
+    for bit (slab material, transparent boundary)."""
+    n = 64
+    g, _ = _cn_grid(n, 500.0)
+    mat = O.SlabMaterial(-100, 100, 0.0, 0.0, 4.5e-5, 3.5e-6)
+    u0 = np.exp(-np.linspace(-3, 3, n) ** 2).astype(complex)
+    a = O.cn_run(g, mat, u0, 1, 7)
+    b = O.cn_run(g, mat, u0, 1, 7, equation=O.PolyEq(DEFAULT_EQ))
+    np.testing.assert_array_equal(a, b)
+
+
+def test_cn_equation_j_shifts_delta():
+    """J enters as delta - J (src/crancNicholson.cpp:45-46): a constant J in a
+    medium of decrement d is the default equation in a medium of d - J, bit for bit."""
+    n, d, b, j = 48, 4.5e-5, 3.5e-6, 1.75e-5
+    g, _ = _cn_grid(n, 500.0)
+    u0 = np.exp(-np.linspace(-2, 2, n) ** 2).astype(complex)
+    eq = O.PolyEq([[1, 0, 0, 0], [1, 0, 0, 0], [1, 0, 0, 0], [j, 0, 0, 0]])
+    a = O.cn_run(g, O.SlabMaterial(-1e9, 1e9, d, b, 0, 0), u0, 1, 5, equation=eq)
+    ref = O.cn_run(g, O.SlabMaterial(-1e9, 1e9, d - j, b, 0, 0), u0, 1, 5)
+    np.testing.assert_array_equal(a, ref)
+    assert O.rel_l2(a, O.cn_run(g, O.SlabMaterial(-1e9, 1e9, d, b, 0, 0), u0, 1, 5)) > 1e-3
+
+
+@pytest.mark.parametrize("m", [1, 5, 20])
+def test_cn_equation_constant_coefficients_sine_modes(m):
+    """Constant F = c, G = g, H = h, vacuum, Dirichlet: the sine modes stay
+    eigenvectors and each step multiplies mode m by (c - a)/(c + a),
+    a = (i rho g h/2)(1 - cos t): F scales the z derivative, G H the Laplacian."""
+    n, dz, s = 47, 500.0, 4
+    c, gg, h = 1.7, 0.6, 1.9
+    g, dx = _cn_grid(n, dz)
+    t = np.pi * m / (n + 1)
+    u0 = np.sin(t * (np.arange(n) + 1)).astype(complex)
+    eq = O.PolyEq([[c, 0, 0, 0], [gg, 0, 0, 0], [h, 0, 0, 0], [0, 0, 0, 0]])
+    u = O.cn_run(g, O.NoMaterial, u0, 1, s, bc=O.BC_DIRICHLET, equation=eq)
+    rho = dz / (K * dx * dx)
+    a = 0.5j * rho * gg * h * (1 - np.cos(t))
+    np.testing.assert_allclose(u, ((c - a) / (c + a)) ** s * u0, rtol=0, atol=1e-12)
+    # and not the default equation's factor
+    a0 = 0.5j * rho * (1 - np.cos(t))
+    assert abs(((c - a) / (c + a)) ** s - ((1 - a0) / (1 + a0)) ** s) > 1e-5   # seven orders above the tolerance
+
+
+@pytest.mark.parametrize("m", [2, 9])
+def test_cn_equation_z_dependent_coefficients_sine_modes(m):
+    """F and G varying with z, H = h, vacuum, Dirichlet: step s (z = s dz, primes at
+    z - dz) multiplies mode m by
+      (F' + (i rho h/2)(G' cos t - G)) / (F + (i rho h/2) G (1 - cos t)),
+    with G, not G', in the centre term of the right-hand side, as the reference
+    writes it (src/crancNicholson.cpp:87)."""
+    n, dz, s = 31, 500.0, 5
+    f0, fz, g0, gz, h = 1.3, 2e-4, 0.8, -1.5e-4, 1.2
+    g, dx = _cn_grid(n, dz)
+    t = np.pi * m / (n + 1)
+    u0 = np.sin(t * (np.arange(n) + 1)).astype(complex)
+    eq = O.PolyEq([[f0, 0, fz, 0], [g0, 0, gz, 0], [h, 0, 0, 0], [0, 0, 0, 0]])
+    u = O.cn_run(g, O.NoMaterial, u0, 1, s, bc=O.BC_DIRICHLET, equation=eq)
+    rho = dz / (K * dx * dx)
+    gain = 1.0 + 0j
+    for st in range(1, s + 1):
+        z = st * dz
+        F, G = f0 + fz * z, g0 + gz * z
+        Fp, Gp = f0 + fz * (z - dz), g0 + gz * (z - dz)
+        gain *= (Fp + 0.5j * rho * h * (Gp * np.cos(t) - G)) / (F + 0.5j * rho * h * G * (1 - np.cos(t)))
+    np.testing.assert_allclose(u, gain * u0, rtol=0, atol=1e-12)
