@@ -140,7 +140,7 @@ def test_fft3d_vacuum_and_unsupported(ctx):
     _, (out,) = _run_gpu(ctx, fft3d_problem(g, n, 5), u0, 4)
     assert rel_l2(out, ref) < 1e-12
     with pytest.raises(P.PxgError) as e:
-        P.Plan(ctx, P.problem(P.FFT3D, 48, 5, 1.0, 1.0, 1.0))
+        P.Plan(ctx, P.problem(P.FFT3D, 4100, 5, 1.0, 1.0, 1.0))   # beyond 4096 per side
     assert e.value.code == P.ERR_UNSUPPORTED
     plan = P.Plan(ctx, fft3d_problem(g, n, 5))
     with pytest.raises(P.PxgError) as e:

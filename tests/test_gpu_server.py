@@ -102,11 +102,11 @@ def test_server_failures_are_isolated(ctx):
     srv = P.Server(ctx, slots=2, budget=small_budget, chunk_steps=2)
     big = P.problem(P.FFT3D, 4096, 3, 1.0, 1.0, K_01, batch=1)   # 4096^2 c128 > budget
     u_big = np.zeros((4096, 4096), complex, order="F")
-    bad = P.problem(P.FFT3D, 48, 3, 1.0, 1.0, K_01)               # not a power of two
+    bad = P.problem(P.FFT3D, 4100, 3, 1.0, 1.0, K_01)             # beyond 4096 points per side
     ids = {}
     ids["ok0"] = srv.submit(jobs[0]["prob"], jobs[0]["field"], jobs[0]["nsteps"], csg=jobs[0]["csg"])
     ids["big"] = srv.submit(big, u_big, 2)
-    ids["bad"] = srv.submit(bad, np.zeros((48, 48), complex), 2)
+    ids["bad"] = srv.submit(bad, np.zeros((2, 2), complex), 2)   # rejected before its field is read
     ids["inj"] = srv.submit(jobs[1]["prob"], jobs[1]["field"], jobs[1]["nsteps"], csg=jobs[1]["csg"], inject=1)
     ids["cancel"] = srv.submit(jobs[2]["prob"], jobs[2]["field"], 2000, csg=jobs[2]["csg"])
     ids["ok2"] = srv.submit(jobs[2]["prob"], jobs[2]["field"], jobs[2]["nsteps"], csg=jobs[2]["csg"])
