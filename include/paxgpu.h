@@ -220,6 +220,17 @@ int pxg_set_equation_planes(pxg_plan plan, const double *coef, int n_planes, int
  * first_step .. first_step+nsteps-1, storing solution slices as configured.
  * Enqueued on the plan's stream; returns after enqueue (use pxg_sync). */
 int pxg_step(pxg_plan plan, unsigned first_step, unsigned nsteps);
+/* Several separate FFT3D plans advanced in shared launches: plan i through steps
+ * first_steps[i] .. first_steps[i]+nsteps-1, one launch per pass and z-step covering
+ * every plan, each plan with its own grid, wavenumber, material and absorber.  The
+ * plans share a context, a power-of-two nx and the precision, and have batch 1 and no
+ * solution storage; at most 8.  Each plan's field ends as pxg_step(plans[i],
+ * first_steps[i], nsteps) leaves it (the same kernels per cell).  Enqueued on
+ * plans[0]'s stream after every plan's earlier work; every plan's stream waits for
+ * it.
+ * No counterpart in the reference, which runs one simulation per object
+ * (src/paraxialSimulation.cpp:40-48): for servers that batch separate requests. */
+int pxg_step_group(pxg_plan *plans, int n_plans, const unsigned *first_steps, unsigned nsteps);
 int pxg_sync(pxg_plan plan);
 /* Same as pxg_step, bracketing every kernel launch with HIP events on the plan's
  * stream; reports the mean duration per kernel name (ms) and the wall time of

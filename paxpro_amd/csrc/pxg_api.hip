@@ -192,6 +192,9 @@ struct pxg_plan_s {
   // host-side copy / precision conversion of the other
   void *stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // group stepping (pxg_step_group), batch-1 FFT3D plans: the event after this
+  // plan's earlier work, and the event after a group's launches that it led
+  hipEvent_t ready_ev = nullptr, group_done_ev = nullptr;
   struct Rec {
     const char *name;
     int e0, e1;
@@ -578,6 +581,51 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
       }
     }
   }
+}
+
+// Steps [first[i], first[i] + nsteps) of batch-1 FFT3D plans pl[0..m) of one
+// power-of-two line length and precision: the steps of run_fft3d (forward
+// FFT_y, then per step the material map, pass X and pass Y, the last pass Y
+// leaving real space), each launch covering every member.  pl[0]'s stream waits
+// for every member's earlier work and runs the launches; every member's stream
+// then waits for them.
+template <class R>
+void run_fft3d_group(pxg_plan *pl, int m, const unsigned *first, unsigned nsteps)
+{
+  pxg_plan L = pl[0];
+  const int n = L->p.nx;
+  hipStream_t s = L->stream;
+  GroupArgs<R> g{};
+  bool any_csg = false;
+  for (int i = 0; i < m; ++i) {
+    pxg_plan P = pl[i];
+    if (i > 0) {
+      hip_check(hipEventRecord(P->ready_ev, P->stream), "group join");
+      hip_check(hipStreamWaitEvent(s, P->ready_ev, 0), "group join");
+    }
+    g.m[i] = fft_args<R>(P);
+    g.m[i].map_in_pass_x = 0;  // batch 1: the map has its own launch
+    g.m[i].step = first[i];
+    g.m[i].map_prev_valid = P->map_step >= 0 && P->map_step + 1 == static_cast<long>(first[i]);
+    any_csg |= P->mat_mode == MAT_CSG;
+    P->map_step = -1;  // until the run's maps are launched
+  }
+  g.mode = Y_FWD;
+  hip_check(launch_pass_y_group<R>(n, L->lines_y, g, m, s), "pass y");
+  for (unsigned k = 0; k < nsteps; ++k) {
+    for (int i = 0; i < m; ++i) {
+      g.m[i].step = first[i] + k;
+      if (k > 0) g.m[i].map_prev_valid = 1;  // the previous step's map is in mcode
+    }
+    if (any_csg) hip_check(launch_material_map_group<R>(n, g, m, s), "material map");
+    hip_check(launch_pass_x_group<R>(n, L->lines_x, g, m, s), "pass x");
+    g.mode = Y_INV | (k + 1 == nsteps ? 0 : Y_FWD);
+    hip_check(launch_pass_y_group<R>(n, L->lines_y, g, m, s), "pass y");
+  }
+  for (int i = 0; i < m; ++i)
+    if (pl[i]->mat_mode == MAT_CSG) pl[i]->map_step = static_cast<long>(first[i] + nsteps - 1);
+  hip_check(hipEventRecord(L->group_done_ev, s), "group done");
+  for (int i = 1; i < m; ++i) hip_check(hipStreamWaitEvent(pl[i]->stream, L->group_done_ev, 0), "group done");
 }
 
 // Lines per work-group the FFT3D passes are built for at size n: powers of two
@@ -1183,6 +1231,10 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
       while (P->lines_y > 1 && (prob->nx / P->lines_y) * prob->batch < 512) P->lines_y /= 2;
       while (P->lines_x > 4 && (prob->nx / P->lines_x) * prob->batch < 512) P->lines_x /= 2;
     }
+    if (prob->kind == PXG_FFT3D && prob->batch == 1) {
+      hip_check(hipEventCreateWithFlags(&P->ready_ev, hipEventDisableTiming), "event");
+      hip_check(hipEventCreateWithFlags(&P->group_done_ev, hipEventDisableTiming), "event");
+    }
     if (prob->store_dsz > 0) {
       if (is3d) {
         const int cn = prob->nx / prob->store_dsx;
@@ -1365,7 +1417,7 @@ int pxg_plan_destroy(pxg_plan P)
         (void)hipStreamSynchronize(x);
         (void)hipStreamDestroy(x);
       }
-    for (hipEvent_t e : {P->fork_ev, P->join_ev, P->join2_ev})
+    for (hipEvent_t e : {P->fork_ev, P->join_ev, P->join2_ev, P->ready_ev, P->group_done_ev})
       if (e) (void)hipEventDestroy(e);
     for (auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
     if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -1688,10 +1740,9 @@ int pxg_plan_autotune(pxg_plan P, int reps)
   });
 }
 
-int pxg_step(pxg_plan P, unsigned first_step, unsigned nsteps)
+// Host-sampled material and equation planes cover the steps of a call.
+static void check_planes_cover(pxg_plan P, unsigned first_step, unsigned nsteps)
 {
-  return guarded([&] {
-    check_plan(P);
     if (P->mat_mode == MAT_PLANES && !P->z_invariant) {
       const pxg_problem &p = P->p;
       const int per_step = p.kind == PXG_ADI ? 2 : 1;
@@ -1706,8 +1757,47 @@ int pxg_step(pxg_plan P, unsigned first_step, unsigned nsteps)
          static_cast<long>(first_step + nsteps) - 1 >= static_cast<long>(P->eq_first) + P->eq_n))
       fail(PXG_ERR_INVALID, "equation planes [%d, %d) do not cover steps %u..%u", P->eq_first,
            P->eq_first + P->eq_n, first_step, first_step + nsteps - 1);
+}
+
+int pxg_step(pxg_plan P, unsigned first_step, unsigned nsteps)
+{
+  return guarded([&] {
+    check_plan(P);
+    check_planes_cover(P, first_step, nsteps);
     P->profiling = false;
     run(P, first_step, nsteps);
+  });
+}
+
+int pxg_step_group(pxg_plan *plans, int n_plans, const unsigned *first_steps, unsigned nsteps)
+{
+  return guarded([&] {
+    if (!plans || !first_steps) fail(PXG_ERR_INVALID, "null argument");
+    if (n_plans < 1 || n_plans > MAX_GROUP)
+      fail(PXG_ERR_INVALID, "a group holds 1 to %d plans (got %d)", MAX_GROUP, n_plans);
+    const pxg_plan L = plans[0];
+    check_plan(L);
+    for (int i = 0; i < n_plans; ++i) {
+      const pxg_plan P = plans[i];
+      check_plan(P);
+      const pxg_problem &p = P->p;
+      if (p.kind != PXG_FFT3D || p.batch != 1 || p.store_dsz != 0 || P->is_jobs)
+        fail(PXG_ERR_UNSUPPORTED, "group member %d: FFT3D plans with batch 1 and no solution storage only", i);
+      if (!fft_size_supported(p.nx))
+        fail(PXG_ERR_UNSUPPORTED, "group member %d: power-of-two line lengths only (nx = %d)", i, p.nx);
+      if (P->ctx != L->ctx || p.nx != L->p.nx || P->mixed != L->mixed)
+        fail(PXG_ERR_INVALID, "group member %d: context, nx or precision differs from member 0", i);
+      for (int j = 0; j < i; ++j)
+        if (plans[j] == P) fail(PXG_ERR_INVALID, "plan given twice (members %d and %d)", j, i);
+      if (!P->field_set) fail(PXG_ERR_STATE, "group member %d: initial conditions not set", i);
+      if (first_steps[i] < 1) fail(PXG_ERR_INVALID, "steps start at 1 (member %d)", i);
+      check_planes_cover(P, first_steps[i], nsteps);
+    }
+    if (nsteps == 0) return;
+    if (L->mixed)
+      run_fft3d_group<float>(plans, n_plans, first_steps, nsteps);
+    else
+      run_fft3d_group<double>(plans, n_plans, first_steps, nsteps);
   });
 }
 

@@ -82,16 +82,15 @@ __device__ __forceinline__ void store8(const cd (&v)[8], cx<R> *f, size_t stride
 
 // ---------------------------------------------------------------------------
 // Pass Y: contiguous y-lines (matrix columns); lane t of line `line` holds rows
-// t + m*N/8.  L lines (columns) per work-group, blockIdx.y = field.
+// t + m*N/8.  L lines (columns) per work-group, field b of the batch; `lds`
+// holds L * LdsLine<N>::STRIDE values.
 template <int N, class R, int L>
-__global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
+__device__ __forceinline__ void pass_y_body(const FftArgs<R> &a, int b, cd *lds)
 {
   constexpr int T = N / 8;
   constexpr bool WL = (N / 8) <= 64;
-  __shared__ cd lds[L * LdsLine<N>::STRIDE];
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
   const int c = blockIdx.x * L + line;
-  const int b = blockIdx.y;
   cx<R> *f = a.field + b * a.fstride + static_cast<size_t>(c) * N + t;
   cd *ll = lds + line * LdsLine<N>::STRIDE;
   const bool csg = (a.mode & Y_INV) && a.mat.mode == MAT_CSG;
@@ -183,6 +182,32 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
   }
   if (a.mode & Y_FWD) fft_line<N, -1, double, WL>(v, ll, t, twb);
   store8(v, f, T);
+}
+
+// blockIdx.y = field of the batch
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_y(FftArgs<R> a)
+{
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  pass_y_body<N, R, L>(a, static_cast<int>(blockIdx.y), lds);
+}
+
+// Arguments of group member i (batch-1 plans), with the launch's pass-Y mode
+template <class R>
+__device__ __forceinline__ FftArgs<R> member_args(const GroupArgs<R> &g, int i)
+{
+  FftArgs<R> a = g.m[i];
+  a.mode = g.mode;
+  return a;
+}
+
+// blockIdx.y = group member
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_y_group(GroupArgs<R> g)
+{
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  const FftArgs<R> a = member_args(g, static_cast<int>(blockIdx.y));
+  pass_y_body<N, R, L>(a, 0, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,6 +380,22 @@ __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes);
 }
 
+// The same for the members of a group, blockIdx.z = member (members without a
+// CSG material have no map)
+template <class R, int CPL>
+__global__ __launch_bounds__(64) void k_material_map_group(GroupArgs<R> g, int n)
+{
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  __shared__ __align__(16) char scr[map_scratch_bytes<CPL>(64)];
+  const FftArgs<R> a = member_args(g, static_cast<int>(blockIdx.z));
+  if (a.mat.mode != MAT_CSG) return;
+  const int lanes = static_cast<int>(blockDim.x);
+  const int c = blockIdx.y, r0 = blockIdx.x * lanes * CPL;
+  const int r = r0 + static_cast<int>(threadIdx.x);
+  material_map_cells<CPL>(a, mask, n, c, c, r0, r0 + lanes * CPL - 1,
+                          [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes);
+}
+
 // The same for line lengths n that are not a multiple of the strip: cells past
 // the last row are clamped to row n - 1, so those lanes repeat that cell's work
 // and write the same code (and border values) to it.
@@ -381,13 +422,11 @@ __global__ __launch_bounds__(64) void k_material_map_clamped(FftArgs<R> a, int n
 // between need no work-group barrier.  With a CSG material the field-0
 // work-groups then build the material map of the step for their rows.
 template <int N, class R, int L>
-__global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
+__device__ __forceinline__ void pass_x_body(const FftArgs<R> &a, int b, cd *lds)
 {
   constexpr int T = N / 8;
-  __shared__ cd lds[L * LdsLine<N>::STRIDE];
   const int tid = threadIdx.x;
   const int la = tid % L, ta = tid / L;  // mapping A
-  const int b = blockIdx.y;
   // Row group of this work-group.  Work-groups are dealt to the 8 XCDs in
   // turn; the groups that share a 128-byte line of a column (128 / (L s)
   // neighbours) are given to work-groups of one XCD, so that one L2 serves
@@ -431,6 +470,23 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
     material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
                              [&](int m) { return make_int2(ta + m * T, ra); }, wscr, WL);
   }
+}
+
+// blockIdx.y = field of the batch
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
+{
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  pass_x_body<N, R, L>(a, static_cast<int>(blockIdx.y), lds);
+}
+
+// blockIdx.y = group member (the host clears map_in_pass_x: batch-1 plans)
+template <int N, class R, int L>
+__global__ __launch_bounds__(L *N / 8) void k_pass_x_group(GroupArgs<R> g)
+{
+  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  const FftArgs<R> a = member_args(g, static_cast<int>(blockIdx.y));
+  pass_x_body<N, R, L>(a, 0, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,6 +643,34 @@ hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 }
 
 template <class R>
+hipError_t launch_pass_y_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s)
+{
+  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, members); };
+  PXG_SIZE_SWITCH(k_pass_y_group)
+}
+
+template <class R>
+hipError_t launch_pass_x_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s)
+{
+  auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, members); };
+  PXG_SIZE_SWITCH(k_pass_x_group)
+}
+
+template <class R>
+hipError_t launch_material_map_group(int n, const GroupArgs<R> &a, int members, hipStream_t s)
+{
+  // the strips of launch_material_map (power-of-two n only)
+  if (!fft_size_supported(n)) return hipErrorInvalidValue;
+  if (n >= 128) {
+    hipLaunchKernelGGL((k_material_map_group<R, 2>), dim3(n / 128, n, members), dim3(64), 0, s, a, n);
+  } else {
+    const int rows = n < 64 ? n : 64;
+    hipLaunchKernelGGL((k_material_map_group<R, 1>), dim3(n / rows, n, members), dim3(rows), 0, s, a, n);
+  }
+  return hipGetLastError();
+}
+
+template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s)
 {
   if (a.blue_m) return launch_blue_fft2d<R>(n, a, batch, s);
@@ -613,6 +697,12 @@ template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, 
 template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_material_map<float>(int, const FftArgs<float> &, hipStream_t);
 template hipError_t launch_material_map<double>(int, const FftArgs<double> &, hipStream_t);
+template hipError_t launch_pass_y_group<float>(int, int, const GroupArgs<float> &, int, hipStream_t);
+template hipError_t launch_pass_y_group<double>(int, int, const GroupArgs<double> &, int, hipStream_t);
+template hipError_t launch_pass_x_group<float>(int, int, const GroupArgs<float> &, int, hipStream_t);
+template hipError_t launch_pass_x_group<double>(int, int, const GroupArgs<double> &, int, hipStream_t);
+template hipError_t launch_material_map_group<float>(int, const GroupArgs<float> &, int, hipStream_t);
+template hipError_t launch_material_map_group<double>(int, const GroupArgs<double> &, int, hipStream_t);
 template hipError_t launch_fft2d_step<float>(int, const FftArgs<float> &, int, hipStream_t);
 template hipError_t launch_fft2d_step<double>(int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_cube_store<float>(const cx<float> *, size_t, int, int, cx<double> *,

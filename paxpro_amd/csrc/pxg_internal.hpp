@@ -42,6 +42,18 @@ struct FftArgs {
   const cd *bf, *bi;       // [M] FFT_M of the forward / inverse convolution kernel, / M
 };
 
+// Batch-1 FFT3D plans of one line length advanced in shared launches
+// (pxg_step_group): the members' arguments travel by value with each launch
+// (kernel arguments keep their buffers known as global memory); MAX_GROUP
+// members keep them within 4 KiB.
+constexpr int MAX_GROUP = 8;
+template <class R>
+struct GroupArgs {
+  FftArgs<R> m[MAX_GROUP];
+  int mode;
+};
+static_assert(sizeof(GroupArgs<double>) <= 4096, "group kernel arguments within 4 KiB");
+
 // Bit of a material-map code marking a border cell (material differs between
 // the two planes of the step); the other bits hold the deciding part index.
 constexpr unsigned MAP_CROSSING = 0x8000u;
@@ -55,6 +67,13 @@ template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
+// The same three launches over the members of a group (grid y or z = member)
+template <class R>
+hipError_t launch_pass_y_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s);
+template <class R>
+hipError_t launch_pass_x_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s);
+template <class R>
+hipError_t launch_material_map_group(int n, const GroupArgs<R> &a, int members, hipStream_t s);
 template <class R>
 hipError_t launch_fft2d_step(int n, const FftArgs<R> &a, int batch, hipStream_t s);
 // Bluestein variants of the passes for other line lengths (pxg_bluestein.hip);

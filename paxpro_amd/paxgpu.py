@@ -33,7 +33,7 @@ EXPORTS = [
     "pxg_plan_create", "pxg_plan_destroy", "pxg_plan_bytes", "pxg_set_wavenumbers",
     "pxg_set_field", "pxg_get_field", "pxg_get_solution", "pxg_solution_dims",
     "pxg_set_material_csg", "pxg_set_material_curved_wg", "pxg_set_material_planes",
-    "pxg_set_material_none", "pxg_step", "pxg_sync", "pxg_step_profiled",
+    "pxg_set_material_none", "pxg_step", "pxg_step_group", "pxg_sync", "pxg_step_profiled",
     "pxg_field_device_ptr", "pxg_thomas_batched", "pxg_plan_create_jobs",
     "pxg_set_job_material_curved_wg", "pxg_job_solution_dims", "pxg_plan_reset",
     "pxg_plan_set_lines", "pxg_plan_get_lines", "pxg_plan_autotune",
@@ -126,6 +126,7 @@ def lib():
         L.pxg_set_material_none.argtypes = [P]
         L.pxg_set_equation_planes.argtypes = [P, P, I, I, I]
         L.pxg_step.argtypes = [P, U, U]
+        L.pxg_step_group.argtypes = [P, I, P, U]
         L.pxg_sync.argtypes = [P]
         L.pxg_step_profiled.argtypes = [P, U, U, P, I, P, P]
         L.pxg_field_device_ptr.argtypes = [P, I, P]
@@ -226,6 +227,19 @@ def plan_bytes(prob):
     n = C.c_size_t()
     _check(lib().pxg_plan_bytes(C.byref(prob), C.byref(n)))
     return n.value
+
+
+MAX_GROUP = 8
+
+
+def step_group(plans, first_steps, nsteps):
+    """pxg_step_group: advance batch-1 FFT3D plans of one nx and precision in
+    shared launches, plan i from first_steps[i] (an int applies to all)."""
+    if isinstance(first_steps, int):
+        first_steps = [first_steps] * len(plans)
+    hs = (C.c_void_p * len(plans))(*[p.h.value for p in plans])
+    fs = (C.c_uint * len(plans))(*first_steps)
+    _check(lib().pxg_step_group(C.cast(hs, C.c_void_p), len(plans), C.cast(fs, C.c_void_p), nsteps))
 
 
 class Plan:
