@@ -438,7 +438,7 @@ def transmittivity_timing(ctx, job, steps):
                     "is not in this time"}
 
 
-def served_run(ctx, requests, rate, slots, chunk):
+def served_run(ctx, requests, rate, slots, chunk, max_group=P.MAX_GROUP, group_max_nx=0):
     """The request server on a Poisson trace of `requests` arrivals at `rate` per
     second: served Mcell*steps/s over the whole job, latencies from submission to
     result on the host, and the queue's growth (medians by arrival quarter)."""
@@ -449,7 +449,7 @@ def served_run(ctx, requests, rate, slots, chunk):
         if key not in fields:
             fields[key] = (W.gaussian_field(r["n"], 20000.0 / r["n"], -10000.0, 3000.0) if r["kind"] == "fft3d"
                            else np.ones(r["n"], complex))
-    srv = P.Server(ctx, slots=slots, chunk_steps=chunk)
+    srv = P.Server(ctx, slots=slots, chunk_steps=chunk, max_group=max_group, group_max_nx=group_max_nx)
     ids = []
     t0 = time.perf_counter()
     for r in trace:
@@ -487,7 +487,8 @@ def served_run(ctx, requests, rate, slots, chunk):
             "utilisation": round(st["busy_s"] / wall, 3), "wall_s": round(wall, 3),
             "served_requests_per_busy_s": round(len(lat) / max(st["busy_s"], 1e-9), 2),
             "p50_ms_by_arrival_quarter": quarters, "p99_ms_first_second_half": halves,
-            "slots": slots, "chunk_steps": chunk}
+            "slots": slots, "chunk_steps": chunk, "max_group": max_group, "group_max_nx": group_max_nx,
+            "chunks_in_shared_launches": f"{st['grouped_chunks']}/{st['chunks']}"}
 
 
 SERVED_WORKLOAD = ("70% FFT3D N in {256..2048}, 200..2000 steps, 64-sphere phantoms; "
@@ -496,7 +497,7 @@ SERVED_WORKLOAD = ("70% FFT3D N in {256..2048}, 200..2000 steps, 64-sphere phant
 
 def bench_served(args, ctx):
     """The request server on a Poisson trace: throughput and p50/p99 latency."""
-    r = served_run(ctx, args.requests, args.rate, args.slots, args.chunk)
+    r = served_run(ctx, args.requests, args.rate, args.slots, args.chunk, args.max_group, args.group_max_nx)
     return line("Mcell*steps/s served, with p50/p99 request latency (mixed FFT3D + CN stream)",
                 r["value"], "Mcell*steps/s", r["requests"], 0, r["wall_s"] * 1e3 / r["requests"],
                 "c64 storage, fp64 arithmetic", "synthetic Poisson trace (seed 0x5EED0004)",
@@ -669,6 +670,9 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     ap.add_argument("--slots", type=int, default=8, help="served: jobs running at once")
     ap.add_argument("--chunk", type=int, default=64, help="served: steps per chunk")
+    ap.add_argument("--max-group", type=int, default=P.MAX_GROUP,
+                    help="served: FFT3D chunks per shared launch (1: every chunk alone)")
+    ap.add_argument("--group-max-nx", type=int, default=0, help="served: largest nx that shares launches (0: any)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -374,6 +374,8 @@ typedef struct {
   double cell_steps;                        /* sum over completed jobs */
   double busy_s;                            /* wall time with at least one job running */
   size_t hbm_reserved, hbm_budget;
+  uint64_t chunks;                          /* job chunks enqueued */
+  uint64_t grouped_chunks;                  /* of them, in shared launches (pxg_step_group) */
 } pxg_server_stats;
 /* slots: jobs running at once; hbm_budget: bytes of plans (0 = 64 GiB);
  * chunk_steps: z-steps enqueued per job before the next job is served.
@@ -390,6 +392,13 @@ int pxg_server_status(pxg_server s, uint64_t id, int *state);
 int pxg_server_wait(pxg_server s, uint64_t id, double *latency_ms, double timeout_s);
 int pxg_server_cancel(pxg_server s, uint64_t id);
 int pxg_server_stats_get(pxg_server s, pxg_server_stats *st);
+/* Chunks of up to max_group running FFT3D jobs (batch 1, no storage, one
+ * power-of-two nx <= max_nx and precision, chunks of equal length) that are ready
+ * together are enqueued as one pxg_step_group; a ready job of such a shape waits
+ * for a partner whose chunk is in flight.  max_group 1 runs every chunk alone;
+ * max_nx 0: any nx.  Defaults: 8 (the most a group holds) and any nx.  Results
+ * do not depend on them. */
+int pxg_server_set_grouping(pxg_server s, int max_group, int max_nx);
 int pxg_server_destroy(pxg_server s);
 
 #ifdef __cplusplus

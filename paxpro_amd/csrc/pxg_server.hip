@@ -56,6 +56,16 @@ bool same_shape(const pxg_problem &a, const pxg_problem &b)
          a.store_dsx == b.store_dsx && a.store_dsz == b.store_dsz && (a.store_dsz == 0 || a.nz == b.nz);
 }
 
+// Jobs whose chunks can share launches (pxg_step_group): FFT3D, batch 1, no
+// solution storage, a power-of-two line length.
+bool groupable(const pxg_problem &p)
+{
+  return p.kind == PXG_FFT3D && p.batch == 1 && p.store_dsz == 0 && p.nx >= 16 && p.nx <= 4096 &&
+         (p.nx & (p.nx - 1)) == 0;
+}
+
+bool same_group(const pxg_problem &a, const pxg_problem &b) { return a.nx == b.nx && a.precision == b.precision; }
+
 }  // namespace
 
 struct pxg_server_s {
@@ -74,6 +84,9 @@ struct pxg_server_s {
   std::vector<Pooled> pool;
   std::atomic<size_t> reserved{0};  // running + pooled plan bytes
   std::atomic<uint64_t> n_running{0};
+  std::atomic<int> max_group{pxg::MAX_GROUP};  // 1: every job's chunks run alone
+  std::atomic<int> group_max_nx{0};             // largest nx that groups (0: any)
+  std::atomic<uint64_t> n_chunks{0}, n_grouped{0};  // chunks issued, of them in groups of >= 2
   uint64_t next_id = 1;
   bool stop = false;
   std::thread worker;
@@ -304,19 +317,70 @@ struct pxg_server_s {
     return progressed;
   }
 
-  void retire(size_t i, int state, int err, const std::string &msg)
+  void retire(std::shared_ptr<Job> j, int state, int err, const std::string &msg)
   {
-    std::shared_ptr<Job> j = running[i];
-    running.erase(running.begin() + static_cast<long>(i));
+    running.erase(std::find(running.begin(), running.end(), j));
     --n_running;
     finish(j, state, err, msg);
   }
 
+  unsigned next_chunk(const Job &j) const { return std::min(chunk, j.spec.nsteps - j.done); }
+
+  // A running job of j's group whose chunk is in flight and whose next chunk
+  // will be as long as j's: j waits for it rather than starting a chunk alone.
+  bool partner_in_flight(const Job &j) const
+  {
+    const unsigned c = next_chunk(j);
+    for (const auto &r : running) {
+      if (r.get() == &j || !r->in_flight || !groupable(r->spec.prob) || !same_group(r->spec.prob, j.spec.prob))
+        continue;
+      const unsigned left = r->spec.nsteps - r->done - r->in_flight;
+      if (std::min(chunk, left) == c) return true;
+    }
+    return false;
+  }
+
+  // Enqueues the next chunk of the jobs in g (one job, or a group of jobs with
+  // equal chunk lengths) and records each job's event on its own stream.
+  void issue(const std::vector<std::shared_ptr<Job>> &g)
+  {
+    const unsigned c = next_chunk(*g[0]);
+    int rc;
+    if (g.size() == 1) {
+      rc = pxg_step(g[0]->plan, g[0]->spec.first_step + g[0]->done, c);
+    } else {
+      pxg_plan plans[pxg::MAX_GROUP];
+      unsigned first[pxg::MAX_GROUP];
+      for (size_t i = 0; i < g.size(); ++i) {
+        plans[i] = g[i]->plan;
+        first[i] = g[i]->spec.first_step + g[i]->done;
+      }
+      rc = pxg_step_group(plans, static_cast<int>(g.size()), first, c);
+      n_grouped += g.size();
+    }
+    n_chunks += g.size();
+    const std::string msg = rc == PXG_OK ? std::string() : std::string(pxg_last_error());
+    for (const auto &j : g) {
+      int r = rc;
+      if (r == PXG_OK)
+        r = hipEventRecord(j->ev, pxg::plan_stream(j->plan)) == hipSuccess ? PXG_OK : PXG_ERR_DEVICE;
+      if (r != PXG_OK) {
+        retire(j, PXG_JOB_FAILED, r, rc != PXG_OK ? msg : "hipEventRecord failed");
+        continue;
+      }
+      j->in_flight = c;
+    }
+  }
+
   // Advances running jobs by one chunk each; finished jobs copy their result
-  // out and leave.  Runs without mu except to read cancellation flags.
+  // out and leave.  Jobs that are ready together, share a group shape and have
+  // chunks of equal length run in shared launches (at most max_group per
+  // launch); a ready job of such a shape whose partner is in flight waits for
+  // it.  Runs without mu except to read cancellation flags.
   bool advance()
   {
     bool progressed = false;
+    std::vector<std::shared_ptr<Job>> ready;
     for (size_t i = 0; i < running.size();) {
       std::shared_ptr<Job> j = running[i];
       if (j->in_flight) {
@@ -335,28 +399,41 @@ struct pxg_server_s {
         cancel = j->cancel;
       }
       if (cancel) {
-        retire(i, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled");
+        retire(j, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "cancelled");
         continue;
       }
       if (j->spec.inject_failure_after_chunks >= 0 && j->chunks >= j->spec.inject_failure_after_chunks) {
-        retire(i, PXG_JOB_FAILED, PXG_ERR_INJECTED, "failure injected by the host");
+        retire(j, PXG_JOB_FAILED, PXG_ERR_INJECTED, "failure injected by the host");
         continue;
       }
       if (j->done >= j->spec.nsteps) {
         const int rc = pxg_get_field(j->plan, 0, j->spec.result);
-        retire(i, rc == PXG_OK ? PXG_JOB_DONE : PXG_JOB_FAILED, rc, rc == PXG_OK ? "" : pxg_last_error());
+        retire(j, rc == PXG_OK ? PXG_JOB_DONE : PXG_JOB_FAILED, rc, rc == PXG_OK ? "" : pxg_last_error());
         continue;
       }
-      const unsigned c = std::min(chunk, j->spec.nsteps - j->done);
-      int rc = pxg_step(j->plan, j->spec.first_step + j->done, c);
-      if (rc == PXG_OK) rc = hipEventRecord(j->ev, pxg::plan_stream(j->plan)) == hipSuccess ? PXG_OK : PXG_ERR_DEVICE;
-      if (rc != PXG_OK) {
-        retire(i, PXG_JOB_FAILED, rc, pxg_last_error());
-        continue;
-      }
-      j->in_flight = c;
-      progressed = true;
+      ready.push_back(j);
       ++i;
+    }
+    const size_t gmax = static_cast<size_t>(std::max(1, std::min<int>(max_group, pxg::MAX_GROUP)));
+    std::vector<bool> taken(ready.size(), false);
+    for (size_t a = 0; a < ready.size(); ++a) {
+      if (taken[a]) continue;
+      const std::shared_ptr<Job> &j = ready[a];
+      std::vector<std::shared_ptr<Job>> g{j};
+      taken[a] = true;
+      const int gnx = group_max_nx;
+      if (gmax > 1 && groupable(j->spec.prob) && (gnx == 0 || j->spec.prob.nx <= gnx)) {
+        const unsigned c = next_chunk(*j);
+        for (size_t b = a + 1; b < ready.size() && g.size() < gmax; ++b)
+          if (!taken[b] && groupable(ready[b]->spec.prob) && same_group(j->spec.prob, ready[b]->spec.prob) &&
+              next_chunk(*ready[b]) == c) {
+            g.push_back(ready[b]);
+            taken[b] = true;
+          }
+        if (g.size() == 1 && partner_in_flight(*j)) continue;
+      }
+      issue(g);
+      progressed = true;
     }
     if (running.empty() && busy_since != clk::time_point()) {
       std::lock_guard<std::mutex> lk(mu);
@@ -384,7 +461,7 @@ struct pxg_server_s {
           cv_work.wait_for(lk, std::chrono::microseconds(20), [&] { return stop; });
       }
     }
-    while (!running.empty()) retire(0, PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
+    while (!running.empty()) retire(running.front(), PXG_JOB_CANCELLED, PXG_ERR_CANCELLED, "server stopped");
     std::deque<std::shared_ptr<Job>> left;
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -507,6 +584,16 @@ int pxg_server_stats_get(pxg_server s, pxg_server_stats *st)
                                 : 0.0);
   st->hbm_reserved = s->reserved;
   st->hbm_budget = s->budget;
+  st->chunks = s->n_chunks;
+  st->grouped_chunks = s->n_grouped;
+  return PXG_OK;
+}
+
+int pxg_server_set_grouping(pxg_server s, int max_group, int max_nx)
+{
+  if (!s || max_group < 1 || max_group > pxg::MAX_GROUP || max_nx < 0) return PXG_ERR_INVALID;
+  s->max_group = max_group;
+  s->group_max_nx = max_nx;
   return PXG_OK;
 }
 

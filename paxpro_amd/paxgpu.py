@@ -41,7 +41,7 @@ EXPORTS = [
     "pxg_server_cancel", "pxg_server_stats_get", "pxg_server_destroy",
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
     "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
-    "pxg_thomas_batched_noalloc", "pxg_set_equation_planes",
+    "pxg_thomas_batched_noalloc", "pxg_set_equation_planes", "pxg_server_set_grouping",
 ]
 THOMAS_NOALLOC_MAX_ROWS = 1 << 20
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
@@ -90,7 +90,8 @@ class ServerStats(C.Structure):
                                           "rejected", "running", "queued")] + \
                [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "cell_steps",
                                           "busy_s")] + \
-               [("hbm_reserved", C.c_size_t), ("hbm_budget", C.c_size_t)]
+               [("hbm_reserved", C.c_size_t), ("hbm_budget", C.c_size_t), ("chunks", C.c_uint64),
+                ("grouped_chunks", C.c_uint64)]
 
 
 class KernelStat(C.Structure):
@@ -144,6 +145,7 @@ def lib():
         L.pxg_server_cancel.argtypes = [P, C.c_uint64]
         L.pxg_server_stats_get.argtypes = [P, P]
         L.pxg_server_destroy.argtypes = [P]
+        L.pxg_server_set_grouping.argtypes = [P, I, I]
         L.pxg_set_job_material_curved_wg.argtypes = [P, I, P]
         L.pxg_job_solution_dims.argtypes = [P, I, P]
         L.pxg_far_field_dims.argtypes = [P, I, U, D, D, U, U, P]
@@ -465,11 +467,16 @@ class JobPlan(Plan):
 class Server:
     """pxg_server: persistent request loop with HBM-budget admission."""
 
-    def __init__(self, ctx, slots=4, budget=0, chunk_steps=32):
+    def __init__(self, ctx, slots=4, budget=0, chunk_steps=32, max_group=None, group_max_nx=0):
         self.ctx = ctx
         self.h = C.c_void_p()
         _check(lib().pxg_server_create(ctx.h, slots, budget, chunk_steps, C.byref(self.h)))
         self._keep = {}
+        if max_group is not None or group_max_nx:
+            self.set_grouping(MAX_GROUP if max_group is None else max_group, group_max_nx)
+
+    def set_grouping(self, max_group, max_nx=0):
+        _check(lib().pxg_server_set_grouping(self.h, max_group, max_nx))
 
     def submit(self, prob, field, nsteps, first_step=1, csg=None, curved=None, inject=-1,
                inject_plan=INJECT_NONE):
