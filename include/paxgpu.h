@@ -396,8 +396,8 @@ int pxg_server_stats_get(pxg_server s, pxg_server_stats *st);
  * power-of-two nx <= max_nx and precision, chunks of equal length) that are ready
  * together are enqueued as one pxg_step_group; a ready job of such a shape waits
  * for a partner whose chunk is in flight.  max_group 1 runs every chunk alone;
- * max_nx 0: any nx.  Defaults: 8 (the most a group holds) and any nx.  Results
- * do not depend on them. */
+ * max_nx 0: any nx.  Defaults: 8 (the most a group holds) and 512 (larger grids
+ * fill the device with one job's launches).  Results do not depend on them. */
 int pxg_server_set_grouping(pxg_server s, int max_group, int max_nx);
 int pxg_server_destroy(pxg_server s);
 

@@ -232,6 +232,7 @@ def plan_bytes(prob):
 
 
 MAX_GROUP = 8
+SERVER_GROUP_MAX_NX = 512   # the server's default: larger grids fill the device alone
 
 
 def step_group(plans, first_steps, nsteps):
@@ -467,16 +468,17 @@ class JobPlan(Plan):
 class Server:
     """pxg_server: persistent request loop with HBM-budget admission."""
 
-    def __init__(self, ctx, slots=4, budget=0, chunk_steps=32, max_group=None, group_max_nx=0):
+    def __init__(self, ctx, slots=4, budget=0, chunk_steps=32, max_group=None, group_max_nx=None):
         self.ctx = ctx
         self.h = C.c_void_p()
         _check(lib().pxg_server_create(ctx.h, slots, budget, chunk_steps, C.byref(self.h)))
         self._keep = {}
-        if max_group is not None or group_max_nx:
-            self.set_grouping(MAX_GROUP if max_group is None else max_group, group_max_nx)
+        if max_group is not None or group_max_nx is not None:
+            self.set_grouping(MAX_GROUP if max_group is None else max_group,
+                              SERVER_GROUP_MAX_NX if group_max_nx is None else group_max_nx)
 
-    def set_grouping(self, max_group, max_nx=0):
-        _check(lib().pxg_server_set_grouping(self.h, max_group, max_nx))
+    def set_grouping(self, max_group, max_nx=None):
+        _check(lib().pxg_server_set_grouping(self.h, max_group, SERVER_GROUP_MAX_NX if max_nx is None else max_nx))
 
     def submit(self, prob, field, nsteps, first_step=1, csg=None, curved=None, inject=-1,
                inject_plan=INJECT_NONE):
