@@ -1589,6 +1589,12 @@ int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, cons
       ds[i].op = shapes[i].op;
       std::memcpy(ds[i].T, shapes[i].T, sizeof(double) * 12);
       std::memcpy(ds[i].p, shapes[i].p, sizeof(double) * 3);
+      const double *T = shapes[i].T;
+      ds[i].translate = T[0] == 1.0 && T[1] == 0.0 && T[2] == 0.0 && T[4] == 0.0 && T[5] == 1.0 &&
+                        T[6] == 0.0 && T[8] == 0.0 && T[9] == 0.0 && T[10] == 1.0;
+      const double r = shapes[i].p[0], r2 = r * r;
+      ds[i].r2lo = r2 * (1.0 - 1e-14);
+      ds[i].r2hi = r2 * (1.0 + 1e-14);
     }
     std::vector<DPart> order;
     std::vector<DBox> boxes;

@@ -10,10 +10,16 @@ namespace pxg {
 enum MatMode { MAT_NONE = 0, MAT_CSG = 1, MAT_PLANES = 2, MAT_CURVED = 3 };
 
 // One shape: rows 0..2 of its 4x4 world->shape transform and its parameters.
+// translate: the 3x3 part of T is exactly the identity, so T applied to a finite
+// point (x, y, z) is (x + T[3], y + T[7], z + T[11]) with the same values as the
+// full product (1*x = x, 0*y = +-0, x + +-0 = x).  r2lo, r2hi: for a sphere of
+// radius p[0] > 0, radius_less's band r^2 (1 -+ 1e-14), products formed on the
+// host as the device forms them.
 struct DShape {
-  int type, op;
+  int type, op, translate, pad_;
   double T[12];
   double p[3];
+  double r2lo, r2hi;
 };
 
 // One part in *evaluation order*: modules in order, each module's parts last
@@ -69,10 +75,23 @@ __device__ __forceinline__ bool radius_less(double s, double r)
 __device__ __forceinline__ bool shape_inside(const DShape &s, double x, double y, double z)
 {
 #pragma clang fp contract(off)
-  double px = s.T[0] * x + s.T[1] * y + s.T[2] * z + s.T[3];
-  double py = s.T[4] * x + s.T[5] * y + s.T[6] * z + s.T[7];
-  double pz = s.T[8] * x + s.T[9] * y + s.T[10] * z + s.T[11];
-  if (s.type == 0) return radius_less(px * px + py * py + pz * pz, s.p[0]);
+  double px, py, pz;
+  if (s.translate) {
+    px = x + s.T[3];
+    py = y + s.T[7];
+    pz = z + s.T[11];
+  } else {
+    px = s.T[0] * x + s.T[1] * y + s.T[2] * z + s.T[3];
+    py = s.T[4] * x + s.T[5] * y + s.T[6] * z + s.T[7];
+    pz = s.T[8] * x + s.T[9] * y + s.T[10] * z + s.T[11];
+  }
+  if (s.type == 0) {
+    const double q = px * px + py * py + pz * pz;
+    if (!(s.p[0] > 0.0)) return sqrt(q) < s.p[0];
+    if (q < s.r2lo) return true;
+    if (q > s.r2hi) return false;
+    return sqrt(q) < s.p[0];
+  }
   if (s.type == 1)
     return (px > -s.p[0] / 2.0) && (px < s.p[0] / 2.0) && (py > -s.p[1] / 2.0) &&
            (py < s.p[1] / 2.0) && (pz > -s.p[2] / 2.0) && (pz < s.p[2] / 2.0);
