@@ -144,12 +144,22 @@ int pxg_plan_bytes(const pxg_problem *prob, size_t *bytes);
  * (src/fftSolver3D.cpp:61-62 plans with FFTW_ESTIMATE). */
 int pxg_plan_set_lines(pxg_plan plan, int lines_x, int lines_y);
 int pxg_plan_get_lines(pxg_plan plan, int *lines_x, int *lines_y);
+/* Tile shapes of the ADI half-step kernels (tile-Jacobi path): index of the
+ * x-implicit shape (lines x positions: 0: 16x128, 1: 8x256, 2: 16x256, 3: 32x64,
+ * 4: 32x128) and of the y-implicit shape (0: 4x256, 1: 1x256, 2: 2x256, 3: 8x256,
+ * 4: 2x512); -1 keeps the current one.  Results do not depend on them.  ADI plans
+ * only.  No reference counterpart (src/alternatingDirectionSolver.cpp:8-134
+ * sweeps one line at a time). */
+#define PXG_ADI_TILES 5
+int pxg_plan_set_adi_tiles(pxg_plan plan, int x_tile, int y_tile);
+int pxg_plan_get_adi_tiles(pxg_plan plan, int *x_tile, int *y_tile);
 /* Autotuner, in the manner of FFTW's measured planning: times every supported
- * shape of both FFT3D passes on this plan with its current material and keeps
- * the fastest; the choice is cached in the context for plans of the same nx,
- * batch, precision and material kind.  Overwrites (zeroes) the field: call it
- * after the material and before pxg_set_field.  reps <= 0 means 5 timed
- * launches per shape.  A no-op for the other kinds. */
+ * shape of both FFT3D passes (lines per work-group), or every tile shape of both
+ * ADI half-steps, on this plan with its current material and keeps the fastest;
+ * the choice is cached in the context for plans of the same kind, nx, batch,
+ * precision and material kind.  Overwrites (zeroes) the field: call it after the
+ * material and before pxg_set_field.  reps <= 0 means 5 timed launches per shape.
+ * A no-op for the other kinds. */
 int pxg_plan_autotune(pxg_plan plan, int reps);
 /* FFT3D z-step loops as HIP graphs: with enable != 0, pxg_step captures the
  * kernels of a call into a graph on first use and replays it when a later call has

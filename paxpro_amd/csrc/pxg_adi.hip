@@ -36,16 +36,36 @@ constexpr int ADI_C = 4;            // consecutive positions per lane
 constexpr int ADI_HALO_MAX = 8;     // largest sweep count (and halo) of the tile path
 constexpr int ADI_MASK_WORDS = 16;
 
-template <bool IMPX>
+// A tile of TL_ lines x TP_ positions.  The shapes built (index = the plan's tile
+// choice, pxg_plan_set_adi_tiles): x-implicit (16, 128) (8, 256) (16, 256)
+// (32, 64) (32, 128); y-implicit (4, 256) (1, 256) (2, 256) (8, 256) (2, 512).
+// Every shape gives the same bits (each position sees the same K-sweep iterate).
+template <bool IMPX, int TL_, int TP_>
 struct AdiTile {
-  static constexpr int TL = IMPX ? 16 : 4;    // lines per tile
-  static constexpr int TP = IMPX ? 128 : 256; // positions per line
+  static constexpr int TL = TL_;              // lines per tile
+  static constexpr int TP = TP_;              // positions per line
   static constexpr int G = TP / ADI_C;        // lanes per line
   static constexpr int LANES = TL * G;
   static constexpr int DG = IMPX ? TL : 1;    // lane distance of the next position group
-  static constexpr bool WAVE_LOCAL = !IMPX;   // a line tile is one wavefront
+  static constexpr bool WAVE_LOCAL = !IMPX && G == 64;  // a line tile is one wavefront
+  static_assert(LANES <= 1024 && LANES % 64 == 0, "work-group of whole wavefronts");
 };
-static_assert(AdiTile<false>::G == 64, "y-implicit line tile must be one wavefront");
+
+// Complex product and squared modulus with every rounding spelled out (one
+// product rounded, then one fused multiply-add per real part), so that every
+// instantiation of the kernel (each tile shape) gives the same bits: left to
+// the compiler, a*b + c*d may fuse either product.
+__device__ __forceinline__ cx<double> cmul_fixed(cx<double> a, cx<double> b)
+{
+#pragma clang fp contract(off)
+  return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
+}
+
+__device__ __forceinline__ double norm2_fixed(cx<double> a)
+{
+#pragma clang fp contract(off)
+  return fma(a.x, a.x, a.y * a.y);
+}
 
 // Transparent boundary at the ends of the implicit line: diag correction from
 // the cell's value and its inward neighbour along the line.
@@ -62,7 +82,7 @@ __device__ __forceinline__ cx<double> tbc_rhs(cx<double> rhs, cx<double> u, cx<d
 {
   cx<double> e;
   if (tbc_exp_ikd(u, uin, 1e-6, e)) {
-    const cx<double> q = cx<double>{-0.5 * e.y, 0.5 * e.x} * u;
+    const cx<double> q = cmul_fixed(cx<double>{-0.5 * e.y, 0.5 * e.x}, u);
     rhs = rhs + cx<double>{q.x / dexp, q.y / dexp};
   }
   return rhs;
@@ -94,12 +114,14 @@ __device__ __forceinline__ int xcd_tile(int b, int total)
   return x * q + min(x, r) + (b >> 3);
 }
 
-template <class R, bool IMPX>
-__global__ __launch_bounds__(AdiTile<IMPX>::LANES) void k_adi_jacobi(TriArgs<R> a, AdiCoef c,
-                                                                    const cx<R> *__restrict__ prev,
-                                                                    cx<R> *__restrict__ next, int iters, int nseg)
+template <class R, bool IMPX, int TL, int TP>
+__global__ __launch_bounds__((AdiTile<IMPX, TL, TP>::LANES)) void k_adi_jacobi(TriArgs<R> a, AdiCoef c,
+                                                                            const cx<R> *__restrict__ prev,
+                                                                            cx<R> *__restrict__ next, int iters,
+                                                                            int nseg)
 {
-  using T = AdiTile<IMPX>;
+#pragma clang fp contract(off)  // fused operations below are written out
+  using T = AdiTile<IMPX, TL, TP>;
   constexpr int C = ADI_C;
   __shared__ cx<double> edge[2][2][T::LANES];  // [buffer][first, last position][lane]
   __shared__ uint64_t mask[ADI_MASK_WORDS];
@@ -180,17 +202,18 @@ __global__ __launch_bounds__(AdiTile<IMPX>::LANES) void k_adi_jacobi(TriArgs<R> 
 #pragma unroll
   for (int m = 0; m < C; ++m) {
     if (IMPX || (a.quirks & PXG_FIX_ADI_Y_DIAG)) {
-      dg[m] = {c.ih + bb[m] * k, c.idimp + dd[m] * k};
+      dg[m] = {fma(bb[m], k, c.ih), fma(dd[m], k, c.idimp)};
     } else {
       // 1/h + i / (k dy^2 + k beta + i k delta)
-      const double p = c.ddy + bb[m] * k, q = dd[m] * k, r = recip(p * p + q * q);
-      dg[m] = {c.ih + q * r, p * r};
+      const double p = fma(bb[m], k, c.ddy), q = dd[m] * k, r = recip(fma(p, p, q * q));
+      dg[m] = {fma(q, r, c.ih), p * r};
     }
     const cx<double> u = cvt<double>(ru[m]);
     cx<double> un{0.0, 0.0};
     if (line > 0) un = cvt<double>(rm[m]);
     if (line < N - 1) un = un + cvt<double>(rp[m]);
-    rh[m] = cx<double>{c.ih, -c.iexp} * u + cx<double>{-un.y * c.hexp, un.x * c.hexp};
+    const cx<double> t = cmul_fixed(cx<double>{c.ih, -c.iexp}, u);
+    rh[m] = {fma(-un.y, c.hexp, t.x), fma(un.x, c.hexp, t.y)};
   }
   // Transparent boundary rows, after the loaded values are consumed: the few
   // lanes holding boundary cells read their neighbours again.
@@ -211,10 +234,10 @@ __global__ __launch_bounds__(AdiTile<IMPX>::LANES) void k_adi_jacobi(TriArgs<R> 
   cx<double> x0[C], w[C], x[C];
 #pragma unroll
   for (int m = 0; m < C; ++m) {
-    const double r = recip(norm2(dg[m]));
+    const double r = recip(norm2_fixed(dg[m]));
     const double lr = live[m] ? r : 0.0, lo = live[m] ? c.o * r : 0.0;
     const cx<double> inv = {dg[m].x * lr, -dg[m].y * lr};
-    x0[m] = x[m] = rh[m] * inv;
+    x0[m] = x[m] = cmul_fixed(rh[m], inv);
     w[m] = {dg[m].y * lo, dg[m].x * lo};  // i o conj(diag) / |diag|^2
   }
 
@@ -266,10 +289,10 @@ int adi_jacobi_iters(double dz, double k, double d, double tol)
   return K <= ADI_HALO_MAX ? K : 0;
 }
 
-template <class R, bool IMPX>
+template <class R, bool IMPX, int TL, int TP>
 static hipError_t launch_tiles(const TriArgs<R> &a, const cx<R> *prev, cx<R> *next, int iters, hipStream_t s)
 {
-  using T = AdiTile<IMPX>;
+  using T = AdiTile<IMPX, TL, TP>;
   const int stride = T::TP - 2 * iters;
   const int nseg = (a.n + stride - 1) / stride;
   const int ngrp = (a.n + T::TL - 1) / T::TL;
@@ -287,17 +310,36 @@ static hipError_t launch_tiles(const TriArgs<R> &a, const cx<R> *prev, cx<R> *ne
   c.hexp = 0.5 / c.dexp;
   c.o = -0.5 / c.dimp;
   c.ddy = ddy;
-  hipLaunchKernelGGL((k_adi_jacobi<R, IMPX>), dim3(nseg * ngrp), dim3(T::LANES), 0, s, a, c, prev, next, iters,
-                     nseg);
+  hipLaunchKernelGGL((k_adi_jacobi<R, IMPX, TL, TP>), dim3(nseg * ngrp), dim3(T::LANES), 0, s, a, c, prev, next,
+                     iters, nseg);
   return hipGetLastError();
 }
+
+int adi_tile_count() { return 5; }
 
 template <class R>
 hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
                              int iters, hipStream_t s)
 {
   if (iters < 1 || iters > ADI_HALO_MAX) return hipErrorInvalidValue;
-  return implicit_x ? launch_tiles<R, true>(a, prev, next, iters, s) : launch_tiles<R, false>(a, prev, next, iters, s);
+  if (implicit_x) {
+    switch (a.adi_tile) {
+      case 0: return launch_tiles<R, true, 16, 128>(a, prev, next, iters, s);
+      case 1: return launch_tiles<R, true, 8, 256>(a, prev, next, iters, s);
+      case 2: return launch_tiles<R, true, 16, 256>(a, prev, next, iters, s);
+      case 3: return launch_tiles<R, true, 32, 64>(a, prev, next, iters, s);
+      case 4: return launch_tiles<R, true, 32, 128>(a, prev, next, iters, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (a.adi_tile) {
+    case 0: return launch_tiles<R, false, 4, 256>(a, prev, next, iters, s);
+    case 1: return launch_tiles<R, false, 1, 256>(a, prev, next, iters, s);
+    case 2: return launch_tiles<R, false, 2, 256>(a, prev, next, iters, s);
+    case 3: return launch_tiles<R, false, 8, 256>(a, prev, next, iters, s);
+    case 4: return launch_tiles<R, false, 2, 512>(a, prev, next, iters, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template hipError_t launch_adi_jacobi<float>(const TriArgs<float> &, bool, const cx<float> *, cx<float> *, int,

@@ -121,6 +121,8 @@ struct pxg_ctx_s {
   bool closed = false;
   // autotuned FFT3D work-group shapes: (nx, batch, precision, material) -> (lines_x, lines_y)
   std::map<std::array<int, 4>, std::pair<int, int>> tuned;
+  // autotuned ADI tile shapes: (nx, batch, precision, material) -> (x tile, y tile)
+  std::map<std::array<int, 4>, std::pair<int, int>> tuned_adi;
 };
 
 struct pxg_plan_s {
@@ -148,6 +150,7 @@ struct pxg_plan_s {
   // material
   int mat_mode = MAT_NONE;
   DBuf shapes, parts, boxes, part_fac, planes;
+  int adi_tile[2] = {0, 0};  // ADI: tile shapes of the x- and y-implicit half-steps
   int n_parts = 0, is_ref = 0;
   std::vector<DPart> parts_host;
   int n_planes = 0, first_plane = 0, z_invariant = 0;
@@ -734,6 +737,44 @@ TriArgs<R> tri_args(pxg_plan P)
   return a;
 }
 
+// Mean duration of `reps` ADI half-steps (x- or y-implicit) of a mid-run step
+// with tile shape `tile`, on the plan's own field, scratch and material.
+template <class R>
+double time_adi_half(pxg_plan P, bool implicit_x, int tile, int reps)
+{
+  TriArgs<R> a = tri_args<R>(P);
+  a.first_step = std::max(1, P->p.nz / 2);
+  a.adi_tile = tile;
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float ms = 0.0f;
+  hipError_t err = launch_adi_half<R>(a, implicit_x, P->stream);
+  if (err == hipSuccess) err = hipEventRecord(e0, P->stream);
+  for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch_adi_half<R>(a, implicit_x, P->stream);
+  if (err == hipSuccess) err = hipEventRecord(e1, P->stream);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  hip_check(err, "autotune launch");
+  return static_cast<double>(ms) / reps;
+}
+
+template <class R>
+void autotune_adi(pxg_plan P, int reps)
+{
+  for (int h = 0; h < 2; ++h) {
+    double best = 1e30;
+    int pick = P->adi_tile[h];
+    for (int t = 0; t < adi_tile_count(); ++t) {
+      const double ms = time_adi_half<R>(P, h == 0, t, reps);
+      if (ms < best) best = ms, pick = t;
+    }
+    P->adi_tile[h] = pick;
+  }
+}
+
 template <class R>
 void build_cn_jobs(pxg_plan P)
 {
@@ -900,8 +941,10 @@ void run_adi(pxg_plan P, unsigned first, unsigned nsteps)
   const double pass_bytes = 2.0 * P->cells * P->p.batch * sizeof(cx<R>);
   for (unsigned st = first; st < first + nsteps; ++st) {
     a.first_step = st;
+    a.adi_tile = P->adi_tile[0];
     int ev0 = begin(P);
     record(P, "adi_x", pass_bytes, launch_adi_half<R>(a, true, P->stream), ev0);
+    a.adi_tile = P->adi_tile[1];
     ev0 = begin(P);
     record(P, "adi_y", pass_bytes, launch_adi_half<R>(a, false, P->stream), ev0);
     if (store_step(P, st)) cube_store<R>(P, st);
@@ -1716,10 +1759,56 @@ int pxg_plan_get_lines(pxg_plan P, int *lines_x, int *lines_y)
   });
 }
 
+int pxg_plan_set_adi_tiles(pxg_plan P, int x_tile, int y_tile)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (P->p.kind != PXG_ADI) fail(PXG_ERR_UNSUPPORTED, "tile shapes apply to ADI plans only");
+    for (int t : {x_tile, y_tile})
+      if (t < -1 || t >= adi_tile_count()) fail(PXG_ERR_INVALID, "tile index %d outside [0, %d)", t, adi_tile_count());
+    if (x_tile >= 0) P->adi_tile[0] = x_tile;
+    if (y_tile >= 0) P->adi_tile[1] = y_tile;
+  });
+}
+
+int pxg_plan_get_adi_tiles(pxg_plan P, int *x_tile, int *y_tile)
+{
+  return guarded([&] {
+    check_plan(P);
+    if (!x_tile || !y_tile) fail(PXG_ERR_INVALID, "null argument");
+    *x_tile = P->adi_tile[0];
+    *y_tile = P->adi_tile[1];
+  });
+}
+
 int pxg_plan_autotune(pxg_plan P, int reps)
 {
   return guarded([&] {
     check_plan(P);
+    if (P->p.kind == PXG_ADI) {
+      const std::array<int, 4> key = {P->p.nx, P->p.batch, P->p.precision, P->mat_mode};
+      {
+        std::lock_guard<std::mutex> lk(P->ctx->mu);
+        auto it = P->ctx->tuned_adi.find(key);
+        if (it != P->ctx->tuned_adi.end()) {
+          P->adi_tile[0] = it->second.first;
+          P->adi_tile[1] = it->second.second;
+          return;
+        }
+      }
+      if (reps <= 0) reps = 5;
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+      if (P->mixed)
+        autotune_adi<float>(P, reps);
+      else
+        autotune_adi<double>(P, reps);
+      hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
+      hip_check(hipStreamSynchronize(P->stream), "sync");
+      P->field_set = false;
+      std::lock_guard<std::mutex> lk(P->ctx->mu);
+      P->ctx->tuned_adi[key] = {P->adi_tile[0], P->adi_tile[1]};
+      return;
+    }
     if (P->p.kind != PXG_FFT3D || P->blue_m) return;  // the other kinds have one shape
     const std::array<int, 4> key = {P->p.nx, P->p.batch, P->p.precision, P->mat_mode};
     {

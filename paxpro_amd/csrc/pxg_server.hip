@@ -237,7 +237,7 @@ struct pxg_server_s {
       rc = pxg_set_material_none(j->plan);
     // work-group shapes: timed on the first plan of each shape, cached in the
     // context for the next ones (zeroes the field, set just below)
-    if (rc == PXG_OK && s.prob.kind == PXG_FFT3D) rc = pxg_plan_autotune(j->plan, 3);
+    if (rc == PXG_OK && (s.prob.kind == PXG_FFT3D || s.prob.kind == PXG_ADI)) rc = pxg_plan_autotune(j->plan, 3);
     if (rc == PXG_OK) rc = pxg_set_field(j->plan, 0, s.field);
     return rc;
   }

@@ -27,6 +27,7 @@ struct TriArgs {
   unsigned first_step, nsteps;
   double kmin;           // smallest wavenumber of the batch (largest Jacobi contraction)
   const double *khost;   // [batch] wavenumbers in host memory (launch-time constants)
+  int adi_tile;          // ADI: tile shape of the half-step launched (adi_tile_count() of each)
 };
 
 size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
@@ -39,14 +40,18 @@ size_t tridiag_scratch_bytes(const pxg_problem &p, bool mixed);
 // needed: arg r < 0 exactly when atan2(Im r, Re r) < 0, i.e. Im r < 0, or
 // Im r = -0 with Re r < 0.  Returns false when |u_in| <= thresh (the reference
 // then leaves the row unchanged).
+// Every rounding is spelled out (one product rounded, then one fused
+// multiply-add), so each kernel that inlines it gives the same bits.
 __device__ __forceinline__ bool tbc_exp_ikd(cx<double> u_out, cx<double> u_in, double thresh, cx<double> &e)
 {
-  const double n2 = norm2(u_in);
+#pragma clang fp contract(off)
+  const double n2 = fma(u_in.x, u_in.x, u_in.y * u_in.y);
   if (!(sqrt(n2) > thresh)) return false;
   const double in2 = 1.0 / n2;
-  const cx<double> r = {(u_out.x * u_in.x + u_out.y * u_in.y) * in2, (u_out.y * u_in.x - u_out.x * u_in.y) * in2};
+  const cx<double> r = {fma(u_out.x, u_in.x, u_out.y * u_in.y) * in2,
+                        fma(u_out.y, u_in.x, -(u_out.x * u_in.y)) * in2};
   const bool clamp = r.y < 0.0 || (r.y == 0.0 && signbit(r.y) && r.x < 0.0);
-  e = clamp ? cx<double>{sqrt(norm2(r)), 0.0} : r;
+  e = clamp ? cx<double>{sqrt(fma(r.x, r.x, r.y * r.y)), 0.0} : r;
   return true;
 }
 
@@ -99,6 +104,8 @@ hipError_t launch_adi_half(const TriArgs<R> &a, bool implicit_x, hipStream_t s);
 // Tile-Jacobi ADI half-step for diagonally dominant systems (pxg_adi.hip).
 // Sweeps so that rho^K < tol (0 = not dominant enough for the tile path).
 int adi_jacobi_iters(double dz, double k, double d, double tol);
+// Tile shapes of each ADI half-step kernel (indices 0 .. adi_tile_count() - 1)
+int adi_tile_count();
 template <class R>
 hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
                              int iters, hipStream_t s);

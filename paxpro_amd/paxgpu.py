@@ -42,6 +42,7 @@ EXPORTS = [
     "pxg_far_field_dims", "pxg_far_field", "pxg_intensity_trapezoid", "pxg_plan_set_graphs",
     "pxg_field_quantity_dims", "pxg_field_quantity", "pxg_field_uint8", "pxg_far_field_padded",
     "pxg_thomas_batched_noalloc", "pxg_set_equation_planes", "pxg_server_set_grouping",
+    "pxg_plan_set_adi_tiles", "pxg_plan_get_adi_tiles",
 ]
 THOMAS_NOALLOC_MAX_ROWS = 1 << 20
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
@@ -138,6 +139,8 @@ def lib():
         L.pxg_plan_set_lines.argtypes = [P, I, I]
         L.pxg_plan_get_lines.argtypes = [P, P, P]
         L.pxg_plan_autotune.argtypes = [P, I]
+        L.pxg_plan_set_adi_tiles.argtypes = [P, I, I]
+        L.pxg_plan_get_adi_tiles.argtypes = [P, P, P]
         L.pxg_server_create.argtypes = [P, I, S, U, P]
         L.pxg_server_submit.argtypes = [P, P, P]
         L.pxg_server_status.argtypes = [P, C.c_uint64, P]
@@ -232,6 +235,7 @@ def plan_bytes(prob):
 
 
 MAX_GROUP = 8
+ADI_TILES = 5
 SERVER_GROUP_MAX_NX = 512   # the server's default: larger grids fill the device alone
 
 
@@ -270,11 +274,20 @@ class Plan:
         _check(lib().pxg_plan_get_lines(self.h, C.byref(lx), C.byref(ly)))
         return lx.value, ly.value
 
+    def set_adi_tiles(self, x_tile=-1, y_tile=-1):
+        """Tile shapes of the ADI x- and y-implicit half-steps (-1 keeps one)."""
+        _check(lib().pxg_plan_set_adi_tiles(self.h, x_tile, y_tile))
+
+    def adi_tiles(self):
+        tx, ty = C.c_int(), C.c_int()
+        _check(lib().pxg_plan_get_adi_tiles(self.h, C.byref(tx), C.byref(ty)))
+        return tx.value, ty.value
+
     def autotune(self, reps=0):
-        """Time every supported work-group shape and keep the fastest (FFT3D);
-        zeroes the field, so call it before set_field."""
+        """Time every supported work-group shape (FFT3D) or tile shape (ADI) and
+        keep the fastest; zeroes the field, so call it before set_field."""
         _check(lib().pxg_plan_autotune(self.h, reps))
-        return self.lines()
+        return self.adi_tiles() if self.prob.kind == ADI else self.lines()
 
     def set_field(self, field, b=0):
         order = "F" if self.is3d else "C"

@@ -374,3 +374,61 @@ def test_cn_custom_equation_vs_oracle(ctx, nx, nsteps, bc):
         plan.step(5, 1)
     assert e.value.code == P.ERR_INVALID
     plan.close()
+
+
+@pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_MIXED])
+@pytest.mark.parametrize("n", [300, 1000])
+def test_adi_tile_shapes_bitwise_and_autotune(ctx, prec, n):
+    """Every tile shape of both half-steps gives the same bits (each position
+    sees the same K-sweep iterate, the halo covers the sweeps), with the
+    transparent boundary, the reference's y-diagonal and a CSG phantom; n = 300
+    and 1000 leave partial tiles at the ends of the lines and of the line
+    groups.  The autotuner picks a valid shape, caches it per shape of problem,
+    and its result equals the oracle's."""
+    announce("adi tiles", n, prec)
+    extent, nsteps, dz, k = 20.0 * n, 3, 20.0, K_01
+    dx = extent / n
+    xmin = -extent / 2
+    s = extent / 4000.0
+    csg = spheres_scene(random_spheres(0x5EED0006, 8, (-1500 * s, 1500 * s), (-40, 40), (300 * s, 900 * s),
+                                       (1e-6, 1e-5)))
+    xs = xmin + dx * np.arange(n)
+    w = 800.0 * s
+    u0 = np.asfortranarray(np.exp(-(xs[:, None] ** 2 + (0.9 * xs[None, :]) ** 2) / w ** 2).astype(complex))
+    prob = P.problem(P.ADI, n, nsteps + 1, dx, dz, k, xmin=xmin, zmin=-40.0, use_tbc=1, precision=prec)
+    outs = {}
+    for tx in range(P.ADI_TILES):
+        for ty in range(P.ADI_TILES):
+            if tx != ty and tx != 0 and ty != 0:
+                continue   # every shape of each half-step, paired with the other's default
+            plan = P.Plan(ctx, prob)
+            plan.material_csg(*csg)
+            plan.set_adi_tiles(tx, ty)
+            assert plan.adi_tiles() == (tx, ty)
+            plan.set_field(u0)
+            plan.step(1, nsteps)
+            outs[(tx, ty)] = plan.get_field()
+            plan.close()
+    for key, out in outs.items():
+        np.testing.assert_array_equal(out, outs[(0, 0)], err_msg=str(key))
+    plan = P.Plan(ctx, prob)
+    with pytest.raises(P.PxgError) as e:
+        plan.set_adi_tiles(P.ADI_TILES, 0)
+    assert e.value.code == P.ERR_INVALID
+    plan.material_csg(*csg)
+    picked = plan.autotune(2)
+    announce("adi autotune picked", picked)
+    assert all(0 <= t < P.ADI_TILES for t in picked)
+    plan.set_field(u0)
+    plan.step(1, nsteps)
+    np.testing.assert_array_equal(plan.get_field(), outs[(0, 0)])
+    plan.close()
+    again = P.Plan(ctx, prob)
+    again.material_csg(*csg)
+    assert again.autotune(2) == picked     # cached per shape of problem
+    again.close()
+    g = O.grid(xmin, xmin + (n - 1) * dx, dx, -40.0, 40.0, dz, k)
+    ref = O.adi_run(g, O.CSGMaterial(*csg), u0, 1, nsteps, use_tbc=True)
+    err = rel_l2(outs[(0, 0)], ref)
+    announce("adi tiles vs oracle", err)
+    assert err < (1e-10 if prec == P.PREC_FP64 else 1e-5)
