@@ -92,7 +92,9 @@ class Plane(C.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # PAXORACLE_LIB names another build of paxoracle.c (the mutation
+        # tests load deliberately broken copies through it)
+        path = os.environ.get("PAXORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
         L = C.CDLL(path)
