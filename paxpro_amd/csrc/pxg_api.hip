@@ -146,6 +146,10 @@ struct pxg_plan_s {
   unsigned ff_twm_len = 0, ff_twn_len = 0;
   DBuf mcode, mxdb;  // FFT3D: material map of the current z-step (CSG materials)
   long map_step = -1;  // step whose map mcode holds (-1: none)
+  // FFT3D batches marched as two halves on two streams (split_fft3d): the
+  // second half's map and the step it holds
+  DBuf mcode2, mxdb2;
+  long map_step2 = -1;
   std::vector<double> k_host;
   // material
   int mat_mode = MAT_NONE;
@@ -224,6 +228,14 @@ size_t table_bytes(const pxg_problem &p, int esz)
   return b + 4096;
 }
 
+// FFT3D batches of 16 fields or more march as two halves on two streams
+// (run_fft3d_halves): power-of-two lines, even batch, no solution storage.
+bool fft3d_halves(const pxg_problem &p)
+{
+  return p.kind == PXG_FFT3D && fft_size_supported(p.nx) && p.batch >= 16 && p.batch % 2 == 0 &&
+         p.store_dsz <= 0;
+}
+
 size_t plan_bytes(const pxg_problem &p)
 {
   const int esz = field_c64(p) ? 8 : 16;
@@ -241,7 +253,8 @@ size_t plan_bytes(const pxg_problem &p)
       b += cells * p.nz * esz * p.batch;
     }
   }
-  if (p.kind == PXG_FFT3D) b += cells * (sizeof(uint16_t) + sizeof(cd));  // CSG material map
+  if (p.kind == PXG_FFT3D)  // CSG material map (one per half for batches marched in halves)
+    b += cells * (sizeof(uint16_t) + sizeof(cd)) * (fft3d_halves(p) ? 2 : 1);
   if (p.kind == PXG_ADI) b += cells * p.batch * 16 * 4;   // coefficient planes
   if (p.kind == PXG_CN) b += cells * p.batch * 16 * 6;
   return b;
@@ -547,10 +560,76 @@ void cube_store(pxg_plan P, unsigned st)
          ev0);
 }
 
+// The steps of run_fft3d for the two halves of the batch, each on a stream of
+// its own with its own material map (built by its field-0 work-groups).  The
+// halves' launches overlap: while one half's pass drains (its last work-groups
+// leave CUs idle) the other half's next pass fills them.  Every field sees the
+// same arithmetic as in one launch over the whole batch, so the results are
+// bitwise the same.  The plan stream forks to the second stream at the start
+// and joins it at the end, so work enqueued on the plan later stays ordered.
+template <class R>
+void run_fft3d_halves(pxg_plan P, unsigned first, unsigned nsteps)
+{
+  const pxg_problem &p = P->p;
+  const int n = p.nx, nb = p.batch / 2;
+  FftArgs<R> h[2] = {fft_args<R>(P), fft_args<R>(P)};
+  const size_t b0 = static_cast<size_t>(nb);
+  h[1].field += b0 * P->cells;
+  h[1].kx += b0 * n;
+  h[1].ky += b0 * n;
+  h[1].kvec += b0;
+  if (h[1].mat.part_fac)
+    h[1].mat.part_fac = static_cast<const cd *>(h[1].mat.part_fac) + b0 * (P->n_parts + 1);
+  h[1].mcode = P->mcode2.as<uint16_t>();
+  h[1].mxdb = P->mxdb2.as<cd>();
+  for (auto &a : h) a.map_in_pass_x = nb >= MAP_IN_PASS_X_BATCH;
+  if (!P->aux) {
+    hip_check(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&P->aux2, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming), "event");
+  }
+  const hipStream_t s[2] = {P->stream, P->aux};
+  hip_check(hipEventRecord(P->fork_ev, P->stream), "fork");
+  hip_check(hipStreamWaitEvent(P->aux, P->fork_ev, 0), "fork");
+  long ms[2] = {P->map_step, P->map_step2};
+  for (int i = 0; i < 2; ++i) {
+    h[i].mode = Y_FWD;
+    hip_check(launch_pass_y<R>(n, P->lines_y, h[i], nb, s[i]), "pass y");
+  }
+  for (unsigned st = first; st < first + nsteps; ++st) {
+    const bool last = st + 1 == first + nsteps;
+    for (int i = 0; i < 2; ++i) {
+      h[i].step = st;
+      h[i].map_prev_valid = ms[i] >= 0 && ms[i] + 1 == static_cast<long>(st);
+      ms[i] = -1;
+      if (P->mat_mode == MAT_CSG && !h[i].map_in_pass_x)
+        hip_check(launch_material_map<R>(n, h[i], s[i]), "material map");
+      hip_check(launch_pass_x<R>(n, P->lines_x, h[i], nb, s[i]), "pass x");
+      if (P->mat_mode == MAT_CSG) ms[i] = st;
+    }
+    for (int i = 0; i < 2; ++i) {
+      h[i].mode = Y_INV | (last ? 0 : Y_FWD);
+      hip_check(launch_pass_y<R>(n, P->lines_y, h[i], nb, s[i]), "pass y");
+    }
+  }
+  P->map_step = ms[0];
+  P->map_step2 = ms[1];
+  hip_check(hipEventRecord(P->join_ev, P->aux), "join");
+  hip_check(hipStreamWaitEvent(P->stream, P->join_ev, 0), "join");
+}
+
 template <class R>
 void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
 {
   const pxg_problem &p = P->p;
+  // batches in halves, except under profiling (one launch per pass is what it
+  // times) and graph capture (one stream)
+  if (fft3d_halves(p) && !P->profiling && !P->use_graphs && P->mcode2.p) {
+    run_fft3d_halves<R>(P, first, nsteps);
+    return;
+  }
+  P->map_step2 = -1;  // the second half's map is not kept up to date here
   FftArgs<R> a = fft_args<R>(P);
   const double pass_bytes = 2.0 * P->cells * p.batch * sizeof(cx<R>);
   const int n = p.nx;
@@ -1263,6 +1342,10 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     if (prob->kind == PXG_FFT3D) {
       P->mcode.alloc(P->cells * sizeof(uint16_t));
       P->mxdb.alloc(P->cells * sizeof(cd));
+      if (fft3d_halves(*prob)) {
+        P->mcode2.alloc(P->cells * sizeof(uint16_t));
+        P->mxdb2.alloc(P->cells * sizeof(cd));
+      }
     }
     if (prob->kind == PXG_FFT3D && fft_size_supported(prob->nx)) {
       // untuned defaults (pxg_plan_autotune measures): pass Y one wavefront of
@@ -1408,6 +1491,7 @@ int pxg_plan_reset(pxg_plan P, const pxg_problem *prob)
     P->eq_planes.release();
     P->eq_n = P->eq_first = P->eq_inv = 0;
     P->map_step = -1;
+    P->map_step2 = -1;
     P->field_set = false;
     P->jobs_dirty = true;
     rebuild_tables(P);
@@ -1613,6 +1697,7 @@ int pxg_set_material_none(pxg_plan P)
     check_plan(P);
     P->mat_mode = MAT_NONE;
     P->map_step = -1;
+    P->map_step2 = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1682,6 +1767,7 @@ int pxg_set_material_csg(pxg_plan P, const pxg_shape *shapes, int n_shapes, cons
     P->is_ref = is_reference_run;
     P->mat_mode = MAT_CSG;
     P->map_step = -1;
+    P->map_step2 = -1;
     P->jobs_dirty = true;
     if (P->p.kind == PXG_FFT3D) build_part_factors(P);
   });
@@ -1698,6 +1784,7 @@ int pxg_set_material_curved_wg(pxg_plan P, const pxg_curved_wg *wg)
                     wg->wglength, wg->dx, wg->dz, wg->use_smoothed};
     P->mat_mode = MAT_CURVED;
     P->map_step = -1;
+    P->map_step2 = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1716,6 +1803,7 @@ int pxg_set_material_planes(pxg_plan P, const double *db, int n_planes, int firs
     P->z_invariant = z_invariant;
     P->mat_mode = MAT_PLANES;
     P->map_step = -1;
+    P->map_step2 = -1;
     P->jobs_dirty = true;
   });
 }
@@ -1830,6 +1918,7 @@ int pxg_plan_autotune(pxg_plan P, int reps)
     hip_check(hipStreamSynchronize(P->stream), "sync");
     P->field_set = false;
     P->map_step = -1;  // the timing runs wrote maps of another step
+    P->map_step2 = -1;
     std::lock_guard<std::mutex> lk(P->ctx->mu);
     P->ctx->tuned[key] = {P->lines_x, P->lines_y};
   });

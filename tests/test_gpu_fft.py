@@ -327,3 +327,45 @@ def test_fft3d_graph_replays_equal_eager(ctx, batch, prec):
     assert all(np.array_equal(a, r) for a, r in zip(out[0], ref[0]))
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("prec,batch", [(P.PREC_FP64, 16), (P.PREC_MIXED, 16), (P.PREC_MIXED, 64)])
+def test_fft3d_batch_halves_on_two_streams(ctx, prec, batch):
+    """Batches of 16 fields or more march as two halves on two streams, each half
+    with its own material map; a profiled call marches the whole batch in one
+    launch per pass.  Calls that alternate between the two (steps 1-4 halves,
+    5-7 one launch, 8-12 halves) give, field by field, the same bits as the same
+    three calls with one launch per pass (the calls' boundaries stay the same:
+    each call ends with the field in real space, which rounds complex64 fields),
+    and the FP64 fields match the oracle."""
+    n, nsteps = 128, 12
+    announce("fft3d batch halves", n, nsteps, "precision", prec, "batch", batch)
+    g = fft3d_case(n, dz=200.0, nsteps=nsteps)
+    csg = spheres_scene(random_spheres(0x5EED0002, 24, (-8000, 8000), (-1500, 1500), (500, 3000), (1e-6, 1e-5)))
+    ks = [K_01 * (1 + 0.01 * b) for b in range(batch)]
+    u0 = gaussian_field(g, n, 5000.0)
+    runs = []
+    for mode in ("alternating", "one launch"):
+        plan = P.Plan(ctx, fft3d_problem(g, n, nsteps + 1, precision=prec, batch=batch))
+        plan.set_wavenumbers(ks)
+        plan.material_csg(*csg)
+        for b in range(batch):
+            plan.set_field(u0, b)
+        if mode == "alternating":
+            plan.step(1, 4)
+            plan.step_profiled(5, 3)
+            plan.step(8, 5)
+        else:
+            plan.step_profiled(1, 4)
+            plan.step_profiled(5, 3)
+            plan.step_profiled(8, 5)
+        plan.sync()
+        runs.append([plan.get_field(b) for b in range(batch)])
+        plan.close()
+    for b in range(batch):
+        assert np.array_equal(runs[0][b], runs[1][b]), b
+    if prec == P.PREC_FP64:
+        for b in (0, batch // 2, batch - 1):
+            gb = fft3d_case(n, dz=200.0, nsteps=nsteps, k=ks[b])
+            ref = O.fft3d_run(gb, O.CSGMaterial(*csg), u0, 1, nsteps)
+            assert rel_l2(runs[0][b], ref) < 1e-11, b
