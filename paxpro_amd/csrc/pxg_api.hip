@@ -236,6 +236,15 @@ bool fft3d_halves(const pxg_problem &p)
          p.store_dsz <= 0;
 }
 
+// FFT3D plans whose map has launches of its own (fewer fields than pass X
+// builds it for) keep MAP_RING maps: consecutive steps' maps are built in one
+// launch (run_fft3d).  Slot 0 is the single map of the other paths.
+constexpr int MAP_IN_PASS_X_BATCH = 8;  // fields from which pass X builds the map
+int map_slots(const pxg_problem &p)
+{
+  return p.kind == PXG_FFT3D && fft_size_supported(p.nx) && p.batch < MAP_IN_PASS_X_BATCH ? MAP_RING : 1;
+}
+
 size_t plan_bytes(const pxg_problem &p)
 {
   const int esz = field_c64(p) ? 8 : 16;
@@ -253,8 +262,8 @@ size_t plan_bytes(const pxg_problem &p)
       b += cells * p.nz * esz * p.batch;
     }
   }
-  if (p.kind == PXG_FFT3D)  // CSG material map (one per half for batches marched in halves)
-    b += cells * (sizeof(uint16_t) + sizeof(cd)) * (fft3d_halves(p) ? 2 : 1);
+  if (p.kind == PXG_FFT3D)  // CSG material maps: one per half (batch halves), MAP_RING (maps ahead), else one
+    b += cells * (sizeof(uint16_t) + sizeof(cd)) * (fft3d_halves(p) ? 2 : map_slots(p));
   if (p.kind == PXG_ADI) b += cells * p.batch * 16 * 4;   // coefficient planes
   if (p.kind == PXG_CN) b += cells * p.batch * 16 * 6;
   return b;
@@ -503,8 +512,6 @@ int begin(pxg_plan P, hipStream_t s = nullptr)
   return ev0;
 }
 
-constexpr int MAP_IN_PASS_X_BATCH = 8;  // fields from which pass X builds the map
-
 template <class R>
 FftArgs<R> fft_args(pxg_plan P)
 {
@@ -529,6 +536,7 @@ FftArgs<R> fft_args(pxg_plan P)
   a.n_refr = p.n_refr ? p.n_refr : 10;
   a.quirks = p.quirks;
   a.mcode = P->mcode.as<uint16_t>();
+  a.mprev = a.mcode;
   a.mxdb = P->mxdb.as<cd>();
   // the map is computed once per step for the batch: inside pass X when the
   // batch gives it many work-groups to spread over, else in its own launch
@@ -633,6 +641,12 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
   FftArgs<R> a = fft_args<R>(P);
   const double pass_bytes = 2.0 * P->cells * p.batch * sizeof(cx<R>);
   const int n = p.nx;
+  // Maps of their own launches, MAP_RING steps per launch: the parts are culled
+  // once for the steps of a launch and each step's z0 plane is the previous
+  // step's z1 plane in registers.  Step st reads slot (st - first) mod MAP_RING.
+  const bool ring = P->mat_mode == MAT_CSG && !a.map_in_pass_x && !P->blue_m && map_slots(p) == MAP_RING;
+  uint16_t *const mc0 = a.mcode;
+  cd *const mx0 = a.mxdb;
   a.mode = Y_FWD;
   int ev0 = begin(P);
   record(P, "pass_y_fwd", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
@@ -642,14 +656,33 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
     a.step = st;  // the material map of this step (CSG) is built before pass Y
     a.map_prev_valid = P->map_step >= 0 && P->map_step + 1 == static_cast<long>(st);
     P->map_step = -1;  // until this step's map is launched
-    if (P->mat_mode == MAT_CSG && !a.map_in_pass_x) {
+    if (ring) {
+      const unsigned k = (st - first) % MAP_RING;
+      if (k == 0) {
+        const int nz = static_cast<int>(std::min<unsigned>(MAP_RING, first + nsteps - st));
+        FftArgs<R> m = a;
+        m.mcode = mc0;
+        m.mxdb = mx0;
+        // the previous step's map: slot 0 before this call (the single-map
+        // paths), else the last slot of the previous launch (read by each lane
+        // for its cells before any write of this launch)
+        m.mprev = st == first ? mc0 : mc0 + (MAP_RING - 1) * P->cells;
+        m.map_prev_valid = st == first ? a.map_prev_valid : 1;
+        ev0 = begin(P);
+        record(P, "material_map", static_cast<double>(P->cells) * nz * sizeof(uint16_t),
+               launch_material_map_planes<R>(n, m, nz, P->stream), ev0);
+      }
+      a.mcode = mc0 + k * P->cells;
+      a.mxdb = mx0 + k * P->cells;
+    } else if (P->mat_mode == MAT_CSG && !a.map_in_pass_x) {
       ev0 = begin(P);
       record(P, "material_map", static_cast<double>(P->cells) * sizeof(uint16_t),
              launch_material_map<R>(n, a, P->stream), ev0);
     }
     ev0 = begin(P);
     record(P, "pass_x", pass_bytes, launch_pass_x<R>(n, P->lines_x, a, p.batch, P->stream), ev0);
-    if (P->mat_mode == MAT_CSG) P->map_step = st;
+    // slot 0 holds this step's map (the single-map paths continue from it)
+    if (P->mat_mode == MAT_CSG && (!ring || a.mcode == mc0)) P->map_step = st;
     a.mode = Y_INV | ((last || store) ? 0 : Y_FWD);
     ev0 = begin(P);
     record(P, "pass_y", pass_bytes, launch_pass_y<R>(n, P->lines_y, a, p.batch, P->stream), ev0);
@@ -1340,8 +1373,8 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
     hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
     P->k_host.assign(prob->batch, prob->k);
     if (prob->kind == PXG_FFT3D) {
-      P->mcode.alloc(P->cells * sizeof(uint16_t));
-      P->mxdb.alloc(P->cells * sizeof(cd));
+      P->mcode.alloc(P->cells * sizeof(uint16_t) * map_slots(*prob));
+      P->mxdb.alloc(P->cells * sizeof(cd) * map_slots(*prob));
       if (fft3d_halves(*prob)) {
         P->mcode2.alloc(P->cells * sizeof(uint16_t));
         P->mxdb2.alloc(P->cells * sizeof(cd));

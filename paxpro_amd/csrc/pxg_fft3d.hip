@@ -235,7 +235,7 @@ constexpr int map_scratch_bytes(int lanes)
 template <class R>
 __device__ __forceinline__ void map_border_cells(const FftArgs<R> &a, const uint64_t *mask, int nw, int n,
                                                  const ushort4 *list, int nb, cd *vals, int lanes, double z0,
-                                                 double z1)
+                                                 double z1, cd *mxdb)
 {
   const int lane = static_cast<int>(__lane_id());
   const int last = (a.quirks & 2u) ? static_cast<int>(a.n_refr) : static_cast<int>(a.n_refr) - 1;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void map_border_cells(const FftArgs<R> &a, const uint
       double d, b;
       crossing_integral<1>(a.mat, mask, nw, a.n_refr, a.quirks, coord(a.xmin, en.x, a.dx),
                            coord(a.ymin, en.y, a.dy), z0, z1, d, b);
-      a.mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{d, b};
+      mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{d, b};
     }
     return;
   }
@@ -283,7 +283,7 @@ __device__ __forceinline__ void map_border_cells(const FftArgs<R> &a, const uint
       }
       delta /= (2.0 * a.n_refr);
       beta /= (2.0 * a.n_refr);
-      a.mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{delta, beta};
+      mxdb[static_cast<size_t>(en.x) * n + en.y] = cd{delta, beta};
     }
     lds_sync<true>();
   }
@@ -361,7 +361,104 @@ __device__ __forceinline__ void material_map_cells(const FftArgs<R> &a, uint64_t
   }
   if (nb > 0) {
     lds_sync<true>();
-    map_border_cells(a, mask, nw, n, list, nb, vals, lanes, z0, z1);
+    map_border_cells(a, mask, nw, n, list, nb, vals, lanes, z0, z1, a.mxdb);
+  }
+}
+
+// The maps of the nz z-steps a.step .. a.step + nz - 1 in one pass over the
+// cells: step a.step + k goes to mcode + k n^2 and mxdb + k n^2.  The parts are
+// culled once against the block over the whole z range, and each step's z0
+// classification is the previous step's z1 classification, kept in registers
+// (the first step's from mprev when map_prev_valid, else evaluated).  Culling a
+// superset of a step's parts changes no classification (the first part whose
+// shapes contain a point decides, and a part away from the block contains none
+// of its points), so every code and border value is the one material_map_cells
+// computes for that step.
+template <int CPL, class R, class CellOf>
+__device__ __forceinline__ void material_map_planes(const FftArgs<R> &a, uint64_t *mask, int n, int xlo, int xhi,
+                                                    int ylo, int yhi, CellOf cell, char *wscr, int lanes, int nz)
+{
+  constexpr int K = CPL < 2 ? CPL : 2;
+  const size_t plane = static_cast<size_t>(n) * n;
+  const double zbegin = coord(a.zmin, a.step - 1, a.dz);
+  const double zend = coord(a.zmin, a.step + nz - 1, a.dz);
+  int e0[CPL];  // classification at the current step's z0
+  if (a.map_prev_valid) {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int2 cr = cell(q);
+      e0[q] = a.mprev[static_cast<size_t>(cr.x) * n + cr.y] & (MAP_CROSSING - 1u);
+    }
+  }
+  int nw = (a.mat.n_parts + 63) >> 6;
+  csg_cull(a.mat, mask, nw, coord(a.xmin, xlo, a.dx), coord(a.xmin, xhi, a.dx), coord(a.ymin, ylo, a.dy),
+           coord(a.ymin, yhi, a.dy), fmin(zbegin, zend), fmax(zbegin, zend));
+  bool any = false;
+  for (int w = 0; w < nw; ++w) any |= mask[w] != 0;
+  if (!any || a.mat.is_ref) nw = 0;  // no part near this block: every cell is vacuum
+  if (!a.map_prev_valid) {
+#pragma unroll 1
+    for (int m0 = 0; m0 < CPL; m0 += K) {
+      double xv[K], yv[K];
+      int ez[K], unused[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int2 cr = cell(m0 + q);
+        xv[q] = coord(a.xmin, cr.x, a.dx);
+        yv[q] = coord(a.ymin, cr.y, a.dy);
+      }
+      csg_eval_x2<K, false>(a.mat, mask, nw, xv, yv, zbegin, zbegin, ez, unused);
+#pragma unroll
+      for (int q = 0; q < K; ++q) e0[m0 + q] = ez[q];
+    }
+  }
+  ushort4 *list = reinterpret_cast<ushort4 *>(wscr);
+  cd *vals = reinterpret_cast<cd *>(wscr + lanes * CPL * 8);
+#pragma unroll 1
+  for (int k = 0; k < nz; ++k) {
+    const double z1 = coord(a.zmin, a.step + k, a.dz);
+    const double z0 = coord(a.zmin, a.step + k - 1, a.dz);
+    uint16_t *mcode = a.mcode + k * plane;
+    int nb = 0;  // border cells listed so far (uniform)
+#pragma unroll
+    for (int m0 = 0; m0 < CPL; m0 += K) {
+      double xv[K], yv[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int2 cr = cell(m0 + q);
+        xv[q] = coord(a.xmin, cr.x, a.dx);
+        yv[q] = coord(a.ymin, cr.y, a.dy);
+      }
+      int e1[K], unused[K];
+      csg_eval_x2<K, false>(a.mat, mask, nw, xv, yv, z1, z1, e1, unused);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        unsigned code = static_cast<unsigned>(e1[q]);
+        bool border = false;
+        if (e1[q] != e0[m0 + q]) {
+          double d1, b1, d0, b0;
+          part_values(a.mat, e1[q], d1, b1);
+          part_values(a.mat, e0[m0 + q], d0, b0);
+          border = fabs(d1 - d0) > 1e-10 || fabs(b1 - b0) > 1e-10;
+        }
+        const int2 cr = cell(m0 + q);
+        const uint64_t bal = __ballot(border);
+        if (border) {
+          code |= MAP_CROSSING;
+          const int slot = nb + __popcll(bal & ((1ull << __lane_id()) - 1ull));
+          list[slot] = make_ushort4(static_cast<unsigned short>(cr.x), static_cast<unsigned short>(cr.y),
+                                    static_cast<unsigned short>(e1[q]), static_cast<unsigned short>(e0[m0 + q]));
+        }
+        nb += __popcll(bal);
+        mcode[static_cast<size_t>(cr.x) * n + cr.y] = static_cast<uint16_t>(code);
+        e0[m0 + q] = e1[q];
+      }
+    }
+    if (nb > 0) {
+      lds_sync<true>();
+      map_border_cells(a, mask, nw, n, list, nb, vals, lanes, z0, z1, a.mxdb + k * plane);
+    }
+    lds_sync<true>();  // the next step's list reuses the scratch
   }
 }
 
@@ -378,6 +475,20 @@ __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
   const int r = r0 + static_cast<int>(threadIdx.x);
   material_map_cells<CPL>(a, mask, n, c, c, r0, r0 + lanes * CPL - 1,
                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes);
+}
+
+// The maps of nz consecutive steps in one launch (material_map_planes), with the
+// strips of k_material_map.
+template <class R, int CPL>
+__global__ __launch_bounds__(64) void k_material_map_planes(FftArgs<R> a, int n, int nz)
+{
+  __shared__ uint64_t mask[MAX_MASK_WORDS];
+  __shared__ __align__(16) char scr[map_scratch_bytes<CPL>(64)];
+  const int lanes = static_cast<int>(blockDim.x);
+  const int c = blockIdx.y, r0 = blockIdx.x * lanes * CPL;
+  const int r = r0 + static_cast<int>(threadIdx.x);
+  material_map_planes<CPL>(a, mask, n, c, c, r0, r0 + lanes * CPL - 1,
+                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes, nz);
 }
 
 // The same for the members of a group, blockIdx.z = member (members without a
@@ -643,6 +754,20 @@ hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s)
 }
 
 template <class R>
+hipError_t launch_material_map_planes(int n, const FftArgs<R> &a, int nz, hipStream_t s)
+{
+  // the strips of launch_material_map (power-of-two n only)
+  if (!fft_size_supported(n) || nz < 1 || nz > MAP_RING) return hipErrorInvalidValue;
+  if (n >= 128) {
+    hipLaunchKernelGGL((k_material_map_planes<R, 2>), dim3(n / 128, n), dim3(64), 0, s, a, n, nz);
+  } else {
+    const int rows = n < 64 ? n : 64;
+    hipLaunchKernelGGL((k_material_map_planes<R, 1>), dim3(n / rows, n), dim3(rows), 0, s, a, n, nz);
+  }
+  return hipGetLastError();
+}
+
+template <class R>
 hipError_t launch_pass_y_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s)
 {
   auto grid_of = [&](int nn, int ll) { return dim3(nn / ll, members); };
@@ -697,6 +822,8 @@ template hipError_t launch_pass_x<float>(int, int, const FftArgs<float> &, int, 
 template hipError_t launch_pass_x<double>(int, int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_material_map<float>(int, const FftArgs<float> &, hipStream_t);
 template hipError_t launch_material_map<double>(int, const FftArgs<double> &, hipStream_t);
+template hipError_t launch_material_map_planes<float>(int, const FftArgs<float> &, int, hipStream_t);
+template hipError_t launch_material_map_planes<double>(int, const FftArgs<double> &, int, hipStream_t);
 template hipError_t launch_pass_y_group<float>(int, int, const GroupArgs<float> &, int, hipStream_t);
 template hipError_t launch_pass_y_group<double>(int, int, const GroupArgs<double> &, int, hipStream_t);
 template hipError_t launch_pass_x_group<float>(int, int, const GroupArgs<float> &, int, hipStream_t);

@@ -32,6 +32,7 @@ struct FftArgs {
   // FFT3D CSG material map of the current z-step, shared by the batch (built
   // by pass X, read by pass Y):
   uint16_t *mcode;         // [N*N] deciding part at z1; bit 15: border crossing
+  const uint16_t *mprev;   // the previous step's codes (the map's z0 plane when map_prev_valid)
   cd *mxdb;                // [N*N] (delta, beta) of the crossing integral, border cells only
   int map_in_pass_x;       // build the map inside pass X (large batches) or in its own launch
   int map_prev_valid;      // mcode holds the map of step - 1: reuse its parts as the z0 plane
@@ -67,6 +68,11 @@ template <class R>
 hipError_t launch_pass_x(int n, int lines, const FftArgs<R> &a, int batch, hipStream_t s);
 template <class R>
 hipError_t launch_material_map(int n, const FftArgs<R> &a, hipStream_t s);
+// The maps of nz <= MAP_RING consecutive steps a.step .. a.step + nz - 1 in one
+// launch, step a.step + k into a.mcode + k n^2 and a.mxdb + k n^2 (power-of-two n)
+constexpr int MAP_RING = 8;
+template <class R>
+hipError_t launch_material_map_planes(int n, const FftArgs<R> &a, int nz, hipStream_t s);
 // The same three launches over the members of a group (grid y or z = member)
 template <class R>
 hipError_t launch_pass_y_group(int n, int lines, const GroupArgs<R> &a, int members, hipStream_t s);

@@ -369,3 +369,27 @@ def test_fft3d_batch_halves_on_two_streams(ctx, prec, batch):
             gb = fft3d_case(n, dz=200.0, nsteps=nsteps, k=ks[b])
             ref = O.fft3d_run(gb, O.CSGMaterial(*csg), u0, 1, nsteps)
             assert rel_l2(runs[0][b], ref) < 1e-11, b
+
+
+def test_fft3d_map_ring_across_call_splits(ctx):
+    """Plans of fewer than 8 fields build the material maps of up to MAP_RING
+    consecutive steps in one launch (the parts culled once, each step's z0
+    plane the previous step's z1 plane).  Calls of 3, 9, 8 and 1 steps (groups
+    cut by the calls, the previous map reused or rebuilt at each call) over 24
+    spheres with border crossings match the oracle's single march."""
+    n, dz = 128, 200.0
+    calls = [(1, 3), (4, 9), (13, 8), (21, 1)]
+    nsteps = sum(k for _, k in calls)
+    announce("fft3d map ring across calls", n, nsteps)
+    g = fft3d_case(n, dz=dz, nsteps=nsteps)
+    csg = spheres_scene(random_spheres(0x5EED0002, 24, (-8000, 8000), (-1500, 1500), (500, 3000), (1e-6, 1e-5)))
+    u0 = gaussian_field(g, n, 5000.0)
+    plan = P.Plan(ctx, fft3d_problem(g, n, nsteps + 1))
+    plan.material_csg(*csg)
+    plan.set_field(u0)
+    for first, k in calls:
+        plan.step(first, k)
+    out = plan.get_field()
+    plan.close()
+    ref = O.fft3d_run(g, O.CSGMaterial(*csg), u0, 1, nsteps)
+    assert rel_l2(out, ref) < 1e-11
