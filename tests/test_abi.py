@@ -52,3 +52,22 @@ def test_null_arguments_are_rejected():
     assert L.pxg_plan_create(None, None, None) == P.ERR_INVALID
     assert L.pxg_step(None, 1, 1) == P.ERR_INVALID
     assert b"null" in L.pxg_last_error()
+
+
+def test_plan_bytes_count_the_material_maps():
+    """FFT3D plans reserve one material map (2 + 16 B per cell) for batches whose
+    map pass X builds (8 to 15 fields), two for batches marched in halves (16 or
+    more, even), and a ring of 8 for fewer than 8 fields (maps of 8 steps per
+    launch); lengths without a power-of-two kernel keep one."""
+    def b(n, batch):
+        return P.plan_bytes(P.problem(P.FFT3D, n, 101, 20000.0 / n, 10.0, 62.83, batch=batch,
+                                      precision=P.PREC_MIXED))
+    for n in (256, 512):
+        per_map = n * n * (2 + 16)
+        per_field = b(n, 9) - b(n, 8)
+        assert per_field > 0
+        assert b(n, 16) - b(n, 8) == 8 * per_field + per_map
+        assert b(n, 7) - b(n, 8) == -per_field + 7 * per_map
+        assert b(n, 1) - b(n, 8) == -7 * per_field + 7 * per_map
+    per_field_511 = b(511, 9) - b(511, 8)
+    assert b(511, 7) - b(511, 8) == -per_field_511
