@@ -1,5 +1,5 @@
-// Sources (include/paraxialSource.hpp, gaussianBeam.hpp, planeWave.hpp of the
-// reference).
+// The source base class (include/paraxialSource.hpp of the reference); the
+// Gaussian beam and the plane wave have headers of their own.
 #pragma once
 
 #include <complex>
@@ -33,35 +33,4 @@ class ParaxialSource {
 
  private:
   std::string name;
-};
-
-class GaussianBeam : public ParaxialSource {
- public:
-  GaussianBeam() : ParaxialSource("gaussianBeam") {}
-  cdouble get(double x, double z) const override final;
-  cdouble get(double x, double y, double z) const override final;
-  void setWaist(double w) { waist = w; }
-  void setCenter(double xc, double yc) { centerX = xc; centerY = yc; }
-  void setAngleX(double ang) { angleX = ang; }
-  void setAngleY(double ang) { angleY = ang; }
-  double beamDivergence() const;
-  double getWaist() const { return waist; }
-
- private:
-  double waist{1.0}, centerX{0.0}, centerY{0.0}, angleX{0.0}, angleY{0.0};
-  double rayleighRange() const;
-  double guoyPhase(double z) const;
-  double inverseRadiusOfCurvature(double z) const;
-  double spotSize(double z) const;
-};
-
-class PlaneWave : public ParaxialSource {
- public:
-  PlaneWave() : ParaxialSource("planeWave") {}
-  cdouble get(double x, double z) const override final;
-  cdouble get(double x, double y, double z) const override final;
-  void setAngleDeg(double angle);
-
- private:
-  double angleWithZAxis{0.0};
 };
