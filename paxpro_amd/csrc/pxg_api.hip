@@ -600,7 +600,17 @@ void run_fft3d_halves(pxg_plan P, unsigned first, unsigned nsteps)
   const hipStream_t s[2] = {P->stream, P->aux};
   hip_check(hipEventRecord(P->fork_ev, P->stream), "fork");
   hip_check(hipStreamWaitEvent(P->aux, P->fork_ev, 0), "fork");
+  // the plan stream joins the second stream on every exit, a failed launch
+  // included, so later work on the plan stays ordered behind what was enqueued
+  struct Join {
+    pxg_plan P;
+    ~Join()
+    {
+      if (hipEventRecord(P->join_ev, P->aux) == hipSuccess) (void)hipStreamWaitEvent(P->stream, P->join_ev, 0);
+    }
+  } join{P};
   long ms[2] = {P->map_step, P->map_step2};
+  P->map_step = P->map_step2 = -1;  // until the march has enqueued every step
   for (int i = 0; i < 2; ++i) {
     h[i].mode = Y_FWD;
     hip_check(launch_pass_y<R>(n, P->lines_y, h[i], nb, s[i]), "pass y");
@@ -623,8 +633,6 @@ void run_fft3d_halves(pxg_plan P, unsigned first, unsigned nsteps)
   }
   P->map_step = ms[0];
   P->map_step2 = ms[1];
-  hip_check(hipEventRecord(P->join_ev, P->aux), "join");
-  hip_check(hipStreamWaitEvent(P->stream, P->join_ev, 0), "join");
 }
 
 template <class R>
