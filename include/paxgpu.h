@@ -146,11 +146,12 @@ int pxg_plan_set_lines(pxg_plan plan, int lines_x, int lines_y);
 int pxg_plan_get_lines(pxg_plan plan, int *lines_x, int *lines_y);
 /* Tile shapes of the ADI half-step kernels (tile-Jacobi path): index of the
  * x-implicit shape (lines x positions: 0: 16x128, 1: 8x256, 2: 16x256, 3: 32x64,
- * 4: 32x128) and of the y-implicit shape (0: 4x256, 1: 1x256, 2: 2x256, 3: 8x256,
- * 4: 2x512); -1 keeps the current one.  Results do not depend on them.  ADI plans
- * only.  No reference counterpart (src/alternatingDirectionSolver.cpp:8-134
- * sweeps one line at a time). */
-#define PXG_ADI_TILES 5
+ * 4: 32x128, 5: 8x128) and of the y-implicit shape (0: 4x256, 1: 1x256, 2: 2x256,
+ * 3: 8x256, 4: 2x512, 5: 1x2048, the whole line up to 2048 points); -1 keeps the
+ * current one.  Results do not depend on them.  ADI plans only.  No reference
+ * counterpart (src/alternatingDirectionSolver.cpp:8-134 sweeps one line at a
+ * time). */
+#define PXG_ADI_TILES 6
 int pxg_plan_set_adi_tiles(pxg_plan plan, int x_tile, int y_tile);
 int pxg_plan_get_adi_tiles(pxg_plan plan, int *x_tile, int *y_tile);
 /* Autotuner, in the manner of FFTW's measured planning: times every supported

@@ -38,8 +38,10 @@ constexpr int ADI_MASK_WORDS = 16;
 
 // A tile of TL_ lines x TP_ positions.  The shapes built (index = the plan's tile
 // choice, pxg_plan_set_adi_tiles): x-implicit (16, 128) (8, 256) (16, 256)
-// (32, 64) (32, 128); y-implicit (4, 256) (1, 256) (2, 256) (8, 256) (2, 512).
-// Every shape gives the same bits (each position sees the same K-sweep iterate).
+// (32, 64) (32, 128) (8, 128); y-implicit (4, 256) (1, 256) (2, 256) (8, 256)
+// (2, 512) (1, 2048).  Every shape gives the same bits (each position sees the
+// same K-sweep iterate).  A tile as long as the line or longer covers it whole:
+// it needs no halo, and its lines are not cut into segments.
 template <bool IMPX, int TL_, int TP_>
 struct AdiTile {
   static constexpr int TL = TL_;              // lines per tile
@@ -118,7 +120,7 @@ template <class R, bool IMPX, int TL, int TP>
 __global__ __launch_bounds__((AdiTile<IMPX, TL, TP>::LANES)) void k_adi_jacobi(TriArgs<R> a, AdiCoef c,
                                                                             const cx<R> *__restrict__ prev,
                                                                             cx<R> *__restrict__ next, int iters,
-                                                                            int nseg)
+                                                                            int halo, int nseg)
 {
 #pragma clang fp contract(off)  // fused operations below are written out
   using T = AdiTile<IMPX, TL, TP>;
@@ -131,7 +133,6 @@ __global__ __launch_bounds__((AdiTile<IMPX, TL, TP>::LANES)) void k_adi_jacobi(T
   const int g = IMPX ? tid / T::TL : tid % T::G;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int seg = tile % nseg, grp = tile / nseg;
-  const int halo = iters;
   const int j0 = seg * (T::TP - 2 * halo) - halo;  // position of tile slot 0
   const int jb = j0 + g * C;                        // this lane's first position
   const int line = grp * T::TL + l;
@@ -293,7 +294,9 @@ template <class R, bool IMPX, int TL, int TP>
 static hipError_t launch_tiles(const TriArgs<R> &a, const cx<R> *prev, cx<R> *next, int iters, hipStream_t s)
 {
   using T = AdiTile<IMPX, TL, TP>;
-  const int stride = T::TP - 2 * iters;
+  // a tile that covers the whole line needs no halo: nothing lies beyond its ends
+  const int halo = T::TP >= a.n ? 0 : iters;
+  const int stride = T::TP - 2 * halo;
   const int nseg = (a.n + stride - 1) / stride;
   const int ngrp = (a.n + T::TL - 1) / T::TL;
   // the same expressions as the reference's coefficients, once per launch
@@ -311,11 +314,11 @@ static hipError_t launch_tiles(const TriArgs<R> &a, const cx<R> *prev, cx<R> *ne
   c.o = -0.5 / c.dimp;
   c.ddy = ddy;
   hipLaunchKernelGGL((k_adi_jacobi<R, IMPX, TL, TP>), dim3(nseg * ngrp), dim3(T::LANES), 0, s, a, c, prev, next,
-                     iters, nseg);
+                     iters, halo, nseg);
   return hipGetLastError();
 }
 
-int adi_tile_count() { return 5; }
+int adi_tile_count() { return 6; }
 
 template <class R>
 hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *prev, cx<R> *next,
@@ -329,6 +332,7 @@ hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *
       case 2: return launch_tiles<R, true, 16, 256>(a, prev, next, iters, s);
       case 3: return launch_tiles<R, true, 32, 64>(a, prev, next, iters, s);
       case 4: return launch_tiles<R, true, 32, 128>(a, prev, next, iters, s);
+      case 5: return launch_tiles<R, true, 8, 128>(a, prev, next, iters, s);
       default: return hipErrorInvalidValue;
     }
   }
@@ -338,6 +342,7 @@ hipError_t launch_adi_jacobi(const TriArgs<R> &a, bool implicit_x, const cx<R> *
     case 2: return launch_tiles<R, false, 2, 256>(a, prev, next, iters, s);
     case 3: return launch_tiles<R, false, 8, 256>(a, prev, next, iters, s);
     case 4: return launch_tiles<R, false, 2, 512>(a, prev, next, iters, s);
+    case 5: return launch_tiles<R, false, 1, 2048>(a, prev, next, iters, s);
     default: return hipErrorInvalidValue;
   }
 }

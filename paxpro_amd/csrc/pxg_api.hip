@@ -1398,6 +1398,9 @@ int pxg_plan_create(pxg_ctx ctx, const pxg_problem *prob, pxg_plan *out)
       while (P->lines_y > 1 && (prob->nx / P->lines_y) * prob->batch < 512) P->lines_y /= 2;
       while (P->lines_x > 4 && (prob->nx / P->lines_x) * prob->batch < 512) P->lines_x /= 2;
     }
+    // untuned x-implicit ADI tile: 128-byte rows of a line group, 16 lines of
+    // complex64 (shape 0) or 8 of complex128 (shape 5: 8 x 128)
+    if (prob->kind == PXG_ADI && prob->precision == PXG_PREC_FP64) P->adi_tile[0] = 5;
     if (prob->kind == PXG_FFT3D && prob->batch == 1) {
       hip_check(hipEventCreateWithFlags(&P->ready_ev, hipEventDisableTiming), "event");
       hip_check(hipEventCreateWithFlags(&P->group_done_ev, hipEventDisableTiming), "event");

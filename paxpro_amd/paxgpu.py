@@ -235,7 +235,7 @@ def plan_bytes(prob):
 
 
 MAX_GROUP = 8
-ADI_TILES = 5
+ADI_TILES = 6
 SERVER_GROUP_MAX_NX = 512   # the server's default: larger grids fill the device alone
 
 
