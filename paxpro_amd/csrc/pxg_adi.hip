@@ -218,7 +218,9 @@ __global__ __launch_bounds__((AdiTile<IMPX, TL, TP>::LANES)) void k_adi_jacobi(T
   }
   // Transparent boundary rows, after the loaded values are consumed: the few
   // lanes holding boundary cells read their neighbours again.
-  if (a.use_tbc) {
+  // (a work-group-uniform test first: only tiles at the grid's edges hold such cells)
+  const bool edge_tile = j0 <= 0 || j0 + T::TP >= N || grp == 0 || (grp + 1) * T::TL >= N;
+  if (a.use_tbc && edge_tile) {
 #pragma unroll
     for (int m = 0; m < C; ++m) {
       const int j = jb + m;
