@@ -568,6 +568,25 @@ void cube_store(pxg_plan P, unsigned st)
          ev0);
 }
 
+// The plan's second and third streams (default priority) and their fork and
+// join events, each made on first use; a failed creation leaves the handles
+// made so far, and the next call makes the rest.
+void make_aux_streams(pxg_plan P)
+{
+  for (hipStream_t *x : {&P->aux, &P->aux2})
+    if (!*x) {
+      hipStream_t made = nullptr;
+      hip_check(hipStreamCreateWithFlags(&made, hipStreamNonBlocking), "hipStreamCreate");
+      *x = made;
+    }
+  for (hipEvent_t *e : {&P->fork_ev, &P->join_ev, &P->join2_ev})
+    if (!*e) {
+      hipEvent_t made = nullptr;
+      hip_check(hipEventCreateWithFlags(&made, hipEventDisableTiming), "event");
+      *e = made;
+    }
+}
+
 // The steps of run_fft3d for the two halves of the batch, each on a stream of
 // its own with its own material map (built by its field-0 work-groups).  The
 // halves' launches overlap: while one half's pass drains (its last work-groups
@@ -591,12 +610,7 @@ void run_fft3d_halves(pxg_plan P, unsigned first, unsigned nsteps)
   h[1].mcode = P->mcode2.as<uint16_t>();
   h[1].mxdb = P->mxdb2.as<cd>();
   for (auto &a : h) a.map_in_pass_x = nb >= MAP_IN_PASS_X_BATCH;
-  if (!P->aux) {
-    hip_check(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipStreamCreateWithFlags(&P->aux2, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming), "event");
-    hip_check(hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming), "event");
-  }
+  make_aux_streams(P);
   const hipStream_t s[2] = {P->stream, P->aux};
   hip_check(hipEventRecord(P->fork_ev, P->stream), "fork");
   hip_check(hipStreamWaitEvent(P->aux, P->fork_ev, 0), "fork");
@@ -999,13 +1013,7 @@ void run_cn(pxg_plan P, unsigned first, unsigned nsteps)
   const int sets = (P->njobs_glob > 0) + (P->njobs_big > 0) + (P->njobs_reg > 0);
   hipStream_t gs = P->stream, bs = P->stream;
   if (sets > 1) {
-    if (!P->aux) {
-      hip_check(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking), "hipStreamCreate");
-      hip_check(hipStreamCreateWithFlags(&P->aux2, hipStreamNonBlocking), "hipStreamCreate");
-      hip_check(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming), "event");
-      hip_check(hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming), "event");
-      hip_check(hipEventCreateWithFlags(&P->join2_ev, hipEventDisableTiming), "event");
-    }
+    make_aux_streams(P);
     hip_check(hipEventRecord(P->fork_ev, P->stream), "fork");
     if (P->njobs_glob) {
       hip_check(hipStreamWaitEvent(P->aux, P->fork_ev, 0), "fork");
