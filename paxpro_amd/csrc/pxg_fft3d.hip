@@ -478,17 +478,27 @@ __global__ __launch_bounds__(64) void k_material_map(FftArgs<R> a, int n)
 }
 
 // The maps of nz consecutive steps in one launch (material_map_planes), with the
-// strips of k_material_map.
+// strips of k_material_map.  Work-groups along z take ppw consecutive steps each
+// (small grids: more work-groups, shorter chains); with more than one group every
+// group classifies its first z0 plane itself, at the coordinate the group before
+// used for its last z1 plane, so the maps are the same bits (and no group reads
+// a previous map that another group of the launch overwrites).
 template <class R, int CPL>
-__global__ __launch_bounds__(64) void k_material_map_planes(FftArgs<R> a, int n, int nz)
+__global__ __launch_bounds__(64) void k_material_map_planes(FftArgs<R> a, int n, int nz, int ppw)
 {
   __shared__ uint64_t mask[MAX_MASK_WORDS];
   __shared__ __align__(16) char scr[map_scratch_bytes<CPL>(64)];
   const int lanes = static_cast<int>(blockDim.x);
   const int c = blockIdx.y, r0 = blockIdx.x * lanes * CPL;
   const int r = r0 + static_cast<int>(threadIdx.x);
+  const int k0 = static_cast<int>(blockIdx.z) * ppw;
+  const size_t off = static_cast<size_t>(k0) * n * n;
+  a.step += static_cast<unsigned>(k0);
+  a.mcode += off;
+  a.mxdb += off;
+  if (gridDim.z > 1) a.map_prev_valid = 0;
   material_map_planes<CPL>(a, mask, n, c, c, r0, r0 + lanes * CPL - 1,
-                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes, nz);
+                           [&](int m) { return make_int2(c, r + m * lanes); }, scr, lanes, min(ppw, nz - k0));
 }
 
 // The same for the members of a group, blockIdx.z = member (members without a
@@ -758,11 +768,17 @@ hipError_t launch_material_map_planes(int n, const FftArgs<R> &a, int nz, hipStr
 {
   // the strips of launch_material_map (power-of-two n only)
   if (!fft_size_supported(n) || nz < 1 || nz > MAP_RING) return hipErrorInvalidValue;
+  // steps per work-group: all nz when the strips alone give 16384 work-groups
+  // (one wavefront each), fewer on smaller grids
+  const int strips = n >= 128 ? (n / 128) * n : (n / (n < 64 ? n : 64)) * n;
+  const int groups = std::max(1, std::min(nz, 16384 / strips));
+  const int ppw = (nz + groups - 1) / groups;
+  const unsigned gz = static_cast<unsigned>((nz + ppw - 1) / ppw);
   if (n >= 128) {
-    hipLaunchKernelGGL((k_material_map_planes<R, 2>), dim3(n / 128, n), dim3(64), 0, s, a, n, nz);
+    hipLaunchKernelGGL((k_material_map_planes<R, 2>), dim3(n / 128, n, gz), dim3(64), 0, s, a, n, nz, ppw);
   } else {
     const int rows = n < 64 ? n : 64;
-    hipLaunchKernelGGL((k_material_map_planes<R, 1>), dim3(n / rows, n), dim3(rows), 0, s, a, n, nz);
+    hipLaunchKernelGGL((k_material_map_planes<R, 1>), dim3(n / rows, n, gz), dim3(rows), 0, s, a, n, nz, ppw);
   }
   return hipGetLastError();
 }
