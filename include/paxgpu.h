@@ -155,12 +155,15 @@ int pxg_plan_get_lines(pxg_plan plan, int *lines_x, int *lines_y);
 int pxg_plan_set_adi_tiles(pxg_plan plan, int x_tile, int y_tile);
 int pxg_plan_get_adi_tiles(pxg_plan plan, int *x_tile, int *y_tile);
 /* Autotuner, in the manner of FFTW's measured planning: times every supported
- * shape of both FFT3D passes (lines per work-group), or every tile shape of both
- * ADI half-steps, on this plan with its current material and keeps the fastest;
- * the choice is cached in the context for plans of the same kind, nx, batch,
- * precision and material kind.  Overwrites (zeroes) the field: call it after the
- * material and before pxg_set_field.  reps <= 0 means 5 timed launches per shape.
- * A no-op for the other kinds. */
+ * shape of both FFT3D passes (lines per work-group) and, for batches that can
+ * march as two halves on two streams (16 or more fields), steps of the march both
+ * ways; or every tile shape of both ADI half-steps; on this plan with its current
+ * material, and keeps the fastest.  Host-sampled planes that change with z are
+ * timed at the first steps they cover (none covered: nothing is timed).  The
+ * choice is cached in the context for plans of the same kind, nx, batch,
+ * precision and material kind, and no choice changes a result.  Overwrites
+ * (zeroes) the field: call it after the material and before pxg_set_field.
+ * reps <= 0 means 5 timed launches per shape.  A no-op for the other kinds. */
 int pxg_plan_autotune(pxg_plan plan, int reps);
 /* FFT3D z-step loops as HIP graphs: with enable != 0, pxg_step captures the
  * kernels of a call into a graph on first use and replays it when a later call has

@@ -119,8 +119,9 @@ struct pxg_ctx_s {
   // live plans; a destroyed context is freed when its last plan is destroyed
   int plans = 0;
   bool closed = false;
-  // autotuned FFT3D work-group shapes: (nx, batch, precision, material) -> (lines_x, lines_y)
-  std::map<std::array<int, 4>, std::pair<int, int>> tuned;
+  // autotuned FFT3D shapes: (nx, batch, precision, material) -> (lines_x, lines_y,
+  // batch marched in one launch per pass instead of two halves)
+  std::map<std::array<int, 4>, std::array<int, 3>> tuned;
   // autotuned ADI tile shapes: (nx, batch, precision, material) -> (x tile, y tile)
   std::map<std::array<int, 4>, std::pair<int, int>> tuned_adi;
 };
@@ -134,6 +135,7 @@ struct pxg_plan_s {
   size_t cells = 0;       // per field
   size_t reserved = 0;
   int lines_x = 1, lines_y = 1;
+  bool one_launch = false;  // a batch that could march in halves runs one launch per pass (autotuned)
   DBuf field, tw, kx, ky, absx, absy, kvec;
   // FFT kinds with nx not a power of two in [16, 4096]: Bluestein length and tables
   int blue_m = 0;
@@ -655,7 +657,7 @@ void run_fft3d(pxg_plan P, unsigned first, unsigned nsteps)
   const pxg_problem &p = P->p;
   // batches in halves, except under profiling (one launch per pass is what it
   // times) and graph capture (one stream)
-  if (fft3d_halves(p) && !P->profiling && !P->use_graphs && P->mcode2.p) {
+  if (fft3d_halves(p) && !P->one_launch && !P->profiling && !P->use_graphs && P->mcode2.p) {
     run_fft3d_halves<R>(P, first, nsteps);
     return;
   }
@@ -775,12 +777,25 @@ bool lines_supported(int n, int l)
 
 // Mean duration of `reps` launches of one FFT3D pass with `lines` lines per
 // work-group, on the plan's own field and material (mid-run z-step).
+// First of `span` steps at which the autotuner runs the plan: the middle of its
+// z range, or for host-sampled material planes that change with z the first
+// steps they cover (-1: they cover fewer than span steps; the plan is not timed).
+long tune_step(pxg_plan P, unsigned span)
+{
+  const long mid = std::max(1, P->p.nz / 2);
+  if (P->mat_mode != MAT_PLANES || P->z_invariant) return mid;
+  const long per = P->p.kind == PXG_ADI ? 2 : 1;  // planes per step (ADI: one per half-step)
+  const long st = std::max<long>(1, (P->first_plane + per - 1) / per);
+  if ((st + static_cast<long>(span)) * per - 1 >= static_cast<long>(P->first_plane) + P->n_planes) return -1;
+  return st;
+}
+
 template <class R>
 double time_pass(pxg_plan P, bool pass_x, int lines, int reps)
 {
   const pxg_problem &p = P->p;
   FftArgs<R> a = fft_args<R>(P);
-  a.step = std::max(1, p.nz / 2);
+  a.step = static_cast<unsigned>(tune_step(P, 1));
   a.mode = Y_INV | Y_FWD;
   auto launch = [&] {
     return pass_x ? launch_pass_x<R>(p.nx, lines, a, p.batch, P->stream)
@@ -824,6 +839,45 @@ void autotune_fft3d(pxg_plan P, int reps)
   }
   P->lines_x = lx;
   P->lines_y = ly;
+}
+
+// Batches that can march in two halves on two streams (run_fft3d_halves): time
+// steps of the march both ways, each in two rounds keeping its faster one, and
+// keep the faster way.  The results are the same bits either way.
+template <class R>
+void autotune_halves(pxg_plan P, int reps)
+{
+  P->one_launch = false;
+  if (!fft3d_halves(P->p) || !P->mcode2.p || P->profiling || P->use_graphs) return;
+  const unsigned steps = static_cast<unsigned>(std::max(2, reps));
+  const long at = tune_step(P, steps);
+  if (at < 0) return;
+  const unsigned first = static_cast<unsigned>(at);
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  double best[2] = {1e30, 1e30};
+  try {
+    for (int round = 0; round < 2; ++round)
+      for (int way = 0; way < 2; ++way) {
+        P->one_launch = way == 1;
+        run_fft3d<R>(P, first, 1);  // warm: the way's streams and maps
+        hip_check(hipEventRecord(e0, P->stream), "event");
+        run_fft3d<R>(P, first, steps);
+        hip_check(hipEventRecord(e1, P->stream), "event");
+        hip_check(hipEventSynchronize(e1), "event");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+        best[way] = std::min(best[way], static_cast<double>(ms));
+      }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P->one_launch = best[1] < best[0];
 }
 
 template <class R>
@@ -877,7 +931,7 @@ template <class R>
 double time_adi_half(pxg_plan P, bool implicit_x, int tile, int reps)
 {
   TriArgs<R> a = tri_args<R>(P);
-  a.first_step = std::max(1, P->p.nz / 2);
+  a.first_step = static_cast<unsigned>(tune_step(P, 1));
   a.adi_tile = tile;
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -1925,6 +1979,7 @@ int pxg_plan_autotune(pxg_plan P, int reps)
 {
   return guarded([&] {
     check_plan(P);
+    if ((P->p.kind == PXG_ADI || P->p.kind == PXG_FFT3D) && tune_step(P, 1) < 0) return;  // planes cover no step
     if (P->p.kind == PXG_ADI) {
       const std::array<int, 4> key = {P->p.nx, P->p.batch, P->p.precision, P->mat_mode};
       {
@@ -1955,24 +2010,28 @@ int pxg_plan_autotune(pxg_plan P, int reps)
       std::lock_guard<std::mutex> lk(P->ctx->mu);
       auto it = P->ctx->tuned.find(key);
       if (it != P->ctx->tuned.end()) {
-        P->lines_x = it->second.first;
-        P->lines_y = it->second.second;
+        P->lines_x = it->second[0];
+        P->lines_y = it->second[1];
+        P->one_launch = it->second[2] != 0;
         return;
       }
     }
     if (reps <= 0) reps = 5;
     hip_check(hipStreamSynchronize(P->stream), "sync");
-    if (P->mixed)
+    if (P->mixed) {
       autotune_fft3d<float>(P, reps);
-    else
+      autotune_halves<float>(P, reps);
+    } else {
       autotune_fft3d<double>(P, reps);
+      autotune_halves<double>(P, reps);
+    }
     hip_check(hipMemsetAsync(P->field.p, 0, P->field.bytes, P->stream), "memset");
     hip_check(hipStreamSynchronize(P->stream), "sync");
     P->field_set = false;
     P->map_step = -1;  // the timing runs wrote maps of another step
     P->map_step2 = -1;
     std::lock_guard<std::mutex> lk(P->ctx->mu);
-    P->ctx->tuned[key] = {P->lines_x, P->lines_y};
+    P->ctx->tuned[key] = {P->lines_x, P->lines_y, P->one_launch ? 1 : 0};
   });
 }
 

@@ -223,6 +223,52 @@ def test_fft3d_work_group_shapes_bitwise_and_autotune(ctx, prec):
     plan2.close()
 
 
+def test_fft3d_autotune_batch_halves_and_plane_range(ctx):
+    """The autotuner also times a batch of 16 fields marched in halves on two
+    streams against one launch per pass and keeps the faster; either way the
+    fields are the same bits as an untuned plan's.  For host-sampled material
+    planes that change with z it times at the first steps the planes cover
+    (here steps 5..7 of a 41-step plan, not the middle of the z range), and the
+    march afterwards equals an untuned plan's."""
+    n, nsteps = 64, 5
+    announce("fft3d autotune halves and plane range", n)
+    g = fft3d_case(n, dz=200.0, nsteps=40)
+    sph = random_spheres(0x5EED0007, 6, (-6000, 6000), (-1000, 1000), (800, 2500), (1e-6, 1e-5))
+    csg = spheres_scene(sph)
+    u0 = gaussian_field(g, n, 5000.0)
+    prob = fft3d_problem(g, n, 41, precision=P.PREC_MIXED, batch=16)
+    outs = []
+    for tune in (False, True):
+        plan = P.Plan(ctx, prob)
+        plan.material_csg(*csg)
+        if tune:
+            plan.autotune(2)
+        for b in range(16):
+            plan.set_field(u0 * (1.0 + 0.01 * b), b)
+        plan.step(1, nsteps)
+        outs.append([plan.get_field(b) for b in (0, 9, 15)])
+        plan.close()
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+
+    rng = np.random.default_rng(7)
+    db = np.zeros((3, n * n, 2))
+    db[:, :, 0] = 4e-6 * rng.random((3, n * n))
+    db[:, :, 1] = 1e-7 * rng.random((3, n * n))
+    prob1 = fft3d_problem(g, n, 41, precision=P.PREC_FP64)
+    fields = []
+    for tune in (False, True):
+        plan = P.Plan(ctx, prob1)
+        plan.material_planes(db, first_plane=5)
+        if tune:
+            plan.autotune(2)
+        plan.set_field(u0)
+        plan.step(5, 3)
+        fields.append(plan.get_field())
+        plan.close()
+    assert np.array_equal(fields[0], fields[1])
+
+
 def test_context_outlived_by_its_plans():
     """A context destroyed while a plan of it lives stays usable by that plan
     and is freed with it; no new plan can be made from it."""
