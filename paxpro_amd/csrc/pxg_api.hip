@@ -772,7 +772,7 @@ void run_fft3d_group(pxg_plan *pl, int m, const unsigned *first, unsigned nsteps
 bool lines_supported(int n, int l)
 {
   return fft_size_supported(n) && l >= 1 && l <= 16 && (l & (l - 1)) == 0 && l * (n / 8) <= 1024 && n % l == 0 &&
-         static_cast<size_t>(l) * (n + n / 8 + 1) * sizeof(cd) <= 163840 - 1024;
+         static_cast<size_t>(l) * (n + n / 8 + (l >= 8 ? 1 : 8 / l)) * sizeof(cd) <= 163840 - 1024;  // pass X's stride
 }
 
 // Mean duration of `reps` launches of one FFT3D pass with `lines` lines per

@@ -542,6 +542,17 @@ __global__ __launch_bounds__(64) void k_material_map_clamped(FftArgs<R> a, int n
 // (mapping B) and the last exchange of IFFT_x moves it back: the exchanges in
 // between need no work-group barrier.  With a CSG material the field-0
 // work-groups then build the material map of the step for their rows.
+// LDS stride between the lines of a pass X work-group.  In mapping A the lanes
+// hold the L lines side by side, so eight consecutive lanes (the 128 bytes of
+// 16-byte accesses one LDS cycle serves) touch L lines at 8/L consecutive
+// canonical positions each; a stride of 8/L modulo 8 puts those eight accesses
+// in eight different bank groups (an odd stride does for L >= 8).
+template <int N, int L>
+constexpr int pass_x_stride()
+{
+  return LdsLine<N>::SIZE + (L >= 8 ? 1 : 8 / L);
+}
+
 template <int N, class R, int L>
 __device__ __forceinline__ void pass_x_body(const FftArgs<R> &a, int b, cd *lds)
 {
@@ -557,13 +568,13 @@ __device__ __forceinline__ void pass_x_body(const FftArgs<R> &a, int b, cd *lds)
   const int g = (G % 8 == 0) ? (bx % 8) * (G / 8) + bx / 8 : bx;
   const int ra = g * L + la;
   cx<R> *f = a.field + b * a.fstride + ra + static_cast<size_t>(ta) * N;
-  cd *lla = lds + la * LdsLine<N>::STRIDE;
+  cd *lla = lds + la * pass_x_stride<N, L>();
   cd v[8];
   load8(v, f, static_cast<size_t>(T) * N);
   const cd *kx = a.kx + b * N;
   if constexpr (T <= 64) {
     const int lb = tid / T, tb = tid % T;  // mapping B
-    cd *llb = lds + lb * LdsLine<N>::STRIDE;
+    cd *llb = lds + lb * pass_x_stride<N, L>();
     const TwBase<N, double> twa = tw_base<N>(a.tw, ta), twb = tw_base<N>(a.tw, tb);
     fft_line_remap_in<N, -1>(v, lla, llb, ta, tb, twb);
     const cd kyr = a.ky[b * N + g * L + lb];
@@ -585,7 +596,7 @@ __device__ __forceinline__ void pass_x_body(const FftArgs<R> &a, int b, cd *lds)
     // the FFT's LDS is free (the culling's barriers follow every lane's last
     // use of it); each wavefront takes a slice for its border cells
     constexpr int WL = L * T < 64 ? L * T : 64;
-    static_assert((L * T / WL) * map_scratch_bytes<8>(WL) <= L * LdsLine<N>::STRIDE * static_cast<int>(sizeof(cd)),
+    static_assert((L * T / WL) * map_scratch_bytes<8>(WL) <= L * pass_x_stride<N, L>() * static_cast<int>(sizeof(cd)),
                   "map scratch must fit the FFT's LDS");
     char *wscr = reinterpret_cast<char *>(lds) + (tid / WL) * map_scratch_bytes<8>(WL);
     material_map_cells<8>(a, mask, N, 0, N - 1, r0, r0 + L - 1,
@@ -597,7 +608,7 @@ __device__ __forceinline__ void pass_x_body(const FftArgs<R> &a, int b, cd *lds)
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 {
-  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  __shared__ cd lds[L * pass_x_stride<N, L>()];
   pass_x_body<N, R, L>(a, static_cast<int>(blockIdx.y), lds);
 }
 
@@ -605,7 +616,7 @@ __global__ __launch_bounds__(L *N / 8) void k_pass_x(FftArgs<R> a)
 template <int N, class R, int L>
 __global__ __launch_bounds__(L *N / 8) void k_pass_x_group(GroupArgs<R> g)
 {
-  __shared__ cd lds[L * LdsLine<N>::STRIDE];
+  __shared__ cd lds[L * pass_x_stride<N, L>()];
   const FftArgs<R> a = member_args(g, static_cast<int>(blockIdx.y));
   pass_x_body<N, R, L>(a, 0, lds);
 }
